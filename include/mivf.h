@@ -1,0 +1,155 @@
+/*
+ * mivf.h -- C ABI of libmivf, the MI355X (gfx950) IVF-PQ search engine.
+ *
+ * Plain C types only: pointers, sizes and status codes.  Every function
+ * returns MIVF_OK (0) or a non-zero status; mivf_last_error() gives the
+ * message of the last failure on the calling thread.  Pointers named d_* are
+ * device (HBM) pointers; all others are host pointers.  `stream` is a
+ * hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Each entry point cites the reference interface it replaces (paths are in the
+ * reference repository rafaelms101/mestrado).
+ *
+ * Threading: calls on one index handle must be serialised by the caller (the
+ * handle owns a scratch workspace); different handles are independent.
+ */
+#ifndef MIVF_H
+#define MIVF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MIVF_OK = 0,
+    MIVF_EINVAL = 1,      /* bad argument or shape                  */
+    MIVF_EHIP = 2,        /* HIP runtime failure                    */
+    MIVF_ENOMEM = 3,      /* allocation failure / over budget       */
+    MIVF_EIO = 4,         /* file missing, truncated or corrupt     */
+    MIVF_EUNSUPPORTED = 5 /* shape outside the compiled kernel set  */
+};
+
+typedef struct mivf_index mivf_index;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *mivf_last_error(void);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int mivf_version(void);
+
+/* ---- device memory helpers (for callers without a GPU framework) -------- */
+int mivf_device_count(int *count);
+int mivf_malloc(void **d_ptr, size_t bytes);
+int mivf_free(void *d_ptr);
+int mivf_memcpy_htod(void *d_dst, const void *src, size_t bytes);
+int mivf_memcpy_dtoh(void *dst, const void *d_src, size_t bytes);
+int mivf_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes);
+int mivf_synchronize(void);
+
+/* ---- training ------------------------------------------------------------
+ * Replaces ivfpq_new(coarsek, nsq, vtrain, threads)
+ * (parallel/ivf_pq/ivf_training.cpp:60-90; sequential/ivf_pq/ivf_new.cpp:4):
+ * coarse k-means with nlist centroids, residuals, then one k-means with ks
+ * centroids per subquantizer.  Assignment steps run on the GPU with the
+ * reference's distance arithmetic; centroid updates are double-precision
+ * sums in point order (deterministic).
+ * x: host [n][d] float32.  coarse_out: [nlist][d]; pq_out: [m][ks][d/m].   */
+int mivf_train(const float *x, int64_t n, int d, int nlist, int m, int ks,
+               int niter_coarse, int niter_pq, uint64_t seed,
+               float *coarse_out, float *pq_out);
+/* same with x already in HBM */
+int mivf_train_device(const float *d_x, int64_t n, int d, int nlist, int m, int ks,
+                      int niter_coarse, int niter_pq, uint64_t seed,
+                      float *coarse_out, float *pq_out);
+/* One k-means (the building block of mivf_train); x host [n][ldx]. */
+int mivf_kmeans(const float *x, int64_t n, int d, int ldx, int k, int niter,
+                uint64_t seed, float *cent_out);
+
+/* ---- index ----------------------------------------------------------------
+ * Replaces preallocate_gpu_mem(pqtipo, ivf_t*, int)
+ * (parallel2/ivf_pq/mycuda.h:8): the quantizers and all lists become
+ * device-resident and stay so until mivf_index_free
+ * (deallocate_gpu_mem, parallel2/ivf_pq/mycuda.h:9).
+ * coarse: host [nlist][d]; pq: host [m][ks][d/m]; ks <= 256 (8-bit codes).  */
+int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse,
+                      const float *pq, mivf_index **out);
+int mivf_index_free(mivf_index *ix);
+int mivf_index_info(const mivf_index *ix, int *d, int *nlist, int *m, int *ks,
+                    int64_t *ntotal);
+/* Replace all lists.  offsets [nlist+1] (vector offsets), ids [ntotal],
+ * codes [ntotal][m] uint8, list-major.  Host arrays.                       */
+int mivf_index_set_lists(mivf_index *ix, const int64_t *offsets, const int32_t *ids,
+                         const uint8_t *codes);
+/* Same with the reference's int32-per-subquantizer codes (ivf_t.codes,
+ * parallel2/ivf_pq/myIVF.h:29-33); every code must be < ks.                */
+int mivf_index_set_lists_i32(mivf_index *ix, const int64_t *offsets, const int32_t *ids,
+                             const int32_t *codes);
+/* Copy the lists back to the host (same layout as set_lists).             */
+int mivf_index_get_lists(mivf_index *ix, int64_t *offsets, int32_t *ids, uint8_t *codes);
+/* Replaces ivfpq_assign(ivfpq, vbase, ivf) (parallel/ivf_pq/ivf_training.cpp:165-224
+ * and parallel/pq-utils/pq_assign.cpp:12-58): coarse argmin, residual and
+ * per-subquantizer argmin on the GPU, then a stable append to the lists
+ * (ids id0, id0+1, ... in input order; ids ascending within a list).
+ * mivf_index_add takes host x, mivf_index_add_device takes d_x.            */
+int mivf_index_add(mivf_index *ix, const float *x, int64_t n, int32_t id0);
+int mivf_index_add_device(mivf_index *ix, const float *d_x, int64_t n, int32_t id0);
+/* Encode only: d_assign [n] int32, d_codes [n][m] uint8 (device).          */
+int mivf_index_encode(mivf_index *ix, const float *d_x, int64_t n, int32_t *d_assign,
+                      uint8_t *d_codes, void *stream);
+
+/* ---- search ----------------------------------------------------------------
+ * Plain exact form: ids and FP32 distances bit-identical to the reference
+ * arithmetic; results sorted by (distance, id); missing results are padded
+ * with id -1 and distance +inf.  Replaces the sequential ivfpq_search
+ * without rerank and the per-rank search loop of parallel_search
+ * (parallel/ivf_pq/ivf_search.cpp:97-137).  d_q [nq][d]; outputs [nq][k].   */
+int mivf_search(mivf_index *ix, const float *d_q, int nq, int w, int k,
+                int32_t *d_ids, float *d_dis, void *stream);
+/* Per-query w and k (host arrays w[nq], k[nq]); output rows kmax wide.     */
+int mivf_search_var(mivf_index *ix, const float *d_q, int nq, const int32_t *w,
+                    const int32_t *k, int32_t *d_ids, float *d_dis, int kmax,
+                    void *stream);
+/* Coarse assignment only: top-w lists of each query, (dist, id) order, as
+ * knn_full(2, nq, nlist, d, w, coa, q) in parallel/ivf_pq/ivf_assign.cpp:28. */
+int mivf_coarse(mivf_index *ix, const float *d_q, int nq, int w, int32_t *d_idx,
+                float *d_dis, void *stream);
+/* Residual-input search with core_gpu semantics
+ * (parallel2/ivf_pq/mycuda.cu:119-188): d_residual [nq*w][d] and
+ * d_list_of [nq*w] (rid_to_ivf); outputs [nq][k].                          */
+int mivf_search_residual(mivf_index *ix, const float *d_residual,
+                         const int32_t *d_list_of, int nq, int w, int k,
+                         int32_t *d_ids, float *d_dis, void *stream);
+/* RERANK path (sequential/ivf_pq/ivf_search.cpp:98-134): ADC shortlist of kl,
+ * then exact FP32 distances to the raw base rows d_base [nbase][d], best k
+ * by (distance, id).  Distances are returned too.                          */
+int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, int k,
+                       const float *d_base, int64_t nbase, int32_t *d_ids,
+                       float *d_dis, void *stream);
+
+/* ---- options and timing ---------------------------------------------------
+ * "probes_per_group" (0 = automatic), "profile" (1 = record per-stage HIP
+ * event timings of the next searches).                                     */
+int mivf_set_option(mivf_index *ix, const char *name, int64_t value);
+/* Per-stage milliseconds of the last profiled search:
+ * [0] coarse distances, [1] probe select, [2] scan, [3] merge, [4] rerank. */
+int mivf_get_timings(mivf_index *ix, float *ms, int n);
+
+/* ---- synthetic data and ground truth ------------------------------------ */
+/* Seeded Gaussian mixture rows; bit-identical to the CPU generator.
+ * d_centres [ncomp][d]; d_thr NULL (uniform components) or [ncomp] Zipf
+ * thresholds from mivf_zipf_thresholds.                                     */
+int mivf_synth_rows(uint64_t seed, uint64_t stream_id, int64_t row0, int64_t n, int d,
+                    const float *d_centres, int ncomp, const uint64_t *d_thr,
+                    float sigma, float *d_out, void *stream);
+int mivf_synth_centres(uint64_t seed, int ncomp, int d, float *centres);
+int mivf_zipf_thresholds(uint64_t seed, int ncomp, double s, uint64_t *thr);
+/* Exact FP64 nearest neighbour of each query (ties -> lowest id).          */
+int mivf_bruteforce_nn(const float *d_base, int64_t nb, const float *d_q, int nq, int d,
+                       int32_t *d_ids, double *d_dis);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MIVF_H */

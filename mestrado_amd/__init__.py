@@ -1,0 +1,336 @@
+"""mestrado_amd -- MI355X (gfx950) IVF-PQ search engine.
+
+Python access to libmivf.so, the HIP/C++ library behind the C ABI in
+include/mivf.h.  The library is the product; this module only loads it with
+ctypes and moves numpy arrays in and out of device memory.  Nothing here
+computes search results: when the library is missing, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["lib", "MivfError", "DeviceArray", "Index", "train", "kmeans",
+           "synth_centres", "zipf_thresholds", "synth_rows", "bruteforce_nn",
+           "device_count", "synchronize", "LIB_PATH"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmivf.so")
+_lib = None
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_u64 = ctypes.c_uint64
+
+# name -> argtypes (all return int status)
+_SIGS = {
+    "mivf_version": [],
+    "mivf_device_count": [_i32p],
+    "mivf_malloc": [ctypes.POINTER(_vp), ctypes.c_size_t],
+    "mivf_free": [_vp],
+    "mivf_memcpy_htod": [_vp, _vp, ctypes.c_size_t],
+    "mivf_memcpy_dtoh": [_vp, _vp, ctypes.c_size_t],
+    "mivf_memcpy_dtod": [_vp, _vp, ctypes.c_size_t],
+    "mivf_synchronize": [],
+    "mivf_train": [_vp, _i64, _i, _i, _i, _i, _i, _i, _u64, _vp, _vp],
+    "mivf_train_device": [_vp, _i64, _i, _i, _i, _i, _i, _i, _u64, _vp, _vp],
+    "mivf_kmeans": [_vp, _i64, _i, _i, _i, _i, _u64, _vp],
+    "mivf_index_create": [_i, _i, _i, _i, _vp, _vp, ctypes.POINTER(_vp)],
+    "mivf_index_free": [_vp],
+    "mivf_index_info": [_vp, _i32p, _i32p, _i32p, _i32p, _i64p],
+    "mivf_index_set_lists": [_vp, _vp, _vp, _vp],
+    "mivf_index_set_lists_i32": [_vp, _vp, _vp, _vp],
+    "mivf_index_get_lists": [_vp, _vp, _vp, _vp],
+    "mivf_index_add": [_vp, _vp, _i64, ctypes.c_int32],
+    "mivf_index_add_device": [_vp, _vp, _i64, ctypes.c_int32],
+    "mivf_index_encode": [_vp, _vp, _i64, _vp, _vp, _vp],
+    "mivf_search": [_vp, _vp, _i, _i, _i, _vp, _vp, _vp],
+    "mivf_search_var": [_vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp],
+    "mivf_coarse": [_vp, _vp, _i, _i, _vp, _vp, _vp],
+    "mivf_search_residual": [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp],
+    "mivf_search_rerank": [_vp, _vp, _i, _i, _i, _i, _vp, _i64, _vp, _vp, _vp],
+    "mivf_set_option": [_vp, ctypes.c_char_p, _i64],
+    "mivf_get_timings": [_vp, _f32p, _i],
+    "mivf_synth_rows": [_u64, _u64, _i64, _i64, _i, _vp, _i, _vp, ctypes.c_float, _vp, _vp],
+    "mivf_synth_centres": [_u64, _i, _i, _vp],
+    "mivf_zipf_thresholds": [_u64, _i, ctypes.c_double, _vp],
+    "mivf_bruteforce_nn": [_vp, _i64, _vp, _i, _i, _vp, _vp],
+}
+
+
+class MivfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libmivf error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libmivf.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m mestrado_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.mivf_last_error.argtypes = []
+        L.mivf_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MivfError(rc, lib().mivf_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def device_count():
+    n = ctypes.c_int32(0)
+    _check(lib().mivf_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def synchronize():
+    _check(lib().mivf_synchronize())
+
+
+class DeviceArray:
+    """A device (HBM) buffer with a numpy dtype and shape."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().mivf_malloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if d.nbytes:
+            _check(lib().mivf_memcpy_htod(d.ptr, _ptr(a), d.nbytes))
+        return d
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            _check(lib().mivf_memcpy_dtoh(_ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def offset(self, nbytes):
+        return ctypes.c_void_p(self.ptr.value + int(nbytes))
+
+    def free(self):
+        if self.ptr is not None and _lib is not None:
+            lib().mivf_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _dev(x, dtype):
+    """Device pointer for x (DeviceArray or numpy array copied to the device)."""
+    if isinstance(x, DeviceArray):
+        return x, None
+    d = DeviceArray.from_numpy(np.ascontiguousarray(x, dtype=dtype))
+    return d, d
+
+
+def train(x, nlist, m, ks=256, niter_coarse=20, niter_pq=25, seed=1234):
+    """ivfpq_new: returns (coarse [nlist][d], pq [m][ks][d/m]) float32."""
+    if isinstance(x, DeviceArray):
+        n, d = x.shape
+        fn, ptr = lib().mivf_train_device, x.ptr
+    else:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n, d = x.shape
+        fn, ptr = lib().mivf_train, _ptr(x)
+    coarse = np.empty((nlist, d), np.float32)
+    pq = np.empty((m, ks, d // m), np.float32)
+    _check(fn(ptr, n, d, nlist, m, ks, niter_coarse, niter_pq, seed, _ptr(coarse), _ptr(pq)))
+    return coarse, pq
+
+
+def kmeans(x, k, niter, seed, ldx=None):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n, d = x.shape
+    cent = np.empty((k, d), np.float32)
+    _check(lib().mivf_kmeans(_ptr(x), n, d, ldx or d, k, niter, seed, _ptr(cent)))
+    return cent
+
+
+def synth_centres(seed, ncomp, d):
+    c = np.empty((ncomp, d), np.float32)
+    _check(lib().mivf_synth_centres(seed, ncomp, d, _ptr(c)))
+    return c
+
+
+def zipf_thresholds(seed, ncomp, s):
+    t = np.empty(ncomp, np.uint64)
+    _check(lib().mivf_zipf_thresholds(seed, ncomp, s, _ptr(t)))
+    return t
+
+
+def synth_rows(seed, stream, row0, n, centres, thr=None, sigma=20.0, out=None):
+    """Mixture rows generated on the device; returns a DeviceArray [n][d]."""
+    cd, _ = _dev(centres, np.float32)
+    ncomp, d = cd.shape
+    td = None
+    if thr is not None:
+        td, _ = _dev(thr, np.uint64)
+    if out is None:
+        out = DeviceArray((n, d), np.float32)
+    _check(lib().mivf_synth_rows(seed, stream, row0, n, d, cd.ptr, ncomp,
+                                 td.ptr if td is not None else None, sigma, out.ptr, None))
+    synchronize()
+    return out
+
+
+def bruteforce_nn(base, q):
+    """Exact FP64 nearest neighbour: returns (ids [nq] int32, dist [nq] float64)."""
+    bd, _ = _dev(base, np.float32)
+    qd, _ = _dev(q, np.float32)
+    nb, d = bd.shape
+    nq = qd.shape[0]
+    ids = DeviceArray((nq,), np.int32)
+    dis = DeviceArray((nq,), np.float64)
+    _check(lib().mivf_bruteforce_nn(bd.ptr, nb, qd.ptr, nq, d, ids.ptr, dis.ptr))
+    return ids.numpy(), dis.numpy()
+
+
+class Index:
+    """Device-resident IVF-PQ index (coarse + PQ quantizers and inverted lists)."""
+
+    def __init__(self, coarse, pq):
+        coarse = np.ascontiguousarray(coarse, dtype=np.float32)
+        pq = np.ascontiguousarray(pq, dtype=np.float32)
+        self.nlist, self.d = coarse.shape
+        self.m, self.ks, self.ds = pq.shape
+        h = ctypes.c_void_p()
+        _check(lib().mivf_index_create(self.d, self.nlist, self.m, self.ks, _ptr(coarse), _ptr(pq),
+                                       ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            lib().mivf_index_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ntotal(self):
+        n = ctypes.c_int64(0)
+        _check(lib().mivf_index_info(self.h, None, None, None, None, ctypes.byref(n)))
+        return n.value
+
+    def add(self, x, id0=0):
+        if isinstance(x, DeviceArray):
+            _check(lib().mivf_index_add_device(self.h, x.ptr, x.shape[0], id0))
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float32)
+            _check(lib().mivf_index_add(self.h, _ptr(x), x.shape[0], id0))
+
+    def encode(self, x):
+        xd, _ = _dev(x, np.float32)
+        n = xd.shape[0]
+        a = DeviceArray((n,), np.int32)
+        c = DeviceArray((n, self.m), np.uint8)
+        _check(lib().mivf_index_encode(self.h, xd.ptr, n, a.ptr, c.ptr, None))
+        return a.numpy(), c.numpy()
+
+    def set_lists(self, offsets, ids, codes):
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        if np.asarray(codes).dtype == np.int32:
+            codes = np.ascontiguousarray(codes, dtype=np.int32)
+            _check(lib().mivf_index_set_lists_i32(self.h, _ptr(offsets), _ptr(ids), _ptr(codes)))
+        else:
+            codes = np.ascontiguousarray(codes, dtype=np.uint8)
+            _check(lib().mivf_index_set_lists(self.h, _ptr(offsets), _ptr(ids), _ptr(codes)))
+
+    def get_lists(self):
+        n = self.ntotal
+        off = np.empty(self.nlist + 1, np.int64)
+        ids = np.empty(max(n, 1), np.int32)
+        codes = np.empty((max(n, 1), self.m), np.uint8)
+        _check(lib().mivf_index_get_lists(self.h, _ptr(off), _ptr(ids), _ptr(codes)))
+        return off, ids[:n], codes[:n]
+
+    def set_option(self, name, value):
+        _check(lib().mivf_set_option(self.h, name.encode(), int(value)))
+
+    def timings(self):
+        t = np.zeros(8, np.float32)
+        _check(lib().mivf_get_timings(self.h, t.ctypes.data_as(_f32p), 8))
+        return t
+
+    # -- search ------------------------------------------------------------
+    def search(self, q, w, k, out=None, stream=None):
+        qd, _ = _dev(q, np.float32)
+        nq = qd.shape[0]
+        ids, dis = out if out is not None else (DeviceArray((nq, k), np.int32), DeviceArray((nq, k), np.float32))
+        _check(lib().mivf_search(self.h, qd.ptr, nq, w, k, ids.ptr, dis.ptr, stream))
+        if out is not None:
+            return ids, dis
+        return ids.numpy(), dis.numpy()
+
+    def search_var(self, q, w, k, kmax=None):
+        qd, _ = _dev(q, np.float32)
+        nq = qd.shape[0]
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        k = np.ascontiguousarray(k, dtype=np.int32)
+        kmax = int(kmax or k.max())
+        ids = DeviceArray((nq, kmax), np.int32)
+        dis = DeviceArray((nq, kmax), np.float32)
+        _check(lib().mivf_search_var(self.h, qd.ptr, nq, _ptr(w), _ptr(k), ids.ptr, dis.ptr, kmax, None))
+        return ids.numpy(), dis.numpy()
+
+    def coarse(self, q, w):
+        qd, _ = _dev(q, np.float32)
+        nq = qd.shape[0]
+        idx = DeviceArray((nq, w), np.int32)
+        dis = DeviceArray((nq, w), np.float32)
+        _check(lib().mivf_coarse(self.h, qd.ptr, nq, w, idx.ptr, dis.ptr, None))
+        return idx.numpy(), dis.numpy()
+
+    def search_residual(self, residual, list_of, w, k):
+        rd, _ = _dev(residual, np.float32)
+        ld, _ = _dev(list_of, np.int32)
+        nq = rd.shape[0] // w
+        ids = DeviceArray((nq, k), np.int32)
+        dis = DeviceArray((nq, k), np.float32)
+        _check(lib().mivf_search_residual(self.h, rd.ptr, ld.ptr, nq, w, k, ids.ptr, dis.ptr, None))
+        return ids.numpy(), dis.numpy()
+
+    def search_rerank(self, q, w, kl, k, base):
+        qd, _ = _dev(q, np.float32)
+        bd, _ = _dev(base, np.float32)
+        nq = qd.shape[0]
+        ids = DeviceArray((nq, k), np.int32)
+        dis = DeviceArray((nq, k), np.float32)
+        _check(lib().mivf_search_rerank(self.h, qd.ptr, nq, w, kl, k, bd.ptr, bd.shape[0], ids.ptr,
+                                        dis.ptr, None))
+        return ids.numpy(), dis.numpy()

@@ -1,0 +1,787 @@
+// Host layer of libmivf: index lifecycle, GPU-assisted training and encoding,
+// search orchestration and the C ABI declared in include/mivf.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mivf.h"
+#include "mivf_internal.h"
+
+namespace mivf {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+const char *last_error() { return g_err.c_str(); }
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want) {
+        if (want <= bytes) return kOk;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            set_error(std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
+            return kNoMem;
+        }
+        bytes = want;
+        return kOk;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+// Per-stage HIP event timing on the launch stream (events owned by the index).
+struct Timer {
+    bool on = false;
+    hipStream_t s = nullptr;
+    hipEvent_t *ev = nullptr;
+    int n = 0;
+    int start(bool enable, hipStream_t st, hipEvent_t *evs) {
+        on = enable && evs;
+        s = st;
+        n = 0;
+        ev = evs;
+        if (!on) return kOk;
+        for (int i = 0; i < 8; ++i)
+            if (!ev[i]) MIVF_CHECK_HIP(hipEventCreate(&ev[i]));
+        return mark();
+    }
+    int mark() {
+        if (!on) return kOk;
+        MIVF_CHECK_HIP(hipEventRecord(ev[n++], s));
+        return kOk;
+    }
+};
+
+}  // namespace mivf
+
+using namespace mivf;
+
+struct mivf_index {
+    int d = 0, nlist = 0, m = 0, ks = 0, ds = 0;
+    // quantizers (device)
+    DevBuf coarse, coarse_n32, pq, pq_n64, pq_n32;
+    // lists: host mirror (list-major) + device arrays
+    std::vector<std::vector<int32_t>> h_ids;
+    std::vector<std::vector<uint8_t>> h_codes;
+    bool dirty = true;
+    int64_t ntotal = 0;
+    std::vector<int64_t> h_off;
+    DevBuf off, ids, codes;
+    // scratch
+    DevBuf qn, dmat, probes, coadis, partial, wq, kq, tmp_ids, tmp_dis, keys, res, resn, assign;
+    // options
+    int ppg_opt = 0;
+    bool profile = false;
+    float timings[8] = {};
+    hipEvent_t ev[8] = {};
+    ~mivf_index() {
+        for (auto &e : ev)
+            if (e) hipEventDestroy(e);
+    }
+};
+
+namespace {
+
+int upload_lists(mivf_index *ix) {
+    if (!ix->dirty) return kOk;
+    std::vector<int64_t> off(ix->nlist + 1, 0);
+    for (int l = 0; l < ix->nlist; ++l) off[l + 1] = off[l] + int64_t(ix->h_ids[l].size());
+    const int64_t n = off[ix->nlist];
+    MIVF_TRY(ix->off.ensure(sizeof(int64_t) * (ix->nlist + 1)));
+    MIVF_TRY(ix->ids.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    MIVF_TRY(ix->codes.ensure(size_t(ix->m) * std::max<int64_t>(n, 1) + 16));
+    MIVF_CHECK_HIP(hipMemcpy(ix->off.p, off.data(), sizeof(int64_t) * (ix->nlist + 1), hipMemcpyHostToDevice));
+    if (n > 0) {
+        std::vector<int32_t> ids(n);
+        std::vector<uint8_t> codes(size_t(n) * ix->m);
+        for (int l = 0; l < ix->nlist; ++l) {
+            std::copy(ix->h_ids[l].begin(), ix->h_ids[l].end(), ids.begin() + off[l]);
+            std::copy(ix->h_codes[l].begin(), ix->h_codes[l].end(), codes.begin() + off[l] * ix->m);
+        }
+        MIVF_CHECK_HIP(hipMemcpy(ix->ids.p, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        MIVF_CHECK_HIP(hipMemcpy(ix->codes.p, codes.data(), codes.size(), hipMemcpyHostToDevice));
+    }
+    ix->h_off = off;
+    ix->ntotal = n;
+    ix->dirty = false;
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// k-means: GPU assignment with the reference distance arithmetic (a = centroid,
+// b = point, strict '<' -> lowest index), host update in double precision
+// (sums in point order), deterministic split of the largest cluster for
+// empty ones.  Identical, step for step, to the CPU oracle's k-means.
+// ---------------------------------------------------------------------------
+int kmeans_dev(const float *d_x, const float *h_x, int64_t n, int d, int ldx, int k,
+               int niter, uint64_t seed, float *cent) {
+    MIVF_REQUIRE(k >= 1 && n >= k && d >= 1, kInvalid, "kmeans: need n >= k >= 1");
+    // seeded distinct initial points (partial Fisher-Yates, Lemire reduction)
+    std::vector<int64_t> perm(n);
+    for (int64_t i = 0; i < n; ++i) perm[i] = i;
+    const uint64_t base = synth_stream_base(seed, 0x6b6d65616e73ull);
+    for (int64_t i = 0; i < k; ++i) {
+        uint64_t r = synth_draw(base, uint64_t(i));
+        int64_t j = i + int64_t((unsigned __int128)r * uint64_t(n - i) >> 64);
+        std::swap(perm[i], perm[j]);
+    }
+    for (int c = 0; c < k; ++c)
+        for (int t = 0; t < d; ++t) cent[int64_t(c) * d + t] = h_x[perm[c] * ldx + t];
+
+    DevBuf dc, dcn, dxn, dkeys, dasg;
+    MIVF_TRY(dc.ensure(sizeof(float) * size_t(k) * d));
+    MIVF_TRY(dcn.ensure(sizeof(float) * k));
+    MIVF_TRY(dxn.ensure(sizeof(double) * n));
+    MIVF_TRY(dkeys.ensure(sizeof(uint64_t) * n));
+    MIVF_TRY(dasg.ensure(sizeof(int32_t) * n));
+    MIVF_CHECK_HIP(launch_norm_f64(d_x, n, d, ldx, dxn.as<double>(), nullptr));
+    std::vector<int32_t> assign(n);
+    std::vector<double> sums(size_t(k) * d);
+    std::vector<int64_t> cnt(k);
+    for (int it = 0; it < niter; ++it) {
+        MIVF_CHECK_HIP(hipMemcpy(dc.p, cent, sizeof(float) * size_t(k) * d, hipMemcpyHostToDevice));
+        MIVF_CHECK_HIP(launch_norm_f32(dc.as<float>(), k, d, d, dcn.as<float>(), nullptr));
+        MIVF_CHECK_HIP(launch_fill_u64(dkeys.as<uint64_t>(), n, kNoKey, nullptr));
+        MIVF_CHECK_HIP(launch_dist_argmin(dc.as<float>(), d, dcn.as<float>(), k, d_x, ldx,
+                                          dxn.as<double>(), n, d, dkeys.as<uint64_t>(), nullptr));
+        MIVF_CHECK_HIP(launch_keys_to_idx(dkeys.as<uint64_t>(), n, dasg.as<int32_t>(), nullptr));
+        MIVF_CHECK_HIP(hipMemcpy(assign.data(), dasg.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        std::fill(sums.begin(), sums.end(), 0.0);
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int64_t i = 0; i < n; ++i) {
+            const int c = assign[i];
+            MIVF_REQUIRE(c >= 0 && c < k, kInvalid, "kmeans: non-finite distances in training data");
+            cnt[c]++;
+            double *s = sums.data() + int64_t(c) * d;
+            const float *xi = h_x + i * ldx;
+            for (int t = 0; t < d; ++t) s[t] += double(xi[t]);
+        }
+        for (int c = 0; c < k; ++c)
+            if (cnt[c] > 0)
+                for (int t = 0; t < d; ++t)
+                    cent[int64_t(c) * d + t] = float(sums[int64_t(c) * d + t] / double(cnt[c]));
+        for (int c = 0; c < k; ++c) {
+            if (cnt[c] > 0) continue;
+            int big = 0;
+            for (int j = 1; j < k; ++j)
+                if (cnt[j] > cnt[big]) big = j;
+            for (int t = 0; t < d; ++t) {
+                double v = double(cent[int64_t(big) * d + t]);
+                double delta = (1.0 / 1024.0) * (1.0 + std::fabs(v));
+                cent[int64_t(c) * d + t] = float(v + delta);
+                cent[int64_t(big) * d + t] = float(v - delta);
+            }
+            cnt[c] = cnt[big] / 2;
+            cnt[big] -= cnt[c];
+        }
+    }
+    MIVF_CHECK_HIP(hipDeviceSynchronize());
+    return kOk;
+}
+
+int train_impl(const float *d_x, const float *h_x, int64_t n, int d, int nlist, int m, int ks,
+               int niter_c, int niter_pq, uint64_t seed, float *coarse_out, float *pq_out) {
+    MIVF_REQUIRE(d > 0 && m > 0 && d % m == 0, kInvalid, "train: d must be a multiple of m");
+    MIVF_REQUIRE(ks >= 1 && ks <= 256, kInvalid, "train: ks must be in 1..256");
+    MIVF_REQUIRE(n >= nlist && n >= ks, kInvalid, "train: need n >= nlist and n >= ks");
+    const int ds = d / m;
+    MIVF_TRY(kmeans_dev(d_x, h_x, n, d, d, nlist, niter_c, seed, coarse_out));
+    // assign training points, residuals
+    DevBuf dc, dcn, dxn, dkeys, dasg, dres;
+    MIVF_TRY(dc.ensure(sizeof(float) * size_t(nlist) * d));
+    MIVF_TRY(dcn.ensure(sizeof(float) * nlist));
+    MIVF_TRY(dxn.ensure(sizeof(double) * n));
+    MIVF_TRY(dkeys.ensure(sizeof(uint64_t) * n));
+    MIVF_TRY(dasg.ensure(sizeof(int32_t) * n));
+    MIVF_TRY(dres.ensure(sizeof(float) * size_t(n) * d));
+    MIVF_CHECK_HIP(hipMemcpy(dc.p, coarse_out, sizeof(float) * size_t(nlist) * d, hipMemcpyHostToDevice));
+    MIVF_CHECK_HIP(launch_norm_f32(dc.as<float>(), nlist, d, d, dcn.as<float>(), nullptr));
+    MIVF_CHECK_HIP(launch_norm_f64(d_x, n, d, d, dxn.as<double>(), nullptr));
+    MIVF_CHECK_HIP(launch_fill_u64(dkeys.as<uint64_t>(), n, kNoKey, nullptr));
+    MIVF_CHECK_HIP(launch_dist_argmin(dc.as<float>(), d, dcn.as<float>(), nlist, d_x, d,
+                                      dxn.as<double>(), n, d, dkeys.as<uint64_t>(), nullptr));
+    MIVF_CHECK_HIP(launch_keys_to_idx(dkeys.as<uint64_t>(), n, dasg.as<int32_t>(), nullptr));
+    MIVF_CHECK_HIP(launch_residual(d_x, n, d, dc.as<float>(), dasg.as<int32_t>(), dres.as<float>(), nullptr));
+    dxn.release();
+    dkeys.release();
+    std::vector<float> hres(size_t(n) * d);
+    MIVF_CHECK_HIP(hipMemcpy(hres.data(), dres.p, sizeof(float) * hres.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < m; ++s) {
+        MIVF_TRY(kmeans_dev(dres.as<float>() + s * ds, hres.data() + s * ds, n, ds, d, ks, niter_pq,
+                            seed + 1 + uint64_t(s), pq_out + int64_t(s) * ks * ds));
+    }
+    return kOk;
+}
+
+// encode n rows of d_x: coarse argmin, residual, per-subquantizer argmin
+int encode_impl(mivf_index *ix, const float *d_x, int64_t n, int32_t *d_assign, uint8_t *d_codes,
+                hipStream_t st) {
+    const int d = ix->d, m = ix->m, ds = ix->ds;
+    MIVF_TRY(ix->qn.ensure(sizeof(double) * std::max<int64_t>(n, 1)));
+    MIVF_TRY(ix->keys.ensure(sizeof(uint64_t) * std::max<int64_t>(n, 1)));
+    MIVF_TRY(ix->res.ensure(sizeof(float) * size_t(std::max<int64_t>(n, 1)) * d));
+    MIVF_TRY(ix->resn.ensure(sizeof(double) * std::max<int64_t>(n, 1)));
+    uint64_t *keys = ix->keys.as<uint64_t>();
+    MIVF_CHECK_HIP(launch_norm_f64(d_x, n, d, d, ix->qn.as<double>(), st));
+    MIVF_CHECK_HIP(launch_fill_u64(keys, n, kNoKey, st));
+    MIVF_CHECK_HIP(launch_dist_argmin(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), ix->nlist,
+                                      d_x, d, ix->qn.as<double>(), n, d, keys, st));
+    MIVF_CHECK_HIP(launch_keys_to_idx(keys, n, d_assign, st));
+    MIVF_CHECK_HIP(launch_residual(d_x, n, d, ix->coarse.as<float>(), d_assign, ix->res.as<float>(), st));
+    for (int s = 0; s < m; ++s) {
+        const float *rs = ix->res.as<float>() + s * ds;
+        MIVF_CHECK_HIP(launch_norm_f64(rs, n, ds, d, ix->resn.as<double>(), st));
+        MIVF_CHECK_HIP(launch_fill_u64(keys, n, kNoKey, st));
+        MIVF_CHECK_HIP(launch_dist_argmin(ix->pq.as<float>() + int64_t(s) * ix->ks * ds, ds,
+                                          ix->pq_n32.as<float>() + s * ix->ks, ix->ks, rs, d,
+                                          ix->resn.as<double>(), n, ds, keys, st));
+        MIVF_CHECK_HIP(launch_codes_from_keys(keys, n, m, s, d_codes, st));
+    }
+    return kOk;
+}
+
+int add_impl(mivf_index *ix, const float *d_x, int64_t n, int32_t id0) {
+    const int64_t CH = 1 << 20;
+    DevBuf da, dcodes;
+    MIVF_TRY(da.ensure(sizeof(int32_t) * std::min(n, CH)));
+    MIVF_TRY(dcodes.ensure(size_t(ix->m) * std::min(n, CH)));
+    std::vector<int32_t> a(std::min(n, CH));
+    std::vector<uint8_t> c(size_t(ix->m) * std::min(n, CH));
+    for (int64_t r0 = 0; r0 < n; r0 += CH) {
+        const int64_t nn = std::min(CH, n - r0);
+        MIVF_TRY(encode_impl(ix, d_x + r0 * ix->d, nn, da.as<int32_t>(), dcodes.as<uint8_t>(), nullptr));
+        MIVF_CHECK_HIP(hipMemcpy(a.data(), da.p, sizeof(int32_t) * nn, hipMemcpyDeviceToHost));
+        MIVF_CHECK_HIP(hipMemcpy(c.data(), dcodes.p, size_t(ix->m) * nn, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < nn; ++i) {
+            const int l = a[i];
+            MIVF_REQUIRE(l >= 0 && l < ix->nlist, kInvalid, "add: non-finite input vector");
+            ix->h_ids[l].push_back(id0 + int32_t(r0 + i));
+            ix->h_codes[l].insert(ix->h_codes[l].end(), c.begin() + i * ix->m, c.begin() + (i + 1) * ix->m);
+        }
+    }
+    ix->dirty = true;
+    return kOk;
+}
+
+int set_lists_impl(mivf_index *ix, const int64_t *offsets, const int32_t *ids, const uint8_t *codes8,
+                   const int32_t *codes32) {
+    MIVF_REQUIRE(offsets && offsets[0] == 0, kInvalid, "set_lists: offsets[0] must be 0");
+    for (int l = 0; l < ix->nlist; ++l)
+        MIVF_REQUIRE(offsets[l + 1] >= offsets[l], kInvalid, "set_lists: offsets must be non-decreasing");
+    const int64_t n = offsets[ix->nlist];
+    MIVF_REQUIRE(n < (int64_t(1) << 31), kInvalid, "set_lists: more than 2^31-1 vectors");
+    MIVF_REQUIRE(n == 0 || (ids && (codes8 || codes32)), kInvalid, "set_lists: null ids or codes");
+    if (codes32)
+        for (int64_t i = 0; i < n * ix->m; ++i)
+            MIVF_REQUIRE(codes32[i] >= 0 && codes32[i] < ix->ks, kInvalid,
+                         "set_lists: code out of range [0, ks) (codes must fit 8 bits)");
+    if (codes8 && ix->ks < 256)
+        for (int64_t i = 0; i < n * ix->m; ++i)
+            MIVF_REQUIRE(codes8[i] < ix->ks, kInvalid, "set_lists: code out of range [0, ks)");
+    for (int l = 0; l < ix->nlist; ++l) {
+        const int64_t b = offsets[l], e = offsets[l + 1];
+        ix->h_ids[l].assign(ids + b, ids + e);
+        ix->h_codes[l].resize(size_t(e - b) * ix->m);
+        for (int64_t i = 0; i < (e - b) * ix->m; ++i)
+            ix->h_codes[l][i] = codes8 ? codes8[b * ix->m + i] : uint8_t(codes32[b * ix->m + i]);
+    }
+    ix->dirty = true;
+    return upload_lists(ix);
+}
+
+int check_index(const mivf_index *ix) {
+    MIVF_REQUIRE(ix != nullptr, kInvalid, "null index");
+    return kOk;
+}
+
+int choose_groups(const mivf_index *ix, int nq, int wmax, int *ppg) {
+    int p = ix->ppg_opt;
+    if (p <= 0) {
+        // enough workgroups to fill 256 CUs several times over
+        const int target = 2048;
+        int groups = nq >= target ? 1 : std::min(wmax, (target + nq - 1) / nq);
+        p = (wmax + groups - 1) / groups;
+    }
+    p = std::max(1, std::min(p, wmax));
+    *ppg = p;
+    return (wmax + p - 1) / p;
+}
+
+// Core search over a chunk of queries already on the device.
+//   coarse (q) -> probes -> scan (-> merge)
+// resid_mode: d_q holds residuals [nq*w][d] and d_probes_in the list ids.
+int search_core(mivf_index *ix, const float *d_q, int nq, int wmax, const int32_t *d_wq,
+                const int32_t *d_kq, int kuni, int kstride, int32_t *d_ids, float *d_dis,
+                hipStream_t st, const int32_t *d_probes_in, Timer &tm) {
+    const int nlist = ix->nlist, d = ix->d;
+    const int32_t *probes = d_probes_in;
+    if (!probes) {
+        MIVF_TRY(ix->qn.ensure(sizeof(double) * nq));
+        MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nq) * nlist));
+        MIVF_TRY(ix->probes.ensure(sizeof(int32_t) * size_t(nq) * wmax));
+        MIVF_TRY(ix->coadis.ensure(sizeof(float) * size_t(nq) * wmax));
+        MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ix->qn.as<double>(), st));
+        MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), nlist, d_q, d,
+                                         ix->qn.as<double>(), nq, d, ix->dmat.as<float>(), nlist, st));
+        MIVF_TRY(tm.mark());
+        MIVF_CHECK_HIP(launch_row_topk(ix->dmat.as<float>(), nq, nlist, nlist, wmax,
+                                       ix->probes.as<int32_t>(), ix->coadis.as<float>(), st));
+        MIVF_TRY(tm.mark());
+        probes = ix->probes.as<int32_t>();
+    } else {
+        MIVF_TRY(tm.mark());
+        MIVF_TRY(tm.mark());
+    }
+    int ppg = 0;
+    const int ngroups = d_probes_in ? 1 : choose_groups(ix, nq, wmax, &ppg);
+    if (d_probes_in) ppg = wmax;
+    ScanArgs a{};
+    a.queries = d_q;
+    a.coarse = ix->coarse.as<float>();
+    a.pq = ix->pq.as<float>();
+    a.pq_norm = ix->pq_n64.as<double>();
+    a.list_off = ix->off.as<int64_t>();
+    a.list_ids = ix->ids.as<int32_t>();
+    a.codes = ix->codes.as<uint8_t>();
+    a.probes = probes;
+    a.wq = d_wq;
+    a.kq = d_kq;
+    a.nq = nq;
+    a.w = wmax;
+    a.wstride = wmax;
+    a.k = kuni;
+    a.kstride = kstride;
+    a.ks = ix->ks;
+    a.nlist = nlist;
+    a.ppg = ppg;
+    a.ngroups = ngroups;
+    a.resid_mode = d_probes_in ? 1 : 0;
+    if (ngroups > 1) {
+        MIVF_TRY(ix->partial.ensure(sizeof(uint64_t) * size_t(nq) * ngroups * kstride));
+        a.partial = ix->partial.as<uint64_t>();
+    } else {
+        a.out_ids = d_ids;
+        a.out_dis = d_dis;
+    }
+    MIVF_CHECK_HIP(launch_scan(a, ix->m, st));
+    MIVF_TRY(tm.mark());
+    if (ngroups > 1)
+        MIVF_CHECK_HIP(launch_merge(ix->partial.as<uint64_t>(), nq, ngroups, kstride, d_kq, kuni, d_ids,
+                                    d_dis, st));
+    MIVF_TRY(tm.mark());
+    return kOk;
+}
+
+int finish_timer(mivf_index *ix, Timer &tm) {
+    if (!tm.on) return kOk;
+    MIVF_CHECK_HIP(hipEventSynchronize(tm.ev[tm.n - 1]));
+    for (int i = 0; i + 1 < tm.n && i < 8; ++i) {
+        float ms = 0;
+        MIVF_CHECK_HIP(hipEventElapsedTime(&ms, tm.ev[i], tm.ev[i + 1]));
+        ix->timings[i] += ms;
+    }
+    return kOk;
+}
+
+int search_entry(mivf_index *ix, const float *d_q, int nq, int wmax, const int32_t *d_wq,
+                 const int32_t *d_kq, int kuni, int kstride, int32_t *d_ids, float *d_dis,
+                 hipStream_t st) {
+    MIVF_TRY(upload_lists(ix));
+    Timer tm;
+    std::fill(ix->timings, ix->timings + 8, 0.0f);
+    // bound the coarse distance matrix to ~512 MiB per chunk
+    const int64_t maxq = std::max<int64_t>(1, (int64_t(512) << 20) / (4 * int64_t(ix->nlist)));
+    for (int64_t q0 = 0; q0 < nq; q0 += maxq) {
+        const int nc = int(std::min<int64_t>(maxq, nq - q0));
+        MIVF_TRY(tm.start(ix->profile, st, ix->ev));
+        MIVF_TRY(search_core(ix, d_q + q0 * ix->d, nc, wmax, d_wq ? d_wq + q0 : nullptr,
+                             d_kq ? d_kq + q0 : nullptr, kuni, kstride, d_ids + q0 * kstride,
+                             d_dis + q0 * kstride, st, nullptr, tm));
+        MIVF_TRY(finish_timer(ix, tm));
+    }
+    return kOk;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *mivf_last_error(void) { return last_error(); }
+int mivf_version(void) { return 100; }
+
+int mivf_device_count(int *count) {
+    MIVF_REQUIRE(count, kInvalid, "null count");
+    MIVF_CHECK_HIP(hipGetDeviceCount(count));
+    return kOk;
+}
+int mivf_malloc(void **p, size_t bytes) {
+    MIVF_REQUIRE(p, kInvalid, "null pointer");
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return kNoMem;
+    }
+    return kOk;
+}
+int mivf_free(void *p) {
+    MIVF_CHECK_HIP(hipFree(p));
+    return kOk;
+}
+int mivf_memcpy_htod(void *dst, const void *src, size_t bytes) {
+    MIVF_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return kOk;
+}
+int mivf_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+    MIVF_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return kOk;
+}
+int mivf_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+    MIVF_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return kOk;
+}
+int mivf_synchronize(void) {
+    MIVF_CHECK_HIP(hipDeviceSynchronize());
+    return kOk;
+}
+
+int mivf_kmeans(const float *x, int64_t n, int d, int ldx, int k, int niter, uint64_t seed,
+                float *cent_out) {
+    MIVF_REQUIRE(x && cent_out && n > 0 && d > 0 && ldx >= d, kInvalid, "kmeans: bad arguments");
+    DevBuf dx;
+    MIVF_TRY(dx.ensure(sizeof(float) * size_t(n) * ldx));
+    MIVF_CHECK_HIP(hipMemcpy(dx.p, x, sizeof(float) * size_t(n) * ldx, hipMemcpyHostToDevice));
+    return kmeans_dev(dx.as<float>(), x, n, d, ldx, k, niter, seed, cent_out);
+}
+
+int mivf_train(const float *x, int64_t n, int d, int nlist, int m, int ks, int niter_coarse,
+               int niter_pq, uint64_t seed, float *coarse_out, float *pq_out) {
+    MIVF_REQUIRE(x && coarse_out && pq_out && n > 0 && d > 0 && nlist > 0, kInvalid, "train: bad arguments");
+    DevBuf dx;
+    MIVF_TRY(dx.ensure(sizeof(float) * size_t(n) * d));
+    MIVF_CHECK_HIP(hipMemcpy(dx.p, x, sizeof(float) * size_t(n) * d, hipMemcpyHostToDevice));
+    return train_impl(dx.as<float>(), x, n, d, nlist, m, ks, niter_coarse, niter_pq, seed, coarse_out, pq_out);
+}
+
+int mivf_train_device(const float *d_x, int64_t n, int d, int nlist, int m, int ks, int niter_coarse,
+                      int niter_pq, uint64_t seed, float *coarse_out, float *pq_out) {
+    MIVF_REQUIRE(d_x && coarse_out && pq_out && n > 0 && d > 0 && nlist > 0, kInvalid, "train: bad arguments");
+    std::vector<float> hx(size_t(n) * d);
+    MIVF_CHECK_HIP(hipMemcpy(hx.data(), d_x, sizeof(float) * hx.size(), hipMemcpyDeviceToHost));
+    return train_impl(d_x, hx.data(), n, d, nlist, m, ks, niter_coarse, niter_pq, seed, coarse_out, pq_out);
+}
+
+int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, const float *pq,
+                      mivf_index **out) {
+    MIVF_REQUIRE(out && coarse && pq, kInvalid, "index_create: null argument");
+    MIVF_REQUIRE(nlist >= 1 && m >= 1 && d >= 1 && d % m == 0, kInvalid, "index_create: d must be a multiple of m");
+    MIVF_REQUIRE(ks >= 1 && ks <= 256, kInvalid, "index_create: ks must be in 1..256 (8-bit codes)");
+    MIVF_REQUIRE(d == 128 && (m == 4 || m == 8 || m == 16 || m == 32), kUnsupported,
+                 "index_create: compiled search kernels cover d = 128 with m in {4, 8, 16, 32}");
+    for (int64_t i = 0; i < int64_t(nlist) * d; ++i)
+        MIVF_REQUIRE(std::isfinite(coarse[i]), kInvalid, "index_create: non-finite coarse centroid");
+    for (int64_t i = 0; i < int64_t(ks) * d; ++i)
+        MIVF_REQUIRE(std::isfinite(pq[i]), kInvalid, "index_create: non-finite PQ centroid");
+    mivf_index *ix = new mivf_index();
+    ix->d = d;
+    ix->nlist = nlist;
+    ix->m = m;
+    ix->ks = ks;
+    ix->ds = d / m;
+    ix->h_ids.resize(nlist);
+    ix->h_codes.resize(nlist);
+    auto fail = [&](int rc) {
+        delete ix;
+        return rc;
+    };
+    int rc;
+    if ((rc = ix->coarse.ensure(sizeof(float) * size_t(nlist) * d)) ||
+        (rc = ix->coarse_n32.ensure(sizeof(float) * nlist)) ||
+        (rc = ix->pq.ensure(sizeof(float) * size_t(ks) * d)) ||
+        (rc = ix->pq_n64.ensure(sizeof(double) * size_t(m) * ks)) ||
+        (rc = ix->pq_n32.ensure(sizeof(float) * size_t(m) * ks)))
+        return fail(rc);
+    if (hipMemcpy(ix->coarse.p, coarse, sizeof(float) * size_t(nlist) * d, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ix->pq.p, pq, sizeof(float) * size_t(ks) * d, hipMemcpyHostToDevice) != hipSuccess ||
+        launch_norm_f32(ix->coarse.as<float>(), nlist, d, d, ix->coarse_n32.as<float>(), nullptr) != hipSuccess ||
+        launch_norm_f64(ix->pq.as<float>(), int64_t(m) * ks, ix->ds, ix->ds, ix->pq_n64.as<double>(), nullptr) != hipSuccess ||
+        launch_norm_f32(ix->pq.as<float>(), int64_t(m) * ks, ix->ds, ix->ds, ix->pq_n32.as<float>(), nullptr) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        set_error("index_create: HIP upload failed");
+        return fail(kHip);
+    }
+    if ((rc = upload_lists(ix))) return fail(rc);
+    *out = ix;
+    return kOk;
+}
+
+int mivf_index_free(mivf_index *ix) {
+    if (!ix) return kOk;
+    hipDeviceSynchronize();
+    delete ix;
+    return kOk;
+}
+
+int mivf_index_info(const mivf_index *ix, int *d, int *nlist, int *m, int *ks, int64_t *ntotal) {
+    MIVF_TRY(check_index(ix));
+    if (d) *d = ix->d;
+    if (nlist) *nlist = ix->nlist;
+    if (m) *m = ix->m;
+    if (ks) *ks = ix->ks;
+    if (ntotal) {
+        int64_t n = 0;
+        for (auto &v : ix->h_ids) n += int64_t(v.size());
+        *ntotal = n;
+    }
+    return kOk;
+}
+
+int mivf_index_set_lists(mivf_index *ix, const int64_t *offsets, const int32_t *ids, const uint8_t *codes) {
+    MIVF_TRY(check_index(ix));
+    return set_lists_impl(ix, offsets, ids, codes, nullptr);
+}
+
+int mivf_index_set_lists_i32(mivf_index *ix, const int64_t *offsets, const int32_t *ids, const int32_t *codes) {
+    MIVF_TRY(check_index(ix));
+    return set_lists_impl(ix, offsets, ids, nullptr, codes);
+}
+
+int mivf_index_get_lists(mivf_index *ix, int64_t *offsets, int32_t *ids, uint8_t *codes) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(offsets, kInvalid, "get_lists: null offsets");
+    int64_t o = 0;
+    offsets[0] = 0;
+    for (int l = 0; l < ix->nlist; ++l) {
+        const int64_t len = int64_t(ix->h_ids[l].size());
+        if (ids) std::memcpy(ids + o, ix->h_ids[l].data(), sizeof(int32_t) * len);
+        if (codes) std::memcpy(codes + o * ix->m, ix->h_codes[l].data(), size_t(ix->m) * len);
+        o += len;
+        offsets[l + 1] = o;
+    }
+    return kOk;
+}
+
+int mivf_index_add(mivf_index *ix, const float *x, int64_t n, int32_t id0) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(n >= 0 && (n == 0 || x), kInvalid, "add: bad arguments");
+    MIVF_REQUIRE(int64_t(id0) + n <= (int64_t(1) << 31), kInvalid, "add: ids exceed int32");
+    if (n == 0) return kOk;
+    const int64_t CH = 1 << 20;
+    DevBuf dx;
+    MIVF_TRY(dx.ensure(sizeof(float) * size_t(std::min(n, CH)) * ix->d));
+    for (int64_t r0 = 0; r0 < n; r0 += CH) {
+        const int64_t nn = std::min(CH, n - r0);
+        MIVF_CHECK_HIP(hipMemcpy(dx.p, x + r0 * ix->d, sizeof(float) * size_t(nn) * ix->d, hipMemcpyHostToDevice));
+        MIVF_TRY(add_impl(ix, dx.as<float>(), nn, id0 + int32_t(r0)));
+    }
+    return kOk;
+}
+
+int mivf_index_add_device(mivf_index *ix, const float *d_x, int64_t n, int32_t id0) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(n >= 0 && (n == 0 || d_x), kInvalid, "add: bad arguments");
+    MIVF_REQUIRE(int64_t(id0) + n <= (int64_t(1) << 31), kInvalid, "add: ids exceed int32");
+    if (n == 0) return kOk;
+    return add_impl(ix, d_x, n, id0);
+}
+
+int mivf_index_encode(mivf_index *ix, const float *d_x, int64_t n, int32_t *d_assign, uint8_t *d_codes,
+                      void *stream) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(n >= 0 && (n == 0 || (d_x && d_assign && d_codes)), kInvalid, "encode: bad arguments");
+    return encode_impl(ix, d_x, n, d_assign, d_codes, static_cast<hipStream_t>(stream));
+}
+
+int mivf_search(mivf_index *ix, const float *d_q, int nq, int w, int k, int32_t *d_ids, float *d_dis,
+                void *stream) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(nq >= 0, kInvalid, "search: nq < 0");
+    MIVF_REQUIRE(w >= 1 && w <= ix->nlist, kInvalid, "search: w must be in 1..nlist");
+    MIVF_REQUIRE(w <= 1024, kUnsupported, "search: w > 1024");
+    MIVF_REQUIRE(k >= 1 && k <= 1024, kInvalid, "search: k must be in 1..1024");
+    if (nq == 0) return kOk;
+    MIVF_REQUIRE(d_q && d_ids && d_dis, kInvalid, "search: null pointer");
+    return search_entry(ix, d_q, nq, w, nullptr, nullptr, k, k, d_ids, d_dis, static_cast<hipStream_t>(stream));
+}
+
+int mivf_search_var(mivf_index *ix, const float *d_q, int nq, const int32_t *w, const int32_t *k,
+                    int32_t *d_ids, float *d_dis, int kmax, void *stream) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(nq >= 0 && kmax >= 1 && kmax <= 1024, kInvalid, "search_var: kmax must be in 1..1024");
+    if (nq == 0) return kOk;
+    MIVF_REQUIRE(d_q && w && k && d_ids && d_dis, kInvalid, "search_var: null pointer");
+    int wmax = 1;
+    for (int i = 0; i < nq; ++i) {
+        MIVF_REQUIRE(w[i] >= 1 && w[i] <= ix->nlist && w[i] <= 1024, kInvalid, "search_var: w[i] out of range");
+        MIVF_REQUIRE(k[i] >= 1 && k[i] <= kmax, kInvalid, "search_var: k[i] out of 1..kmax");
+        wmax = std::max(wmax, int(w[i]));
+    }
+    MIVF_TRY(ix->wq.ensure(sizeof(int32_t) * nq));
+    MIVF_TRY(ix->kq.ensure(sizeof(int32_t) * nq));
+    MIVF_CHECK_HIP(hipMemcpy(ix->wq.p, w, sizeof(int32_t) * nq, hipMemcpyHostToDevice));
+    MIVF_CHECK_HIP(hipMemcpy(ix->kq.p, k, sizeof(int32_t) * nq, hipMemcpyHostToDevice));
+    return search_entry(ix, d_q, nq, wmax, ix->wq.as<int32_t>(), ix->kq.as<int32_t>(), kmax, kmax, d_ids,
+                        d_dis, static_cast<hipStream_t>(stream));
+}
+
+int mivf_coarse(mivf_index *ix, const float *d_q, int nq, int w, int32_t *d_idx, float *d_dis, void *stream) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(w >= 1 && w <= ix->nlist && w <= 1024, kInvalid, "coarse: w must be in 1..min(nlist,1024)");
+    if (nq <= 0) return kOk;
+    MIVF_REQUIRE(d_q && d_idx && d_dis, kInvalid, "coarse: null pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t maxq = std::max<int64_t>(1, (int64_t(512) << 20) / (4 * int64_t(ix->nlist)));
+    for (int64_t q0 = 0; q0 < nq; q0 += maxq) {
+        const int nc = int(std::min<int64_t>(maxq, nq - q0));
+        MIVF_TRY(ix->qn.ensure(sizeof(double) * nc));
+        MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nc) * ix->nlist));
+        MIVF_CHECK_HIP(launch_norm_f64(d_q + q0 * ix->d, nc, ix->d, ix->d, ix->qn.as<double>(), st));
+        MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), ix->d, ix->coarse_n32.as<float>(), ix->nlist,
+                                         d_q + q0 * ix->d, ix->d, ix->qn.as<double>(), nc, ix->d,
+                                         ix->dmat.as<float>(), ix->nlist, st));
+        MIVF_CHECK_HIP(launch_row_topk(ix->dmat.as<float>(), nc, ix->nlist, ix->nlist, w, d_idx + q0 * w,
+                                       d_dis + q0 * w, st));
+    }
+    return kOk;
+}
+
+int mivf_search_residual(mivf_index *ix, const float *d_residual, const int32_t *d_list_of, int nq, int w,
+                         int k, int32_t *d_ids, float *d_dis, void *stream) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(w >= 1 && k >= 1 && k <= 1024, kInvalid, "search_residual: bad w or k");
+    if (nq <= 0) return kOk;
+    MIVF_REQUIRE(d_residual && d_list_of && d_ids && d_dis, kInvalid, "search_residual: null pointer");
+    MIVF_TRY(upload_lists(ix));
+    Timer tm;
+    MIVF_TRY(tm.start(false, static_cast<hipStream_t>(stream), nullptr));
+    return search_core(ix, d_residual, nq, w, nullptr, nullptr, k, k, d_ids, d_dis,
+                       static_cast<hipStream_t>(stream), d_list_of, tm);
+}
+
+int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, int k, const float *d_base,
+                       int64_t nbase, int32_t *d_ids, float *d_dis, void *stream) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(w >= 1 && w <= ix->nlist && w <= 1024, kInvalid, "rerank: w must be in 1..min(nlist,1024)");
+    MIVF_REQUIRE(kl >= 1 && kl <= 1024 && k >= 1 && k <= 1024, kInvalid, "rerank: kl and k must be in 1..1024");
+    if (nq <= 0) return kOk;
+    MIVF_REQUIRE(d_q && d_base && d_ids && d_dis && nbase >= 0, kInvalid, "rerank: null pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    MIVF_TRY(ix->tmp_ids.ensure(sizeof(int32_t) * size_t(nq) * kl));
+    MIVF_TRY(ix->tmp_dis.ensure(sizeof(float) * size_t(nq) * kl));
+    MIVF_TRY(search_entry(ix, d_q, nq, w, nullptr, nullptr, kl, kl, ix->tmp_ids.as<int32_t>(),
+                          ix->tmp_dis.as<float>(), st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ix->profile) {
+        MIVF_CHECK_HIP(hipEventCreate(&e0));
+        MIVF_CHECK_HIP(hipEventCreate(&e1));
+        MIVF_CHECK_HIP(hipEventRecord(e0, st));
+    }
+    MIVF_CHECK_HIP(launch_rerank(d_base, nbase, d_q, nq, ix->d, ix->tmp_ids.as<int32_t>(), kl, k, d_ids, d_dis, st));
+    if (ix->profile) {
+        MIVF_CHECK_HIP(hipEventRecord(e1, st));
+        MIVF_CHECK_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MIVF_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ix->timings[4] = ms;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    }
+    return kOk;
+}
+
+int mivf_set_option(mivf_index *ix, const char *name, int64_t value) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(name, kInvalid, "set_option: null name");
+    const std::string n(name);
+    if (n == "probes_per_group") {
+        MIVF_REQUIRE(value >= 0 && value <= 1024, kInvalid, "probes_per_group must be in 0..1024");
+        ix->ppg_opt = int(value);
+    } else if (n == "profile") {
+        ix->profile = value != 0;
+    } else {
+        set_error("set_option: unknown option " + n);
+        return kInvalid;
+    }
+    return kOk;
+}
+
+int mivf_get_timings(mivf_index *ix, float *ms, int n) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(ms && n >= 0, kInvalid, "get_timings: bad arguments");
+    for (int i = 0; i < n && i < 8; ++i) ms[i] = ix->timings[i];
+    return kOk;
+}
+
+int mivf_synth_rows(uint64_t seed, uint64_t stream_id, int64_t row0, int64_t n, int d, const float *d_centres,
+                    int ncomp, const uint64_t *d_thr, float sigma, float *d_out, void *stream) {
+    MIVF_REQUIRE(n >= 0 && d > 0 && ncomp > 0 && d_centres && d_out, kInvalid, "synth_rows: bad arguments");
+    MIVF_CHECK_HIP(launch_gen_rows(seed, stream_id, row0, n, d, d_centres, ncomp, d_thr, sigma, d_out,
+                                   static_cast<hipStream_t>(stream)));
+    return kOk;
+}
+
+int mivf_synth_centres(uint64_t seed, int ncomp, int d, float *centres) {
+    MIVF_REQUIRE(ncomp > 0 && d > 0 && centres, kInvalid, "synth_centres: bad arguments");
+    const uint64_t b = synth_stream_base(seed, 1);
+    for (int64_t i = 0; i < int64_t(ncomp) * d; ++i) {
+        uint64_t u = synth_draw(b, uint64_t(i)) >> 11;
+        centres[i] = float(double(u) * (1.0 / 9007199254740992.0) * 128.0);
+    }
+    return kOk;
+}
+
+int mivf_zipf_thresholds(uint64_t seed, int ncomp, double s, uint64_t *thr) {
+    MIVF_REQUIRE(ncomp > 0 && thr, kInvalid, "zipf_thresholds: bad arguments");
+    std::vector<int> perm(ncomp), rank(ncomp);
+    const uint64_t b = synth_stream_base(seed, 0x7a697066ull);
+    for (int i = 0; i < ncomp; ++i) perm[i] = i;
+    for (int i = 0; i < ncomp - 1; ++i) {
+        uint64_t r = synth_draw(b, uint64_t(i));
+        int j = i + int((unsigned __int128)r * uint64_t(ncomp - i) >> 64);
+        std::swap(perm[i], perm[j]);
+    }
+    for (int i = 0; i < ncomp; ++i) rank[perm[i]] = i;
+    double tot = 0;
+    for (int c = 0; c < ncomp; ++c) tot += 1.0 / std::pow(double(rank[c] + 1), s);
+    double cum = 0;
+    for (int c = 0; c < ncomp; ++c) {
+        cum += 1.0 / std::pow(double(rank[c] + 1), s);
+        double t = std::floor(cum / tot * 9007199254740992.0);
+        thr[c] = (c == ncomp - 1 || t > 9007199254740992.0) ? (uint64_t(1) << 53) : uint64_t(t);
+    }
+    return kOk;
+}
+
+int mivf_bruteforce_nn(const float *d_base, int64_t nb, const float *d_q, int nq, int d, int32_t *d_ids,
+                       double *d_dis) {
+    MIVF_REQUIRE(d == 128, kUnsupported, "bruteforce_nn: d = 128 only");
+    MIVF_REQUIRE(nb > 0 && nq >= 0 && d_base && d_q && d_ids && d_dis, kInvalid, "bruteforce_nn: bad arguments");
+    if (nq == 0) return kOk;
+    const int nchunks = int(std::min<int64_t>(nb, 512));
+    DevBuf pd, pi;
+    MIVF_TRY(pd.ensure(sizeof(double) * size_t(nchunks) * nq));
+    MIVF_TRY(pi.ensure(sizeof(int32_t) * size_t(nchunks) * nq));
+    MIVF_CHECK_HIP(launch_bruteforce_nn(d_base, nb, d_q, nq, d, d_ids, d_dis, pd.as<double>(), pi.as<int32_t>(),
+                                        nchunks, nullptr));
+    MIVF_CHECK_HIP(hipDeviceSynchronize());
+    return kOk;
+}
+
+}  // extern "C"

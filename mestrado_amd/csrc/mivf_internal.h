@@ -1,0 +1,155 @@
+// Internal declarations shared by the HIP kernels and the C++ host layer of
+// libmivf (MI355X IVF-PQ search).  Not part of the public C ABI.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace mivf {
+
+// ---------------------------------------------------------------------------
+// Errors: every host entry point returns an int status and keeps a
+// thread-local message (read through mivf_last_error()).
+// ---------------------------------------------------------------------------
+enum Status : int {
+    kOk = 0,
+    kInvalid = 1,     // bad argument / shape
+    kHip = 2,         // HIP runtime error
+    kNoMem = 3,       // allocation failure or over budget
+    kIo = 4,          // file missing, truncated or corrupt
+    kUnsupported = 5  // shape outside the compiled kernel set
+};
+
+void set_error(const std::string &msg);
+const char *last_error();
+
+#define MIVF_CHECK_HIP(call)                                                     \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            ::mivf::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return ::mivf::kHip;                                                 \
+        }                                                                        \
+    } while (0)
+
+#define MIVF_REQUIRE(cond, code, msg)  \
+    do {                               \
+        if (!(cond)) {                 \
+            ::mivf::set_error(msg);    \
+            return (code);             \
+        }                              \
+    } while (0)
+
+#define MIVF_TRY(expr)              \
+    do {                            \
+        int rc_ = (expr);           \
+        if (rc_ != ::mivf::kOk)     \
+            return rc_;             \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Order keys.  A candidate (distance, id) becomes one unsigned 64-bit key
+// whose integer order is (distance ascending, id ascending), the order of the
+// reference GPU top-k (parallel2/ivf_pq/topk.cu:26-32).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t fkey(float f) {
+    f = f + 0.0f;  // -0 and +0 compare equal
+    uint32_t b = __builtin_bit_cast(uint32_t, f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ inline uint64_t ckey(float d, int32_t id) {
+    return (uint64_t(fkey(d)) << 32) | uint32_t(id);
+}
+__host__ __device__ inline float key_dist(uint64_t key) {
+    uint32_t h = uint32_t(key >> 32);
+    uint32_t b = (h & 0x80000000u) ? (h & 0x7FFFFFFFu) : ~h;
+    return __builtin_bit_cast(float, b);
+}
+__host__ __device__ inline int32_t key_id(uint64_t key) {
+    return int32_t(uint32_t(key & 0xFFFFFFFFu));
+}
+constexpr uint64_t kNoKey = ~uint64_t(0);
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (defined in the .hip files).  All take a hipStream_t and
+// return hipError_t from the launch.
+// ---------------------------------------------------------------------------
+
+// Row norms: f32 accumulator (a side) / f64 accumulator of f32 products (b side).
+hipError_t launch_norm_f32(const float *x, int64_t n, int d, int ldx, float *out, hipStream_t s);
+hipError_t launch_norm_f64(const float *x, int64_t n, int d, int ldx, double *out, hipStream_t s);
+
+// Exact distances out[j*ldo + i] = dist(a_i, b_j) for i < na, j < nb.
+hipError_t launch_dist_store(const float *a, int lda, const float *an, int na,
+                             const float *b, int ldb, const double *bn, int64_t nb,
+                             int d, float *out, int64_t ldo, hipStream_t s);
+// Exact argmin: keys[j] = min_i ckey(dist(a_i, b_j), i).  keys must be preset
+// to kNoKey by the caller (fill kernel below).
+hipError_t launch_dist_argmin(const float *a, int lda, const float *an, int na,
+                              const float *b, int ldb, const double *bn, int64_t nb,
+                              int d, uint64_t *keys, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s);
+hipError_t launch_keys_to_idx(const uint64_t *keys, int64_t n, int32_t *idx, hipStream_t s);
+
+// Row top-k over a distance matrix (rows of length n, stride ld): sorted
+// (dist, col) ascending, k <= 1024.  Writes idx/dis [rows][k].
+hipError_t launch_row_topk(const float *D, int64_t rows, int n, int64_t ld, int k,
+                           int32_t *idx, float *dis, hipStream_t s);
+
+// Residuals: r[i] = x[i] - c[assign[i]]
+hipError_t launch_residual(const float *x, int64_t n, int d, const float *c,
+                           const int32_t *assign, float *r, hipStream_t s);
+hipError_t launch_codes_from_keys(const uint64_t *keys, int64_t n, int m, int sub,
+                                  uint8_t *codes, hipStream_t s);
+
+// Fused plain search: per (query, probe group) residual -> m x 256 table in
+// LDS -> ADC scan -> running exact top-k.
+struct ScanArgs {
+    const float *queries;     // [nq][d], or residuals [nq*w][d] when resid_mode
+    const float *coarse;      // [nlist][d]
+    const float *pq;          // [m][ks][ds]
+    const double *pq_norm;    // [m][ks] (f64-accumulated norms)
+    const int64_t *list_off;  // [nlist+1]
+    const int32_t *list_ids;  // [N]
+    const uint8_t *codes;     // [N][m]
+    const int32_t *probes;    // [nq][wstride] list ids (coarse order)
+    const int32_t *wq;        // per-query w or nullptr
+    const int32_t *kq;        // per-query k or nullptr
+    int nq, w, wstride, k, kstride;
+    int ks;                   // centroids per subquantizer (<= 256)
+    int nlist;                // probes outside [0, nlist) are skipped
+    int ppg;                  // probes per group
+    int ngroups;
+    int resid_mode;           // 1: queries holds residuals, row = q*wstride + p
+    uint64_t *partial;        // [nq][ngroups][kstride] when ngroups > 1
+    int32_t *out_ids;         // [nq][kstride] when ngroups == 1
+    float *out_dis;
+};
+hipError_t launch_scan(const ScanArgs &a, int m, hipStream_t s);
+// Merge per-group partial keys to the final sorted top-k.
+hipError_t launch_merge(const uint64_t *partial, int nq, int ngroups, int kstride,
+                        const int32_t *kq, int k, int32_t *out_ids, float *out_dis,
+                        hipStream_t s);
+
+// Exact rerank of a shortlist against raw vectors (a = base row, b = query).
+hipError_t launch_rerank(const float *base, int64_t nbase, const float *queries, int nq,
+                         int d, const int32_t *short_ids, int kl, int k,
+                         int32_t *out_ids, float *out_dis, hipStream_t s);
+
+// Synthetic mixture rows (bit-identical to the CPU generator).
+uint64_t synth_stream_base(uint64_t seed, uint64_t stream);
+uint64_t synth_draw(uint64_t base, uint64_t ctr);
+hipError_t launch_gen_rows(uint64_t seed, uint64_t stream, int64_t row0, int64_t n, int d,
+                           const float *centres, int ncomp, const uint64_t *thr,
+                           float sigma, float *out, hipStream_t s);
+
+// FP64 brute-force nearest neighbour (ties -> lowest id).
+hipError_t launch_bruteforce_nn(const float *base, int64_t nb, const float *q, int nq,
+                                int d, int32_t *ids, double *dis, double *part_d,
+                                int32_t *part_i, int nchunks, hipStream_t s);
+
+}  // namespace mivf

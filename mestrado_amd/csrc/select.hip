@@ -1,0 +1,170 @@
+// Row top-k (coarse probe selection), the merge of per-group partial top-k
+// lists, and the exact rerank of a shortlist.  All selections order by
+// (distance ascending, id ascending) through 64-bit keys and are exact.
+#include "select_dev.h"
+
+namespace mivf {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int KBUF = 1024;  // max k
+
+// Collect the selected keys into buf (unsorted), return count.
+template <typename KeyFn>
+__device__ int collect(KeyFn keyfn, int n, SelectResult r, uint64_t *buf, int *cnt, int *eqt) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        *cnt = 0;
+        *eqt = 0;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) {
+        uint64_t key = keyfn(i);
+        if (key == kNoKey) continue;
+        bool take = key < r.thr;
+        if (!take && r.eq > 0 && key == r.thr) take = atomicAdd(eqt, 1) < r.eq;
+        if (take) {
+            const int pos = atomicAdd(cnt, 1);
+            if (pos < KBUF) buf[pos] = key;
+        }
+    }
+    __syncthreads();
+    return min(*cnt, KBUF);
+}
+
+__device__ void sort_and_emit(uint64_t *buf, int c, int k, int32_t *idx, float *dis) {
+    const int tid = threadIdx.x;
+    const int np = next_pow2(c > 0 ? c : 1);
+    for (int i = c + tid; i < np; i += NT) buf[i] = kNoKey;
+    __syncthreads();
+    block_bitonic_sort<NT>(buf, np);
+    for (int i = tid; i < k; i += NT) {
+        if (i < c) {
+            idx[i] = key_id(buf[i]);
+            dis[i] = key_dist(buf[i]);
+        } else {
+            idx[i] = -1;
+            dis[i] = __builtin_huge_valf();
+        }
+    }
+}
+
+__global__ __launch_bounds__(NT) void row_topk_kernel(const float *__restrict__ D, int n,
+                                                      int64_t ld, int k,
+                                                      int32_t *__restrict__ idx,
+                                                      float *__restrict__ dis) {
+    __shared__ SelectShared sh;
+    __shared__ uint64_t buf[KBUF];
+    __shared__ int cnt, eqt;
+    const int64_t row = blockIdx.x;
+    const float *Dr = D + row * ld;
+    auto keyfn = [&](int i) { return ckey(Dr[i], i); };
+    const int kk = k < n ? k : n;
+    SelectResult r = block_select<NT>(keyfn, n, kk, sh);
+    int c = collect(keyfn, n, r, buf, &cnt, &eqt);
+    sort_and_emit(buf, c, k, idx + row * k, dis + row * k);
+}
+
+__global__ __launch_bounds__(NT) void merge_kernel(const uint64_t *__restrict__ partial,
+                                                   int ngroups, int kstride,
+                                                   const int32_t *__restrict__ kq, int k,
+                                                   int32_t *__restrict__ out_ids,
+                                                   float *__restrict__ out_dis) {
+    __shared__ SelectShared sh;
+    __shared__ uint64_t buf[KBUF];
+    __shared__ int cnt, eqt;
+    const int q = blockIdx.x;
+    const uint64_t *P = partial + int64_t(q) * ngroups * kstride;
+    const int n = ngroups * kstride;
+    const int kk = kq ? kq[q] : k;
+    auto keyfn = [&](int i) { return P[i]; };
+    SelectResult r = block_select<NT>(keyfn, n, kk, sh);
+    int c = collect(keyfn, n, r, buf, &cnt, &eqt);
+    // rows are kstride wide; entries past this query's k are padding
+    sort_and_emit(buf, c, kstride, out_ids + int64_t(q) * kstride, out_dis + int64_t(q) * kstride);
+}
+
+// rerank: a = base row (f32-accumulated norm), b = query (f64-accumulated norm),
+// as knn_full(2, 1, ktmp, d, k, nearest, query) in sequential/ivf_pq/ivf_search.cpp:130
+__global__ __launch_bounds__(NT) void rerank_kernel(const float *__restrict__ base, int64_t nbase,
+                                                    const float *__restrict__ queries, int d,
+                                                    const int32_t *__restrict__ short_ids,
+                                                    int kl, int k, int32_t *__restrict__ out_ids,
+                                                    float *__restrict__ out_dis) {
+    __shared__ SelectShared sh;
+    __shared__ uint64_t cand[KBUF];
+    __shared__ uint64_t buf[KBUF];
+    __shared__ float qv[512];
+    __shared__ double qn;
+    __shared__ int cnt, eqt;
+    const int tid = threadIdx.x;
+    const int q = blockIdx.x;
+    for (int t = tid; t < d; t += NT) qv[t] = queries[int64_t(q) * d + t];
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int t = 0; t < d; ++t) {
+            float p = qv[t] * qv[t];
+            s = s + double(p);
+        }
+        qn = s;
+    }
+    __syncthreads();
+    const double nb = qn;
+    for (int i = tid; i < kl; i += NT) {
+        int32_t id = short_ids[int64_t(q) * kl + i];
+        uint64_t key = kNoKey;
+        if (id >= 0 && int64_t(id) < nbase) {
+            const float *x = base + int64_t(id) * d;
+            float na = 0.0f, dot = 0.0f;
+            for (int t = 0; t < d; ++t) {
+                float xv = x[t];
+                float p = xv * xv;
+                na = na + p;
+                float pd = xv * qv[t];
+                dot = dot + pd;
+            }
+            float c = float(nb + double(na));
+            float m2 = -2.0f * dot;
+            key = ckey(m2 + c, id);
+        }
+        cand[i] = key;
+    }
+    __syncthreads();
+    auto keyfn = [&](int i) { return cand[i]; };
+    SelectResult r = block_select<NT>(keyfn, kl, k, sh);
+    int c = collect(keyfn, kl, r, buf, &cnt, &eqt);
+    sort_and_emit(buf, c, k, out_ids + int64_t(q) * k, out_dis + int64_t(q) * k);
+}
+
+}  // namespace
+
+hipError_t launch_row_topk(const float *D, int64_t rows, int n, int64_t ld, int k,
+                           int32_t *idx, float *dis, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    if (k < 1 || k > KBUF) return hipErrorInvalidValue;
+    row_topk_kernel<<<unsigned(rows), NT, 0, s>>>(D, n, ld, k, idx, dis);
+    return hipGetLastError();
+}
+
+hipError_t launch_merge(const uint64_t *partial, int nq, int ngroups, int kstride,
+                        const int32_t *kq, int k, int32_t *out_ids, float *out_dis,
+                        hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    if (kstride < 1 || kstride > KBUF) return hipErrorInvalidValue;
+    merge_kernel<<<unsigned(nq), NT, 0, s>>>(partial, ngroups, kstride, kq, k, out_ids, out_dis);
+    return hipGetLastError();
+}
+
+hipError_t launch_rerank(const float *base, int64_t nbase, const float *queries, int nq,
+                         int d, const int32_t *short_ids, int kl, int k,
+                         int32_t *out_ids, float *out_dis, hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    if (kl < 1 || kl > KBUF || k < 1 || k > KBUF || d > 512) return hipErrorInvalidValue;
+    rerank_kernel<<<unsigned(nq), NT, 0, s>>>(base, nbase, queries, d, short_ids, kl, k, out_ids,
+                                              out_dis);
+    return hipGetLastError();
+}
+
+}  // namespace mivf

@@ -1,0 +1,226 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle and the
+golden fixtures, bit for bit (integer, index and FP32 distance outputs)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from mestrado_amd.workloads import D, NCOMP, SIGMA, STREAM_BASE, STREAM_QUERY, STREAM_TRAIN, WORKLOADS
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def same_bits(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.shape == b.shape and a.dtype == b.dtype and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def assert_same_results(ids, dis, rids, rdis, what=""):
+    assert ids.shape == rids.shape, what
+    bad = np.nonzero((ids != rids).any(1) | (dis.view(np.uint32) != rdis.view(np.uint32)).any(1))[0]
+    assert bad.size == 0, f"{what}: {bad.size} rows differ, first {bad[:5]}; " \
+                          f"gpu {ids[bad[0], :5]} {dis[bad[0], :5]} vs oracle {rids[bad[0], :5]} {rdis[bad[0], :5]}"
+
+
+@pytest.fixture(scope="module")
+def c1(orc):
+    c = WORKLOADS["ivfpq_10k_l256_m8"]
+    cent = orc.gen_centres(c["seed"], NCOMP, D)
+    return dict(
+        cfg=c, cent=cent,
+        train=orc.gen_rows(c["seed"], STREAM_TRAIN, 0, c["ntrain"], cent, None, SIGMA),
+        base=orc.gen_rows(c["seed"], STREAM_BASE, 0, c["nbase"], cent, None, SIGMA),
+        query=orc.gen_rows(c["seed"], STREAM_QUERY, 0, c["nquery"], cent, None, SIGMA),
+        gold_q=np.load(os.path.join(GOLD, "c1_quantizers.npz")),
+    )
+
+
+@pytest.fixture(scope="module")
+def c1_index(mivf, orc, c1):
+    """GPU index over the C1 base with the golden quantizers."""
+    q = c1["gold_q"]
+    ix = mivf.Index(q["coarse"], q["pq"])
+    ix.add(c1["base"])
+    return ix
+
+
+def test_generator_bitexact(mivf, orc):
+    for thr_s in (None, 1.1):
+        cent = orc.gen_centres(7, NCOMP, D)
+        assert same_bits(cent, mivf.synth_centres(7, NCOMP, D))
+        thr = None
+        if thr_s:
+            thr = orc.zipf_thresholds(7, NCOMP, thr_s)
+            assert same_bits(thr, mivf.zipf_thresholds(7, NCOMP, thr_s))
+        ref = orc.gen_rows(7, 3, 1000, 3000, cent, thr, SIGMA)
+        got = mivf.synth_rows(7, 3, 1000, 3000, cent, thr, SIGMA).numpy()
+        assert same_bits(ref, got)
+
+
+def test_coarse_topw_bitexact(mivf, orc, c1, c1_index):
+    q = c1["gold_q"]
+    for w in (1, 8, 64, 256):
+        idx, dis = c1_index.coarse(c1["query"], w)
+        ridx, rdis = orc.knn(q["coarse"], c1["query"], w)
+        assert_same_results(idx, dis, ridx, rdis, f"coarse w={w}")
+
+
+def test_encode_bitexact(mivf, orc, c1, c1_index):
+    q = c1["gold_q"]
+    a, c = c1_index.encode(c1["base"])
+    ra, rc = orc.encode(q["coarse"], q["pq"], c1["base"])
+    assert np.array_equal(a, ra)
+    assert np.array_equal(c, rc)
+
+
+def test_lists_match_golden_digests(c1_index):
+    import hashlib
+    man = json.load(open(os.path.join(GOLD, "manifest.json")))
+    off, ids, codes = c1_index.get_lists()
+    for name, arr in (("c1_offsets", off), ("c1_ids", ids), ("c1_codes", codes)):
+        assert hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest() == man["sha256"][name], name
+
+
+def test_kmeans_bitexact(mivf, orc, c1):
+    x = c1["train"][:6000]
+    g = mivf.kmeans(x, 64, 6, 99)
+    r = orc.kmeans(x, 64, 6, 99)
+    assert same_bits(g, r)
+
+
+def test_train_matches_golden(mivf, c1):
+    """GPU ivfpq_new reproduces the oracle-trained C1 quantizers bit for bit."""
+    c = c1["cfg"]
+    coarse, pq = mivf.train(c1["train"], c["nlist"], c["m"], 256, c["niter_coarse"], c["niter_pq"], c["seed"])
+    assert same_bits(coarse, c1["gold_q"]["coarse"])
+    assert same_bits(pq, c1["gold_q"]["pq"])
+
+
+def test_search_golden_c1(c1, c1_index):
+    g = np.load(os.path.join(GOLD, "c1_search.npz"))
+    c = c1["cfg"]
+    ids, dis = c1_index.search(c1["query"], c["w"], c["k"])
+    assert_same_results(ids, dis, g["ids"], g["dis"], "c1 golden")
+
+
+def test_search_reference_quantizers(mivf, orc, c1):
+    """The reference's own trained 256-list quantizers and the siftsmall queries."""
+    import sys
+    sys.path.insert(0, GOLD)
+    from make_golden import load_ref_quantizers, load_fvecs
+    coarse, pq = load_ref_quantizers()
+    ix = mivf.Index(coarse, pq)
+    ix.add(c1["base"])
+    sq = load_fvecs(os.path.join(GOLD, "ref", "siftsmall_query.fvecs"))
+    ids, dis = ix.search(sq, 8, 100)
+    g = np.load(os.path.join(GOLD, "refq_search.npz"))
+    assert_same_results(ids, dis, g["ids"], g["dis"], "reference quantizers")
+
+
+@pytest.mark.parametrize("w,k", [(1, 1), (1, 100), (8, 10), (8, 100), (32, 100), (256, 128), (8, 1024)])
+def test_search_vs_oracle(orc, c1, c1_index, w, k):
+    q = c1["gold_q"]
+    index = (q["coarse"], q["pq"]) + tuple(c1_index.get_lists())
+    ids, dis = c1_index.search(c1["query"], w, k)
+    rids, rdis = orc.search(index, c1["query"], w, k)
+    assert_same_results(ids, dis, rids, rdis, f"w={w} k={k}")
+
+
+@pytest.mark.parametrize("ppg", [1, 3, 8])
+def test_probe_groups_merge(orc, c1, c1_index, ppg):
+    """Splitting the probes of a query over workgroups + merge == one workgroup."""
+    q = c1["gold_q"]
+    index = (q["coarse"], q["pq"]) + tuple(c1_index.get_lists())
+    c1_index.set_option("probes_per_group", ppg)
+    try:
+        ids, dis = c1_index.search(c1["query"], 16, 100)
+    finally:
+        c1_index.set_option("probes_per_group", 0)
+    rids, rdis = orc.search(index, c1["query"], 16, 100)
+    assert_same_results(ids, dis, rids, rdis, f"ppg={ppg}")
+
+
+def test_search_var(orc, c1, c1_index):
+    rng = np.random.default_rng(5)
+    nq = c1["query"].shape[0]
+    w = rng.integers(1, 65, nq).astype(np.int32)
+    k = rng.choice([1, 10, 100], nq).astype(np.int32)
+    q = c1["gold_q"]
+    index = (q["coarse"], q["pq"]) + tuple(c1_index.get_lists())
+    ids, dis = c1_index.search_var(c1["query"], w, k, 100)
+    rids, rdis = orc.search_var(index, c1["query"], w, k, 100)
+    assert_same_results(ids, dis, rids, rdis, "var w/k")
+
+
+def test_search_residual(orc, c1, c1_index):
+    """core_gpu semantics: residuals and list ids supplied by the caller."""
+    q = c1["gold_q"]
+    w, k = 8, 50
+    ci, _ = orc.knn(q["coarse"], c1["query"], w)
+    res = (c1["query"][:, None, :] - q["coarse"][ci]).reshape(-1, D).astype(np.float32)
+    index = (q["coarse"], q["pq"]) + tuple(c1_index.get_lists())
+    ids, dis = c1_index.search_residual(res, ci.reshape(-1), w, k)
+    rids, rdis = orc.search_residual(index, res, ci.reshape(-1), w, k)
+    assert_same_results(ids, dis, rids, rdis, "residual mode")
+
+
+@pytest.mark.parametrize("kl,k", [(256, 100), (1024, 100), (64, 10)])
+def test_rerank_vs_oracle(orc, c1, c1_index, kl, k):
+    q = c1["gold_q"]
+    index = (q["coarse"], q["pq"]) + tuple(c1_index.get_lists())
+    ids, dis = c1_index.search_rerank(c1["query"], 8, kl, k, c1["base"])
+    rids, rdis = orc.search_rerank(index, c1["base"], c1["query"], 8, kl, k)
+    assert_same_results(ids, dis, rids, rdis, f"rerank kl={kl}")
+    if kl == 256:
+        g = np.load(os.path.join(GOLD, "c1_rerank.npz"))
+        assert_same_results(ids, dis, g["ids"], g["dis"], "rerank golden")
+
+
+def test_ties_and_empty_lists(mivf, orc, c1):
+    """Many identical codes (exact distance ties -> lower id first), empty
+    lists, and fewer candidates than k (padding with -1 / +inf)."""
+    q = c1["gold_q"]
+    nlist = q["coarse"].shape[0]
+    rng = np.random.default_rng(11)
+    sizes = rng.integers(0, 40, nlist)
+    sizes[::3] = 0
+    off = np.zeros(nlist + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    n = int(off[-1])
+    ids = rng.permutation(n * 3)[:n].astype(np.int32)
+    codes = rng.integers(0, 4, (n, 8)).astype(np.uint8)  # few distinct codes -> many ties
+    ix = mivf.Index(q["coarse"], q["pq"])
+    ix.set_lists(off, ids, codes)
+    index = (q["coarse"], q["pq"], off, ids, codes)
+    for w, k in ((1, 100), (4, 100), (16, 7), (64, 1000)):
+        g = ix.search(c1["query"], w, k)
+        r = orc.search(index, c1["query"], w, k)
+        assert_same_results(g[0], g[1], r[0], r[1], f"ties w={w} k={k}")
+    # padding rows are (-1, +inf) after the real results
+    g = ix.search(c1["query"], 1, 1000)
+    assert (g[0][:, -1] == -1).all() and np.isinf(g[1][:, -1]).all()
+
+
+def test_int32_codes_upload(mivf, orc, c1, c1_index):
+    """Reference layout (int32 per subquantizer) uploads to the same index."""
+    q = c1["gold_q"]
+    off, ids, codes = c1_index.get_lists()
+    ix = mivf.Index(q["coarse"], q["pq"])
+    ix.set_lists(off, ids, codes.astype(np.int32))
+    a = ix.search(c1["query"], 8, 100)
+    b = c1_index.search(c1["query"], 8, 100)
+    assert_same_results(a[0], a[1], b[0], b[1], "int32 codes")
+    bad = codes.astype(np.int32)
+    bad[0, 0] = 256
+    with pytest.raises(mivf.MivfError):
+        ix.set_lists(off, ids, bad)
+
+
+def test_bruteforce_nn(mivf, orc, c1):
+    gi, gd = mivf.bruteforce_nn(c1["base"], c1["query"])
+    ri, rd = orc.bruteforce_f64(c1["base"], c1["query"], 1)
+    assert np.array_equal(gi, ri[:, 0])
+    assert np.array_equal(gd, rd[:, 0])
