@@ -149,6 +149,29 @@ int mivf_zipf_thresholds(uint64_t seed, int ncomp, double s, uint64_t *thr);
 int mivf_bruteforce_nn(const float *d_base, int64_t nb, const float *d_q, int nq, int d,
                        int32_t *d_ids, double *d_dis);
 
+/* ---- reference on-disk formats ---------------------------------------------
+ * Centroid files: write_cent/read_cent (parallel/ivf_pq/ivf_training.cpp:92-143):
+ * header = 8 int32 {nsq, ks, ds, ks*ds, nsq, coarsek, coarsek, d}, cent =
+ * float32 [nsq][ks][ds], coa = float32 [coarsek][d].                        */
+int mivf_write_cent(const char *header, const char *cent, const char *coa, int nsq, int ks,
+                    int ds, int coarsek, int d, const float *pq, const float *coarse);
+int mivf_read_cent_header(const char *header, int *nsq, int *ks, int *ds, int *coarsek, int *d);
+int mivf_read_cent(const char *header, const char *cent, const char *coa, float *pq,
+                   float *coarse);
+/* Per-list files <dir>/<j> (parallel2/genivf.cpp:62-86, read_ivf
+ * parallel2/ivf_pq/ivf_search.cpp:293-321) and the single-file layout
+ * (parallel/ivf_pq/ivf_search.cpp:396-441).  Codes are int32 on disk and
+ * uint8 in memory.  Readers always fill offsets [coarsek+1]; pass NULL
+ * ids/codes to learn the sizes first; cap = capacity of ids in vectors.    */
+int mivf_write_ivf_dir(const char *dir, int coarsek, int m, const int64_t *offsets,
+                       const int32_t *ids, const uint8_t *codes);
+int mivf_read_ivf_dir(const char *dir, int coarsek, int m, int64_t *offsets, int32_t *ids,
+                      uint8_t *codes, int64_t cap);
+int mivf_write_ivf_file(const char *path, int coarsek, int m, const int64_t *offsets,
+                        const int32_t *ids, const uint8_t *codes);
+int mivf_read_ivf_file(const char *path, int coarsek, int m, int64_t *offsets, int32_t *ids,
+                       uint8_t *codes, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif
