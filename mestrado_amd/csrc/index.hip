@@ -104,7 +104,8 @@ int upload_lists(mivf_index *ix) {
     const int64_t n = off[ix->nlist];
     MIVF_TRY(ix->off.ensure(sizeof(int64_t) * (ix->nlist + 1)));
     MIVF_TRY(ix->ids.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1)));
-    MIVF_TRY(ix->codes.ensure(size_t(ix->m) * std::max<int64_t>(n, 1) + 16));
+    // 64 bytes of tail padding: the scan issues clamped, branch-free code loads
+    MIVF_TRY(ix->codes.ensure(size_t(ix->m) * std::max<int64_t>(n, 1) + 64));
     MIVF_CHECK_HIP(hipMemcpy(ix->off.p, off.data(), sizeof(int64_t) * (ix->nlist + 1), hipMemcpyHostToDevice));
     if (n > 0) {
         std::vector<int32_t> ids(n);

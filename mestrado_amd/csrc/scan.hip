@@ -55,23 +55,44 @@ struct CodeVec<32> {
     }
 };
 
-template <int CAP>
-struct ScanShared {
-    uint64_t cand[CAP];
+// Control words of one workgroup.
+struct ScanCtl {
     SelectShared sel;
-    int cnt;
+    int inc[3];  // per-step append counters, rotated so one barrier per step suffices
     int newcnt;
     int eqt;
 };
 
-// Shrink cand[0..cnt) to the kk smallest keys; returns the new threshold.
+constexpr int TB = 2;  // tables built per phase (probes scanned per phase)
+
+// Dynamic LDS layout.  The tables sit at LDS address 0 so that a gather is
+// one ds_read_b32 at code*4 with the table row as immediate offset.
+template <int M, int CAP>
+struct Lay {
+    static constexpr size_t tab = 0;  // float [TB][M][256]
+    static constexpr size_t cand = tab + size_t(TB) * M * 256 * 4;
+    static constexpr size_t qv = cand + size_t(CAP) * 8;
+    static constexpr size_t resid = qv + D * 4;          // float [TB][D]
+    static constexpr size_t rnorm = resid + TB * D * 4;  // float [TB][M]
+    static constexpr size_t ctl = (rnorm + TB * M * 4 + 15) & ~size_t(15);
+    static constexpr size_t total = ctl + sizeof(ScanCtl);
+};
+
+// Candidates carry (distance key, list position) while scanning; positions
+// become ids here, right before a selection.  Then shrink cand[0..c) to the
+// kk smallest (distance, id) keys.  Returns the new count.
 template <int CAP>
-__device__ uint64_t compact(ScanShared<CAP> &S, int kk) {
+__device__ int compact(ScanCtl &C, uint64_t *cand, int c, int conv, int kk,
+                       const int32_t *__restrict__ list_ids, uint64_t *thr) {
     constexpr int PER = CAP / NT;
     const int tid = threadIdx.x;
-    const int c = S.cnt;
-    uint64_t *cand = S.cand;
-    SelectResult r = block_select<NT>([&](int i) { return cand[i]; }, c, kk, S.sel);
+    for (int i = conv + tid; i < c; i += NT) {
+        const uint64_t k = cand[i];
+        cand[i] = (k & 0xFFFFFFFF00000000ull) | uint32_t(list_ids[uint32_t(k)]);
+    }
+    __syncthreads();
+    if (c <= kk) return c;
+    SelectResult r = block_select<NT>([&](int i) { return cand[i]; }, c, kk, C.sel);
     uint64_t mine[PER];
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
@@ -79,8 +100,8 @@ __device__ uint64_t compact(ScanShared<CAP> &S, int kk) {
         mine[e] = i < c ? cand[i] : kNoKey;
     }
     if (tid == 0) {
-        S.newcnt = 0;
-        S.eqt = 0;
+        C.newcnt = 0;
+        C.eqt = 0;
     }
     __syncthreads();
 #pragma unroll
@@ -88,20 +109,47 @@ __device__ uint64_t compact(ScanShared<CAP> &S, int kk) {
         uint64_t key = mine[e];
         if (key == kNoKey) continue;
         bool take = key < r.thr;
-        if (!take && r.eq > 0 && key == r.thr) take = atomicAdd(&S.eqt, 1) < r.eq;
+        if (!take && r.eq > 0 && key == r.thr) take = atomicAdd(&C.eqt, 1) < r.eq;
         if (take) {
-            const int pos = atomicAdd(&S.newcnt, 1);
+            const int pos = atomicAdd(&C.newcnt, 1);
             if (pos < CAP) cand[pos] = key;
         }
     }
     __syncthreads();
-    if (tid == 0) S.cnt = min(S.newcnt, CAP);
+    const int nc = min(C.newcnt, CAP);
     __syncthreads();
-    return r.thr;
+    *thr = r.thr;
+    return nc;
 }
 
+// Branch-free code loads: lanes past the end re-read the last vector (the
+// device code buffer carries tail padding for empty lists).
+template <int M, int U>
+__device__ inline void load_step(const uint8_t *lc, int base, int len, uint32_t (&cw)[U][CodeVec<M>::W]) {
+    const int last = len > 0 ? len - 1 : 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int v = min(base + u * NT + int(threadIdx.x), last);
+        CodeVec<M>::load(lc + int64_t(v) * M, cw[u]);
+    }
+}
+
+// The probed lists of one workgroup, in coarse order, skipping invalid ids.
+struct ProbeSeq {
+    const ScanArgs &a;
+    int q, p, p1;
+    __device__ int next() {  // next valid probe index >= p, or p1
+        while (p < p1) {
+            const int l = a.probes[int64_t(q) * a.wstride + p];
+            if (l >= 0 && l < a.nlist) return p;
+            ++p;
+        }
+        return p1;
+    }
+};
+
 template <int M, int KM>
-__global__ __launch_bounds__(NT) void scan_kernel(ScanArgs a) {
+__global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
     constexpr int DS = D / M;
     constexpr int HM = M / 2;
     constexpr int CW = CodeVec<M>::W;
@@ -109,12 +157,14 @@ __global__ __launch_bounds__(NT) void scan_kernel(ScanArgs a) {
     constexpr int STEP = NT * U;
     constexpr int CAP = KM <= 128 ? 2048 : 4096;
     static_assert(CAP - STEP >= KM, "candidate buffer too small");
-
-    __shared__ float tab[M * 256];
-    __shared__ float qv[D];
-    __shared__ float resid[D];
-    __shared__ float rnorm[M];
-    __shared__ ScanShared<CAP> S;
+    using L = Lay<M, CAP>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *tab = reinterpret_cast<float *>(smem + L::tab);
+    uint64_t *cand = reinterpret_cast<uint64_t *>(smem + L::cand);
+    float *qv = reinterpret_cast<float *>(smem + L::qv);
+    float *resid = reinterpret_cast<float *>(smem + L::resid);
+    float *rnorm = reinterpret_cast<float *>(smem + L::rnorm);
+    ScanCtl &C = *reinterpret_cast<ScanCtl *>(smem + L::ctl);
 
     const int tid = threadIdx.x;
     const int q = blockIdx.x / a.ngroups;
@@ -123,133 +173,185 @@ __global__ __launch_bounds__(NT) void scan_kernel(ScanArgs a) {
     const int kq = a.kq ? a.kq[q] : a.k;
     const int p0 = g * a.ppg;
     const int p1 = min(wq, p0 + a.ppg);
-
     const int j = tid & 255;
-    const int h = tid >> 8;
+    const int h = __builtin_amdgcn_readfirstlane(tid >> 8);  // wave-uniform half index
     const bool jok = j < a.ks;  // table rows past ks are never indexed by valid codes
-    float preg[HM][DS];
-    double pnreg[HM];
-#pragma unroll
-    for (int s2 = 0; s2 < HM; ++s2) {
-        const int s = h * HM + s2;
-        const float *src = a.pq + (int64_t(s) * a.ks + j) * DS;
-#pragma unroll
-        for (int t = 0; t < DS; ++t) preg[s2][t] = jok ? src[t] : 0.0f;
-        pnreg[s2] = jok ? a.pq_norm[s * a.ks + j] : 0.0;
-    }
+
     if (!a.resid_mode && tid < D) qv[tid] = a.queries[int64_t(q) * D + tid];
-    if (tid == 0) S.cnt = 0;
+    if (tid < 3) C.inc[tid] = 0;
+    int fill = 0;      // candidates held at the start of a step (uniform)
+    int conv = 0;      // cand[0..conv) already carry ids
+    int step = 0;      // global step counter (uniform)
     uint64_t thr = kNoKey;
     uint32_t thr_hi = 0xFFFFFFFFu;
     __syncthreads();
 
-    for (int p = p0; p < p1; ++p) {
-        const int l = a.probes[int64_t(q) * a.wstride + p];
-        if (l < 0 || l >= a.nlist) continue;  // uniform: every lane reads the same probe
-        const int64_t beg = a.list_off[l];
-        const int len = int(a.list_off[l + 1] - beg);
-        if (tid < D) {
-            resid[tid] = a.resid_mode ? a.queries[(int64_t(q) * a.wstride + p) * D + tid]
-                                      : qv[tid] - a.coarse[int64_t(l) * D + tid];
-        }
-        __syncthreads();
-        if (tid < M) {
-            float s = 0.0f;
+    ProbeSeq seq{a, q, p0, p1};
+    uint32_t cur[U][CW], nxt[U][CW];
+    for (;;) {
+        // ---- pick up to TB probes
+        int pp[TB], ll[TB], len[TB];
+        int64_t beg[TB];
+        int nb = 0;
 #pragma unroll
-            for (int t = 0; t < DS; ++t) {
-                float pr = resid[tid * DS + t] * resid[tid * DS + t];
-                s = s + pr;
+        for (int t = 0; t < TB; ++t) {
+            pp[t] = seq.next();
+            ll[t] = 0;
+            beg[t] = 0;
+            len[t] = 0;
+            if (pp[t] < p1) {
+                ll[t] = a.probes[int64_t(q) * a.wstride + pp[t]];
+                beg[t] = a.list_off[ll[t]];
+                len[t] = int(a.list_off[ll[t] + 1] - beg[t]);
+                seq.p = pp[t] + 1;
+                nb = t + 1;
             }
-            rnorm[tid] = s;
+        }
+        if (nb == 0) break;
+
+        // ---- residuals of the batch, then the first code step of list 0
+        float cval[TB];
+#pragma unroll
+        for (int t = 0; t < TB; ++t) {
+            const int64_t rrow = a.resid_mode ? (int64_t(q) * a.wstride + pp[t]) * D : int64_t(ll[t]) * D;
+            const float *rbase = a.resid_mode ? a.queries : a.coarse;
+            cval[t] = t < nb ? rbase[rrow + (tid & (D - 1))] : 0.0f;
+        }
+        load_step<M, U>(a.codes + beg[0] * M, 0, len[0], cur);
+        if (tid < D) {  // waves 0 and 1, whole wavefronts
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                const float r = a.resid_mode ? cval[t] : qv[tid] - cval[t];
+                resid[t * D + tid] = r;
+                // f32 norm of each sub-vector, summed in dimension order
+                const float sq = r * r;
+                const int g0 = (tid & 63) & ~(DS - 1);
+                float acc = 0.0f;
+#pragma unroll
+                for (int i = 0; i < DS; ++i) {
+                    const float v = __shfl(sq, g0 + i, 64);
+                    acc = acc + v;
+                }
+                if ((tid & (DS - 1)) == 0) rnorm[t * M + tid / DS] = acc;
+            }
         }
         __syncthreads();
+
+        // ---- TB tables: thread (j, h) streams P[s][j] of its subquantizers
+        //      from L2 once for the whole batch
+        // lane offset kept opaque so the per-subquantizer addresses are not
+        // hoisted out of the batch loop into long-lived 64-bit registers
+        unsigned jj = jok ? unsigned(j) : 0u;
+        asm volatile("" : "+v"(jj));
 #pragma unroll
         for (int s2 = 0; s2 < HM; ++s2) {
             const int s = h * HM + s2;
-            float dot = 0.0f;
+            float pr[DS];
+            const float *src = a.pq + size_t(s) * a.ks * DS;  // wave-uniform row base
 #pragma unroll
-            for (int t = 0; t < DS; ++t) {
-                float pr = resid[s * DS + t] * preg[s2][t];
-                dot = dot + pr;
+            for (int i = 0; i < DS; ++i) pr[i] = src[jj * DS + i];
+            const double pnv = a.pq_norm[size_t(s) * a.ks + jj];
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                float dot = 0.0f;
+#pragma unroll
+                for (int i = 0; i < DS; ++i) {
+                    const float prod = resid[t * D + s * DS + i] * pr[i];
+                    dot = dot + prod;
+                }
+                const float c = float(pnv + double(rnorm[t * M + s]));
+                const float m2 = -2.0f * dot;
+                tab[(t * M + s) * 256 + j] = jok ? m2 + c : __builtin_huge_valf();
             }
-            float c = float(pnreg[s2] + double(rnorm[s]));
-            float m2 = -2.0f * dot;
-            tab[s * 256 + j] = jok ? m2 + c : __builtin_huge_valf();
+            __builtin_amdgcn_sched_barrier(0);  // one subquantizer's operands live at a time
         }
         __syncthreads();
 
-        const uint8_t *lc = a.codes + beg * M;
-        const int32_t *lid = a.list_ids + beg;
-        for (int base = 0; base < len; base += STEP) {
-            uint32_t cw[U][CW];
+        // ---- scan the batch's lists back to back; the prefetch runs one
+        //      step ahead, across list boundaries
+        int cur_t = 0;  // list whose first step `cur` holds (loaded before the tables)
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int v = base + u * NT + tid;
-                if (v < len) {
-                    CodeVec<M>::load(lc + int64_t(v) * M, cw[u]);
-                } else {
-#pragma unroll
-                    for (int w = 0; w < CW; ++w) cw[u][w] = 0;
-                }
+        for (int t = 0; t < TB; ++t) {
+            if (t >= nb) break;
+            const float *tt = tab + t * M * 256;
+            const uint8_t *lc = a.codes + beg[t] * M;
+            if (cur_t != t) {  // the previous list was empty: nothing prefetched this one
+                load_step<M, U>(lc, 0, len[t], cur);
+                cur_t = t;
             }
+            for (int base = 0; base < len[t]; base += STEP) {
+                const int rot = step % 3;
+                if (tid == 0) C.inc[(step + 1) % 3] = 0;
+                bool next_list = false;
+                if (base + STEP < len[t]) {
+                    load_step<M, U>(lc, base + STEP, len[t], nxt);
+                } else if (t + 1 < nb) {
+                    load_step<M, U>(a.codes + beg[t + 1] * M, 0, len[t + 1], nxt);
+                    next_list = true;
+                }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                float acc = 0.0f;
+                for (int u = 0; u < U; ++u) {
+                    float acc = 0.0f;
 #pragma unroll
-                for (int w = 0; w < CW; ++w) {
+                    for (int w = 0; w < CW; ++w) {
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        const uint32_t code = (cw[u][w] >> (8 * b)) & 255u;
-                        acc = acc + tab[(w * 4 + b) * 256 + code];
+                        for (int b = 0; b < 4; ++b) {
+                            const uint32_t code = (cur[u][w] >> (8 * b)) & 255u;
+                            acc = acc + tt[(w * 4 + b) * 256 + code];
+                        }
+                    }
+                    const int v = base + u * NT + tid;
+                    const uint32_t kh = fkey(acc);
+                    // ties with the threshold distance pass too; ids are
+                    // compared exactly at the next selection
+                    const bool pass = (v < len[t]) && kh <= thr_hi;
+                    const uint64_t mask = __ballot(pass);
+                    if (mask) {
+                        const int lane = __lane_id();
+                        const int before = __popcll(mask & ((uint64_t(1) << lane) - 1));
+                        int pos = 0;
+                        if (lane == 0) pos = atomicAdd(&C.inc[rot], __popcll(mask));
+                        pos = fill + __shfl(pos, 0);
+                        if (pass && pos + before < CAP)
+                            cand[pos + before] = (uint64_t(kh) << 32) | uint32_t(beg[t] + v);
                     }
                 }
-                const int v = base + u * NT + tid;
-                const uint32_t kh = fkey(acc);
-                bool pass = (v < len) && kh <= thr_hi;
-                uint64_t key = 0;
-                if (pass) {
-                    key = (uint64_t(kh) << 32) | uint32_t(lid[v]);
-                    pass = key < thr;
+                __syncthreads();
+                fill += C.inc[rot];
+                ++step;
+                if (fill > CAP - STEP) {
+                    fill = compact<CAP>(C, cand, fill, conv, kq, a.list_ids, &thr);
+                    conv = fill;
+                    thr_hi = uint32_t(thr >> 32);
                 }
-                const uint64_t mask = __ballot(pass);
-                if (mask) {
-                    const int lane = __lane_id();
-                    const int before = __popcll(mask & ((uint64_t(1) << lane) - 1));
-                    int pos = 0;
-                    if (lane == 0) pos = atomicAdd(&S.cnt, __popcll(mask));
-                    pos = __shfl(pos, 0);
-                    if (pass && pos + before < CAP) S.cand[pos + before] = key;
-                }
-            }
-            __syncthreads();
-            const int cur = S.cnt;
-            __syncthreads();  // every lane has read cnt before the next appends
-            if (cur > CAP - STEP) {
-                thr = compact<CAP>(S, kq);
-                thr_hi = uint32_t(thr >> 32);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int w = 0; w < CW; ++w) cur[u][w] = nxt[u][w];
+                if (next_list) cur_t = t + 1;
             }
         }
-        __syncthreads();
+        __syncthreads();  // tables and residuals are rewritten by the next batch
     }
 
-    // final selection
-    if (S.cnt > kq) compact<CAP>(S, kq);
-    const int c = S.cnt;
+    // final selection (also resolves the last positions to ids)
+    fill = compact<CAP>(C, cand, fill, conv, kq, a.list_ids, &thr);
+    const int c = fill;
     if (a.ngroups > 1) {
         uint64_t *dst = a.partial + (int64_t(q) * a.ngroups + g) * a.kstride;
-        for (int i = tid; i < a.kstride; i += NT) dst[i] = i < c ? S.cand[i] : kNoKey;
+        for (int i = tid; i < a.kstride; i += NT) dst[i] = i < c ? cand[i] : kNoKey;
         return;
     }
     const int np = next_pow2(c > 0 ? c : 1);
-    for (int i = c + tid; i < np; i += NT) S.cand[i] = kNoKey;
+    for (int i = c + tid; i < np; i += NT) cand[i] = kNoKey;
     __syncthreads();
-    block_bitonic_sort<NT>(S.cand, np);
+    block_bitonic_sort<NT>(cand, np);
     int32_t *oi = a.out_ids + int64_t(q) * a.kstride;
     float *od = a.out_dis + int64_t(q) * a.kstride;
     for (int i = tid; i < a.kstride; i += NT) {
         if (i < c) {
-            oi[i] = key_id(S.cand[i]);
-            od[i] = key_dist(S.cand[i]);
+            oi[i] = key_id(cand[i]);
+            od[i] = key_dist(cand[i]);
         } else {
             oi[i] = -1;
             od[i] = __builtin_huge_valf();
@@ -257,14 +359,25 @@ __global__ __launch_bounds__(NT) void scan_kernel(ScanArgs a) {
     }
 }
 
+template <int M, int KM>
+hipError_t launch_mk(const ScanArgs &a, hipStream_t s) {
+    constexpr int CAP = KM <= 128 ? 2048 : 4096;
+    constexpr size_t lds = Lay<M, CAP>::total;
+    static bool attr_set = false;  // raise the dynamic-LDS limit once per instantiation
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&scan_kernel<M, KM>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const unsigned grid = unsigned(int64_t(a.nq) * a.ngroups);
+    scan_kernel<M, KM><<<grid, NT, lds, s>>>(a);
+    return hipGetLastError();
+}
+
 template <int M>
 hipError_t launch_m(const ScanArgs &a, hipStream_t s) {
-    const unsigned grid = unsigned(int64_t(a.nq) * a.ngroups);
-    if (a.kstride <= 128)
-        scan_kernel<M, 128><<<grid, NT, 0, s>>>(a);
-    else
-        scan_kernel<M, 1024><<<grid, NT, 0, s>>>(a);
-    return hipGetLastError();
+    return a.kstride <= 128 ? launch_mk<M, 128>(a, s) : launch_mk<M, 1024>(a, s);
 }
 
 }  // namespace

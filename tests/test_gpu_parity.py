@@ -224,3 +224,55 @@ def test_bruteforce_nn(mivf, orc, c1):
     ri, rd = orc.bruteforce_f64(c1["base"], c1["query"], 1)
     assert np.array_equal(gi, ri[:, 0])
     assert np.array_equal(gd, rd[:, 0])
+
+
+@pytest.fixture(scope="module")
+def zipf16(mivf, orc):
+    """Headline-shaped (Zipf s=1.1 mixture, m=16) index at reduced size."""
+    seed = 103
+    cent = orc.gen_centres(seed, NCOMP, D)
+    thr = orc.zipf_thresholds(seed, NCOMP, 1.1)
+    train = orc.gen_rows(seed, STREAM_TRAIN, 0, 40_000, cent, thr, SIGMA)
+    base = mivf.synth_rows(seed, STREAM_BASE, 0, 300_000, cent, thr, SIGMA)
+    query = orc.gen_rows(seed, STREAM_QUERY, 0, 300, cent, thr, SIGMA)
+    coarse, pq = mivf.train(train, 512, 16, 256, 8, 8, seed)
+    ix = mivf.Index(coarse, pq)
+    ix.add(base)
+    index = (coarse, pq) + tuple(ix.get_lists())
+    return ix, index, query
+
+
+@pytest.mark.parametrize("w,k,ppg", [(32, 100, 0), (32, 100, 32), (64, 10, 4), (1, 1, 0), (200, 128, 0)])
+def test_search_m16_zipf(orc, zipf16, w, k, ppg):
+    ix, index, query = zipf16
+    ix.set_option("probes_per_group", ppg)
+    try:
+        ids, dis = ix.search(query, w, k)
+    finally:
+        ix.set_option("probes_per_group", 0)
+    rids, rdis = orc.search(index, query, w, k)
+    assert_same_results(ids, dis, rids, rdis, f"m16 zipf w={w} k={k} ppg={ppg}")
+
+
+def test_search_m16_rerank_and_var(orc, zipf16, mivf):
+    ix, index, query = zipf16
+    rng = np.random.default_rng(21)
+    nq = query.shape[0]
+    w = rng.integers(1, 65, nq).astype(np.int32)
+    k = rng.choice([1, 10, 100], nq).astype(np.int32)
+    ids, dis = ix.search_var(query, w, k, 100)
+    rids, rdis = orc.search_var(index, query, w, k, 100)
+    assert_same_results(ids, dis, rids, rdis, "m16 var")
+
+
+@pytest.mark.parametrize("m", [4, 32])
+def test_search_other_m(mivf, orc, c1, m):
+    q = c1["gold_q"]
+    coarse, pq = mivf.train(c1["train"][:12000], 64, m, 256, 4, 4, 77)
+    ix = mivf.Index(coarse, pq)
+    ix.add(c1["base"])
+    index = (coarse, pq) + tuple(ix.get_lists())
+    for w, k in ((4, 100), (16, 1000)):
+        ids, dis = ix.search(c1["query"], w, k)
+        rids, rdis = orc.search(index, c1["query"], w, k)
+        assert_same_results(ids, dis, rids, rdis, f"m={m} w={w} k={k}")
