@@ -34,6 +34,8 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=400, help="queries in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gt-queries", type=int, default=1000)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="build, run the timed steps and a 1 GiB device copy (counter calibration), then exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -90,6 +92,15 @@ def main():
     M.synchronize()
     elapsed = time.perf_counter() - t_start
     ix.set_option("profile", 0)
+    if args.profile_only:
+        # a plain device-to-device copy of known size: the reference point
+        # that scales the HBM byte counters of the search kernels
+        src = M.DeviceArray((1 << 28,), np.float32)
+        dst = M.DeviceArray((1 << 28,), np.float32)
+        M.lib().mivf_memcpy_dtod(dst.ptr, src.ptr, src.nbytes)
+        M.synchronize()
+        log(f"profile-only: {args.steps} steps, {elapsed * 1000 / args.steps:.3f} ms/step, copy 1 GiB")
+        return
     ms_per_step = elapsed * 1000.0 / args.steps
     stage /= args.steps
     qps = nq * args.steps / elapsed

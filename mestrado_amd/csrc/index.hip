@@ -84,6 +84,8 @@ struct mivf_index {
     DevBuf off, ids, codes;
     // scratch
     DevBuf qn, dmat, probes, coadis, partial, wq, kq, tmp_ids, tmp_dis, keys, res, resn, assign;
+    DevBuf errflag;          // kernel-reported precondition failures (int)
+    bool scan_checked = false;
     // options
     int ppg_opt = 0;
     bool profile = false;
@@ -372,6 +374,7 @@ int search_core(mivf_index *ix, const float *d_q, int nq, int wmax, const int32_
     a.ppg = ppg;
     a.ngroups = ngroups;
     a.resid_mode = d_probes_in ? 1 : 0;
+    a.err = ix->errflag.as<int>();
     if (ngroups > 1) {
         MIVF_TRY(ix->partial.ensure(sizeof(uint64_t) * size_t(nq) * ngroups * kstride));
         a.partial = ix->partial.as<uint64_t>();
@@ -380,6 +383,13 @@ int search_core(mivf_index *ix, const float *d_q, int nq, int wmax, const int32_
         a.out_dis = d_dis;
     }
     MIVF_CHECK_HIP(launch_scan(a, ix->m, st));
+    if (!ix->scan_checked) {  // once per index: the kernel's LDS-layout precondition
+        int err = 0;
+        MIVF_CHECK_HIP(hipStreamSynchronize(st));
+        MIVF_CHECK_HIP(hipMemcpy(&err, ix->errflag.p, sizeof(int), hipMemcpyDeviceToHost));
+        MIVF_REQUIRE(err == 0, kHip, "scan kernel: dynamic LDS block not at address 0 (results invalid)");
+        ix->scan_checked = true;
+    }
     MIVF_TRY(tm.mark());
     if (ngroups > 1)
         MIVF_CHECK_HIP(launch_merge(ix->partial.as<uint64_t>(), nq, ngroups, kstride, d_kq, kuni, d_ids,
@@ -517,8 +527,13 @@ int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, cons
         (rc = ix->coarse_n32.ensure(sizeof(float) * nlist)) ||
         (rc = ix->pq.ensure(sizeof(float) * size_t(ks) * d)) ||
         (rc = ix->pq_n64.ensure(sizeof(double) * size_t(m) * ks)) ||
-        (rc = ix->pq_n32.ensure(sizeof(float) * size_t(m) * ks)))
+        (rc = ix->pq_n32.ensure(sizeof(float) * size_t(m) * ks)) ||
+        (rc = ix->errflag.ensure(sizeof(int))))
         return fail(rc);
+    if (hipMemset(ix->errflag.p, 0, sizeof(int)) != hipSuccess) {
+        set_error("index_create: hipMemset failed");
+        return fail(kHip);
+    }
     if (hipMemcpy(ix->coarse.p, coarse, sizeof(float) * size_t(nlist) * d, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(ix->pq.p, pq, sizeof(float) * size_t(ks) * d, hipMemcpyHostToDevice) != hipSuccess ||
         launch_norm_f32(ix->coarse.as<float>(), nlist, d, d, ix->coarse_n32.as<float>(), nullptr) != hipSuccess ||

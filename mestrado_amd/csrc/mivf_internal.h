@@ -125,6 +125,7 @@ struct ScanArgs {
     int ppg;                  // probes per group
     int ngroups;
     int resid_mode;           // 1: queries holds residuals, row = q*wstride + p
+    int *err;                 // set non-zero by the kernel on a violated precondition
     uint64_t *partial;        // [nq][ngroups][kstride] when ngroups > 1
     int32_t *out_ids;         // [nq][kstride] when ngroups == 1
     float *out_dis;

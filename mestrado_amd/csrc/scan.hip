@@ -78,9 +78,11 @@ struct Lay {
     static constexpr size_t total = ctl + sizeof(ScanCtl);
 };
 
-// Candidates carry (distance key, list position) while scanning; positions
-// become ids here, right before a selection.  Then shrink cand[0..c) to the
-// kk smallest (distance, id) keys.  Returns the new count.
+// Candidates carry (distance key, list position) while scanning, so the hot
+// loop issues no id loads; cand[conv..c) become (distance, id) keys here,
+// right before a selection.  Then shrink cand[0..c) to the kk smallest keys.
+// Returns the new count; *thr receives the new filter threshold.  All threads
+// call with the same c and conv.
 template <int CAP>
 __device__ int compact(ScanCtl &C, uint64_t *cand, int c, int conv, int kk,
                        const int32_t *__restrict__ list_ids, uint64_t *thr) {
@@ -123,15 +125,52 @@ __device__ int compact(ScanCtl &C, uint64_t *cand, int c, int conv, int kk,
 }
 
 // Branch-free code loads: lanes past the end re-read the last vector (the
-// device code buffer carries tail padding for empty lists).
+// device code buffer carries tail padding for empty lists).  lc is
+// wave-uniform; the lane offset is 32-bit.
 template <int M, int U>
 __device__ inline void load_step(const uint8_t *lc, int base, int len, uint32_t (&cw)[U][CodeVec<M>::W]) {
     const int last = len > 0 ? len - 1 : 0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int v = min(base + u * NT + int(threadIdx.x), last);
-        CodeVec<M>::load(lc + int64_t(v) * M, cw[u]);
+        const uint32_t v = uint32_t(min(base + u * NT + int(threadIdx.x), last));
+        CodeVec<M>::load(lc + v * uint32_t(M), cw[u]);
     }
+}
+
+typedef __attribute__((address_space(3))) const float lds_cfloat;
+
+// LDS read at an integer LDS byte address; with a compile-time row offset
+// the byte extraction becomes one SDWA shift and the row an immediate.
+__device__ inline float lds_at(uint32_t addr) {
+    return *reinterpret_cast<lds_cfloat *>(uintptr_t(addr));
+}
+
+// ADC of U vectors against the LDS table at byte address tbase:
+// acc = 0f; acc = fl32(acc + T[s][c_s]), s ascending.  Wavefronts whose
+// vectors all lie past the list end (wave0 + u*NT >= len) skip the gathers.
+template <int M, int U>
+__device__ inline void adc_step(const uint32_t (&cw)[U][CodeVec<M>::W], uint32_t tbase, int wave_v0,
+                                int len, float (&acc)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        acc[u] = __builtin_huge_valf();
+        if (wave_v0 + u * NT >= len) continue;  // wave-uniform
+        float s = 0.0f;
+#pragma unroll
+        for (int w = 0; w < CodeVec<M>::W; ++w) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t off4 = ((cw[u][w] >> (8 * b)) & 255u) << 2;
+                s = s + lds_at(tbase + off4 + uint32_t((w * 4 + b) * 1024));
+            }
+        }
+        acc[u] = s;
+    }
+}
+
+// Float whose order key is `kh` (+inf for keys at or past +inf).
+__device__ inline float key_hi_dist(uint32_t kh) {
+    return kh >= 0xFF800000u ? __builtin_huge_valf() : key_dist(uint64_t(kh) << 32);
 }
 
 // The probed lists of one workgroup, in coarse order, skipping invalid ids.
@@ -159,7 +198,7 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
     static_assert(CAP - STEP >= KM, "candidate buffer too small");
     using L = Lay<M, CAP>;
     extern __shared__ __align__(16) unsigned char smem[];
-    float *tab = reinterpret_cast<float *>(smem + L::tab);
+    float *tab = reinterpret_cast<float *>(smem + L::tab);  // written here, read through lds_at()
     uint64_t *cand = reinterpret_cast<uint64_t *>(smem + L::cand);
     float *qv = reinterpret_cast<float *>(smem + L::qv);
     float *resid = reinterpret_cast<float *>(smem + L::resid);
@@ -167,6 +206,14 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
     ScanCtl &C = *reinterpret_cast<ScanCtl *>(smem + L::ctl);
 
     const int tid = threadIdx.x;
+    // The kernel declares no static LDS, so its dynamic block starts at LDS
+    // address 0; the gathers below address the tables absolutely.  If that
+    // ever stops holding, report it (the host checks) instead of reading
+    // wrong tables.
+    if (uint32_t(uintptr_t((__attribute__((address_space(3))) unsigned char *)(smem))) != 0) {
+        if (tid == 0) atomicOr(a.err, 1);
+        return;
+    }
     const int q = blockIdx.x / a.ngroups;
     const int g = blockIdx.x - q * a.ngroups;
     const int wq = a.wq ? a.wq[q] : a.w;
@@ -183,11 +230,12 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
     int conv = 0;      // cand[0..conv) already carry ids
     int step = 0;      // global step counter (uniform)
     uint64_t thr = kNoKey;
-    uint32_t thr_hi = 0xFFFFFFFFu;
+    float thr_f = __builtin_huge_valf();  // distances <= thr_f may enter the top-k
     __syncthreads();
 
     ProbeSeq seq{a, q, p0, p1};
-    uint32_t cur[U][CW], nxt[U][CW];
+    constexpr int RD = 3;  // register ring depth: loads run RD-1 steps ahead
+    uint32_t ring[RD][U][CW];
     for (;;) {
         // ---- pick up to TB probes
         int pp[TB], ll[TB], len[TB];
@@ -217,7 +265,18 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
             const float *rbase = a.resid_mode ? a.queries : a.coarse;
             cval[t] = t < nb ? rbase[rrow + (tid & (D - 1))] : 0.0f;
         }
-        load_step<M, U>(a.codes + beg[0] * M, 0, len[0], cur);
+        // flat step sequence over the batch's lists: step k covers STEP
+        // vectors of list 0 (k < ns0) or list 1; loads run RD-1 steps ahead
+        const int ns0 = (len[0] + STEP - 1) / STEP;
+        const int S = ns0 + (nb > 1 ? (len[1] + STEP - 1) / STEP : 0);
+        auto issue = [&](int k, uint32_t (&dst)[U][CW]) {
+            // unconditional (clamped) so the compiler tracks the loads exactly
+            const int kc = min(k, S > 0 ? S - 1 : 0);
+            const int t1 = kc >= ns0;
+            load_step<M, U>(a.codes + beg[t1] * M, (kc - (t1 ? ns0 : 0)) * STEP, len[t1], dst);
+        };
+#pragma unroll
+        for (int r = 0; r + 1 < RD; ++r) issue(r, ring[r]);
         if (tid < D) {  // waves 0 and 1, whole wavefronts
 #pragma unroll
             for (int t = 0; t < TB; ++t) {
@@ -253,15 +312,28 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
             const double pnv = a.pq_norm[size_t(s) * a.ks + jj];
 #pragma unroll
             for (int t = 0; t < TB; ++t) {
+                // residual sub-vector: broadcast LDS reads, 16 bytes at a time
+                float rr[DS];
+                const float4 *rv = reinterpret_cast<const float4 *>(resid + t * D + s * DS);
+#pragma unroll
+                for (int i4 = 0; i4 < DS / 4; ++i4) {
+                    const float4 x = rv[i4];
+                    rr[4 * i4] = x.x;
+                    rr[4 * i4 + 1] = x.y;
+                    rr[4 * i4 + 2] = x.z;
+                    rr[4 * i4 + 3] = x.w;
+                }
                 float dot = 0.0f;
 #pragma unroll
                 for (int i = 0; i < DS; ++i) {
-                    const float prod = resid[t * D + s * DS + i] * pr[i];
+                    const float prod = rr[i] * pr[i];
                     dot = dot + prod;
                 }
                 const float c = float(pnv + double(rnorm[t * M + s]));
-                const float m2 = -2.0f * dot;
-                tab[(t * M + s) * 256 + j] = jok ? m2 + c : __builtin_huge_valf();
+                // fl32(fl32(-2*dot) + c): the product by -2 is exact, so one
+                // fused rounding gives the same bits
+                const float e = __builtin_fmaf(-2.0f, dot, c);
+                tab[(t * M + s) * 256 + j] = jok ? e : __builtin_huge_valf();
             }
             __builtin_amdgcn_sched_barrier(0);  // one subquantizer's operands live at a time
         }
@@ -269,51 +341,41 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
 
         // ---- scan the batch's lists back to back; the prefetch runs one
         //      step ahead, across list boundaries
-        int cur_t = 0;  // list whose first step `cur` holds (loaded before the tables)
+        for (int k0 = 0; k0 < S; k0 += RD) {
 #pragma unroll
-        for (int t = 0; t < TB; ++t) {
-            if (t >= nb) break;
-            const float *tt = tab + t * M * 256;
-            const uint8_t *lc = a.codes + beg[t] * M;
-            if (cur_t != t) {  // the previous list was empty: nothing prefetched this one
-                load_step<M, U>(lc, 0, len[t], cur);
-                cur_t = t;
-            }
-            for (int base = 0; base < len[t]; base += STEP) {
+            for (int r = 0; r < RD; ++r) {
+                const int k = k0 + r;
+                if (k >= S) break;
+                issue(k + RD - 1, ring[(r + RD - 1) % RD]);
+                const int t1 = k >= ns0;
+                const int base = (k - (t1 ? ns0 : 0)) * STEP;
+                const int ln = t1 ? len[1] : len[0];
+                const int64_t bg = t1 ? beg[1] : beg[0];
                 const int rot = step % 3;
                 if (tid == 0) C.inc[(step + 1) % 3] = 0;
-                bool next_list = false;
-                if (base + STEP < len[t]) {
-                    load_step<M, U>(lc, base + STEP, len[t], nxt);
-                } else if (t + 1 < nb) {
-                    load_step<M, U>(a.codes + beg[t + 1] * M, 0, len[t + 1], nxt);
-                    next_list = true;
-                }
+                // the dynamic LDS block starts at address 0 (checked at entry),
+                // so every table address is a compile-time constant and a
+                // gather is one SDWA byte shift plus a ds_read
+                float accs[U];
+                if (t1)
+                    adc_step<M, U>(ring[r], uint32_t(L::tab) + uint32_t(M * 256 * 4), base + (tid & ~63), ln, accs);
+                else
+                    adc_step<M, U>(ring[r], uint32_t(L::tab), base + (tid & ~63), ln, accs);
+                // ties with the threshold distance pass too; ids are compared
+                // exactly at the next selection
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    float acc = 0.0f;
-#pragma unroll
-                    for (int w = 0; w < CW; ++w) {
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            const uint32_t code = (cur[u][w] >> (8 * b)) & 255u;
-                            acc = acc + tt[(w * 4 + b) * 256 + code];
-                        }
-                    }
                     const int v = base + u * NT + tid;
-                    const uint32_t kh = fkey(acc);
-                    // ties with the threshold distance pass too; ids are
-                    // compared exactly at the next selection
-                    const bool pass = (v < len[t]) && kh <= thr_hi;
+                    const bool pass = (v < ln) && accs[u] <= thr_f;
                     const uint64_t mask = __ballot(pass);
                     if (mask) {
                         const int lane = __lane_id();
                         const int before = __popcll(mask & ((uint64_t(1) << lane) - 1));
                         int pos = 0;
                         if (lane == 0) pos = atomicAdd(&C.inc[rot], __popcll(mask));
-                        pos = fill + __shfl(pos, 0);
-                        if (pass && pos + before < CAP)
-                            cand[pos + before] = (uint64_t(kh) << 32) | uint32_t(beg[t] + v);
+                        pos = fill + __shfl(pos, 0) + before;
+                        if (pass && pos < CAP)
+                            cand[pos] = (uint64_t(fkey(accs[u])) << 32) | uint32_t(bg + v);
                     }
                 }
                 __syncthreads();
@@ -322,13 +384,8 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
                 if (fill > CAP - STEP) {
                     fill = compact<CAP>(C, cand, fill, conv, kq, a.list_ids, &thr);
                     conv = fill;
-                    thr_hi = uint32_t(thr >> 32);
+                    thr_f = key_hi_dist(uint32_t(thr >> 32));
                 }
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-#pragma unroll
-                    for (int w = 0; w < CW; ++w) cur[u][w] = nxt[u][w];
-                if (next_list) cur_t = t + 1;
             }
         }
         __syncthreads();  // tables and residuals are rewritten by the next batch
