@@ -61,6 +61,13 @@ _SIGS = {
     "mivf_synth_centres": [_u64, _i, _i, _vp],
     "mivf_zipf_thresholds": [_u64, _i, ctypes.c_double, _vp],
     "mivf_bruteforce_nn": [_vp, _i64, _vp, _i, _i, _vp, _vp],
+    "mivf_write_cent": [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _i, _i, _i, _i, _i, _vp, _vp],
+    "mivf_read_cent_header": [ctypes.c_char_p, _i32p, _i32p, _i32p, _i32p, _i32p],
+    "mivf_read_cent": [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _vp, _vp],
+    "mivf_write_ivf_dir": [ctypes.c_char_p, _i, _i, _vp, _vp, _vp],
+    "mivf_read_ivf_dir": [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _i64],
+    "mivf_write_ivf_file": [ctypes.c_char_p, _i, _i, _vp, _vp, _vp],
+    "mivf_read_ivf_file": [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _i64],
 }
 
 
