@@ -34,6 +34,9 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=400, help="queries in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gt-queries", type=int, default=1000)
+    ap.add_argument("--fast-kl", type=int, default=256, help="shortlist of the reduced-precision form")
+    ap.add_argument("--fast-seed", type=int, default=1, help="lists scanned exactly to seed its filter")
+    ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="build, run the timed steps and a 1 GiB device copy (counter calibration), then exit")
     args = ap.parse_args()
@@ -92,6 +95,33 @@ def main():
     M.synchronize()
     elapsed = time.perf_counter() - t_start
     ix.set_option("profile", 0)
+
+    fast = None
+    if not args.no_fast:
+        kl = args.fast_kl
+        f_ids = M.DeviceArray((nq, k), np.int32)
+        f_dis = M.DeviceArray((nq, k), np.float32)
+        ix.set_option("fast_seed_lists", args.fast_seed)
+        for _ in range(args.warmup):
+            ix.search_fast(query, w, kl, k, base, out=(f_ids, f_dis))
+        ix.set_option("profile", 1)
+        fstage = np.zeros(8)
+        M.synchronize()
+        t_f = time.perf_counter()
+        for _ in range(args.steps):
+            ix.search_fast(query, w, kl, k, base, out=(f_ids, f_dis))
+            fstage += ix.timings()
+        M.synchronize()
+        f_elapsed = time.perf_counter() - t_f
+        ix.set_option("profile", 0)
+        fstage /= args.steps
+        cands, over = ix.fast_stats()
+        fast = {"qps": round(nq * args.steps / f_elapsed, 1), "ms_per_step": round(f_elapsed * 1000 / args.steps, 4),
+                "kl": kl, "seed_lists": args.fast_seed, "candidates_per_query": round(cands / nq, 1),
+                "overflowed_queries": over,
+                "stages_ms": {n: round(float(v), 4) for n, v in
+                              zip(("coarse", "seed", "pairs", "scan8", "select", "rerank"), fstage)}}
+        log(f"fast form: {fast}")
     if args.profile_only:
         # a plain device-to-device copy of known size: the reference point
         # that scales the HBM byte counters of the search kernels
@@ -112,10 +142,14 @@ def main():
     ng = min(args.gt_queries, nq)
     qsub = M.DeviceArray.from_numpy(query.numpy()[:ng])
     gt, _ = M.bruteforce_nn(base, qsub)
-    rec = {}
-    for R in (1, 10, 100):
-        if R <= k:
-            rec[f"recall@{R}"] = round(100.0 * float(np.mean([gt[i] in ids_h[i, :R] for i in range(ng)])), 3)
+    def recall(ids_h):
+        return {f"recall@{R}": round(100.0 * float(np.mean([gt[i] in ids_h[i, :R] for i in range(ng)])), 3)
+                for R in (1, 10, 100) if R <= k}
+
+    rec = recall(ids_h)
+    if fast is not None:
+        fast.update(recall(f_ids.numpy()))
+        fast["speedup_vs_plain"] = round(fast["qps"] / qps, 3)
 
     cpu = None
     if not args.no_cpu:
@@ -155,6 +189,7 @@ def main():
                      "traffic": None,
                      "kernel": "scan_kernel", "algorithmic_bytes_per_launch": scan_bytes},
         "cpu_baseline": cpu,
+        "fast_form": fast,
     }
     print(json.dumps(res), flush=True)
 

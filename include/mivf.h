@@ -128,12 +128,34 @@ int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
                        const float *d_base, int64_t nbase, int32_t *d_ids,
                        float *d_dis, void *stream);
 
+/* Batched reduced-precision form with exact rerank (no reference
+ * counterpart; the reference's RERANK path is mivf_search_rerank):
+ * (1) coarse top-w, (2) an exact scan of each query's nearest lists (at least
+ * "fast_seed_lists", then as many as hold kl vectors) gives the kl-th exact
+ * ADC distance tau_q, (3) one persistent kernel scans the other
+ * (list, query) pairs list-major, 16 queries per work item, with 8-bit
+ * tables (one 16-byte LDS gather serves all 16 queries of a vector) and keeps
+ * every vector whose approximate distance can be <= tau_q, (4) the kl best by
+ * (approximate distance, id) are reranked exactly in FP32 against d_base.
+ * Returned distances are exact; results sorted by (distance, id).          */
+int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, int k,
+                     const float *d_base, int64_t nbase, int32_t *d_ids, float *d_dis,
+                     void *stream);
+/* Candidates kept by the last profiled mivf_search_fast, and the number of
+ * queries whose candidates exceeded the per-query capacity
+ * max("fast_capacity" (default 8192), 64 kl); the excess is dropped, which
+ * lowers recall and makes that query's shortlist depend on arrival order.  */
+int mivf_fast_stats(mivf_index *ix, int64_t *candidates, int64_t *overflowed_queries);
+
 /* ---- options and timing ---------------------------------------------------
  * "probes_per_group" (0 = automatic), "profile" (1 = record per-stage HIP
- * event timings of the next searches).                                     */
+ * event timings of the next searches), "fast_seed_lists" (lists per query
+ * scanned exactly to seed mivf_search_fast, default 1), "fast_capacity".  */
 int mivf_set_option(mivf_index *ix, const char *name, int64_t value);
 /* Per-stage milliseconds of the last profiled search:
- * [0] coarse distances, [1] probe select, [2] scan, [3] merge, [4] rerank. */
+ * [0] coarse distances, [1] probe select, [2] scan, [3] merge (mivf_search);
+ * [0] coarse, [1] seed scan, [2] pair grouping, [3] 8-bit scan, [4] shortlist,
+ * [5] rerank (mivf_search_fast).                                            */
 int mivf_get_timings(mivf_index *ix, float *ms, int n);
 
 /* ---- synthetic data and ground truth ------------------------------------ */

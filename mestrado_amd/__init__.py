@@ -55,6 +55,8 @@ _SIGS = {
     "mivf_coarse": [_vp, _vp, _i, _i, _vp, _vp, _vp],
     "mivf_search_residual": [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp],
     "mivf_search_rerank": [_vp, _vp, _i, _i, _i, _i, _vp, _i64, _vp, _vp, _vp],
+    "mivf_search_fast": [_vp, _vp, _i, _i, _i, _i, _vp, _i64, _vp, _vp, _vp],
+    "mivf_fast_stats": [_vp, _i64p, _i64p],
     "mivf_set_option": [_vp, ctypes.c_char_p, _i64],
     "mivf_get_timings": [_vp, _f32p, _i],
     "mivf_synth_rows": [_u64, _u64, _i64, _i64, _i, _vp, _i, _vp, ctypes.c_float, _vp, _vp],
@@ -331,6 +333,23 @@ class Index:
         dis = DeviceArray((nq, k), np.float32)
         _check(lib().mivf_search_residual(self.h, rd.ptr, ld.ptr, nq, w, k, ids.ptr, dis.ptr, None))
         return ids.numpy(), dis.numpy()
+
+    def search_fast(self, q, w, kl, k, base, out=None):
+        """Batched reduced-precision search with exact FP32 rerank."""
+        qd, _ = _dev(q, np.float32)
+        bd, _ = _dev(base, np.float32)
+        nq = qd.shape[0]
+        ids, dis = out if out is not None else (DeviceArray((nq, k), np.int32), DeviceArray((nq, k), np.float32))
+        _check(lib().mivf_search_fast(self.h, qd.ptr, nq, w, kl, k, bd.ptr, bd.shape[0], ids.ptr, dis.ptr, None))
+        if out is not None:
+            return ids, dis
+        return ids.numpy(), dis.numpy()
+
+    def fast_stats(self):
+        c = ctypes.c_int64(0)
+        o = ctypes.c_int64(0)
+        _check(lib().mivf_fast_stats(self.h, ctypes.byref(c), ctypes.byref(o)))
+        return c.value, o.value
 
     def search_rerank(self, q, w, kl, k, base):
         qd, _ = _dev(q, np.float32)

@@ -84,10 +84,17 @@ struct mivf_index {
     DevBuf off, ids, codes;
     // scratch
     DevBuf qn, dmat, probes, coadis, partial, wq, kq, tmp_ids, tmp_dis, keys, res, resn, assign;
+    DevBuf pmax;             // [m] max_c |p_sc| (reduced-precision table bound)
+    // batched reduced-precision form scratch
+    DevBuf f_sids, f_sdis, f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_ilist, f_ipair, f_ctrl, f_cand,
+        f_ccnt, f_short, f_ns;
+    int64_t f_last_cands = 0, f_last_overflow = 0;
     DevBuf errflag;          // kernel-reported precondition failures (int)
     bool scan_checked = false;
     // options
     int ppg_opt = 0;
+    int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
+    int fast_cap = 8192;      // candidate capacity per query of the reduced-precision form
     bool profile = false;
     float timings[8] = {};
     hipEvent_t ev[8] = {};
@@ -530,6 +537,23 @@ int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, cons
         (rc = ix->pq_n32.ensure(sizeof(float) * size_t(m) * ks)) ||
         (rc = ix->errflag.ensure(sizeof(int))))
         return fail(rc);
+    {
+        std::vector<float> pm(m, 0.0f);
+        for (int s2 = 0; s2 < m; ++s2)
+            for (int c = 0; c < ks; ++c) {
+                double nn = 0.0;
+                for (int t = 0; t < ix->ds; ++t) {
+                    const double v = pq[(int64_t(s2) * ks + c) * ix->ds + t];
+                    nn += v * v;
+                }
+                pm[s2] = std::max(pm[s2], float(std::sqrt(nn) * (1.0 + 1e-6)));
+            }
+        if ((rc = ix->pmax.ensure(sizeof(float) * m))) return fail(rc);
+        if (hipMemcpy(ix->pmax.p, pm.data(), sizeof(float) * m, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("index_create: upload failed");
+            return fail(kHip);
+        }
+    }
     if (hipMemset(ix->errflag.p, 0, sizeof(int)) != hipSuccess) {
         set_error("index_create: hipMemset failed");
         return fail(kHip);
@@ -722,6 +746,150 @@ int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
     return kOk;
 }
 
+int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, int k, const float *d_base,
+                     int64_t nbase, int32_t *d_ids, float *d_dis, void *stream) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(ix->m == 8 || ix->m == 16, kUnsupported, "search_fast: compiled for m in {8, 16}");
+    MIVF_REQUIRE(w >= 1 && w <= ix->nlist && w <= 1024, kInvalid, "search_fast: w must be in 1..min(nlist,1024)");
+    MIVF_REQUIRE(kl >= 1 && kl <= 1024 && k >= 1 && k <= kl, kInvalid, "search_fast: need 1 <= k <= kl <= 1024");
+    if (nq <= 0) return kOk;
+    MIVF_REQUIRE(d_q && d_base && d_ids && d_dis && nbase >= 0, kInvalid, "search_fast: null pointer");
+    MIVF_TRY(upload_lists(ix));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nlist = ix->nlist, d = ix->d;
+    const int capq = std::max(ix->fast_cap, 64 * kl);
+    MIVF_REQUIRE(double(nq) * nlist * 4.0 <= double(1u << 31), kInvalid,
+                 "search_fast: nq * nlist too large for one batch (split the queries)");
+    Timer tm;
+    std::fill(ix->timings, ix->timings + 8, 0.0f);
+    MIVF_TRY(tm.start(ix->profile, st, ix->ev));
+    // 1. coarse top-w
+    MIVF_TRY(ix->qn.ensure(sizeof(double) * nq));
+    MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nq) * nlist));
+    MIVF_TRY(ix->probes.ensure(sizeof(int32_t) * size_t(nq) * w));
+    MIVF_TRY(ix->coadis.ensure(sizeof(float) * size_t(nq) * w));
+    MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ix->qn.as<double>(), st));
+    MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), nlist, d_q, d,
+                                     ix->qn.as<double>(), nq, d, ix->dmat.as<float>(), nlist, st));
+    MIVF_CHECK_HIP(launch_row_topk(ix->dmat.as<float>(), nq, nlist, nlist, w, ix->probes.as<int32_t>(),
+                                   ix->coadis.as<float>(), st));
+    MIVF_TRY(tm.mark());
+    // 2. seed: exact scan of each query's ns[q] nearest lists (at least
+    // fast_seed, then as many as it takes to hold kl vectors), kl best -> tau
+    MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
+    MIVF_CHECK_HIP(launch_seed_count(ix->probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), nlist, kl,
+                                     std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), st));
+    MIVF_TRY(ix->f_sids.ensure(sizeof(int32_t) * size_t(nq) * kl));
+    MIVF_TRY(ix->f_sdis.ensure(sizeof(float) * size_t(nq) * kl));
+    MIVF_TRY(ix->f_tau.ensure(sizeof(float) * nq));
+    {
+        ScanArgs a{};
+        a.queries = d_q;
+        a.coarse = ix->coarse.as<float>();
+        a.pq = ix->pq.as<float>();
+        a.pq_norm = ix->pq_n64.as<double>();
+        a.list_off = ix->off.as<int64_t>();
+        a.list_ids = ix->ids.as<int32_t>();
+        a.codes = ix->codes.as<uint8_t>();
+        a.probes = ix->probes.as<int32_t>();
+        a.wq = ix->f_ns.as<int32_t>();
+        a.nq = nq;
+        a.w = w;
+        a.wstride = w;
+        a.k = kl;
+        a.kstride = kl;
+        a.ks = ix->ks;
+        a.nlist = nlist;
+        a.ppg = w;
+        a.ngroups = 1;
+        a.err = ix->errflag.as<int>();
+        a.out_ids = ix->f_sids.as<int32_t>();
+        a.out_dis = ix->f_sdis.as<float>();
+        MIVF_CHECK_HIP(launch_scan(a, ix->m, st));
+        MIVF_CHECK_HIP(launch_seed_tau(ix->f_sdis.as<float>(), nq, kl, ix->f_tau.as<float>(), st));
+    }
+    MIVF_TRY(tm.mark());
+    // 3. (list, query) pairs of ranks >= 1, grouped by list, and work items
+    const int64_t np = int64_t(nq) * w;
+    MIVF_TRY(ix->f_cnt.ensure(sizeof(int) * (nlist + 1)));
+    MIVF_TRY(ix->f_poff.ensure(sizeof(int) * (nlist + 1)));
+    MIVF_TRY(ix->f_ioff.ensure(sizeof(int) * (nlist + 1)));
+    MIVF_TRY(ix->f_fill.ensure(sizeof(int) * (nlist + 1)));
+    MIVF_TRY(ix->f_pairq.ensure(sizeof(int32_t) * std::max<int64_t>(np, 1)));
+    MIVF_TRY(ix->f_ilist.ensure(sizeof(int32_t) * std::max<int64_t>(np, 1)));
+    MIVF_TRY(ix->f_ipair.ensure(sizeof(int32_t) * std::max<int64_t>(np, 1)));
+    MIVF_TRY(ix->f_ctrl.ensure(sizeof(int) * 4));
+    int *ctrl = ix->f_ctrl.as<int>();  // [0] total items, [1] next item
+    MIVF_CHECK_HIP(launch_pairs(ix->probes.as<int32_t>(), nq, w, ix->f_ns.as<int32_t>(), nlist, ix->f_cnt.as<int>(), ix->f_poff.as<int>(),
+                                ix->f_ioff.as<int>(), ctrl, ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(),
+                                ix->f_ilist.as<int32_t>(), ix->f_ipair.as<int32_t>(), st));
+    MIVF_TRY(ix->f_cand.ensure(sizeof(uint64_t) * size_t(nq) * capq));
+    MIVF_TRY(ix->f_ccnt.ensure(sizeof(int) * nq));
+    MIVF_CHECK_HIP(hipMemsetAsync(ix->f_ccnt.p, 0, sizeof(int) * nq, st));
+    MIVF_CHECK_HIP(hipMemsetAsync(ctrl + 1, 0, sizeof(int), st));
+    MIVF_TRY(tm.mark());
+    // 4. persistent list-major scan
+    {
+        FastArgs a{};
+        a.queries = d_q;
+        a.coarse = ix->coarse.as<float>();
+        a.pq = ix->pq.as<float>();
+        a.pmax = ix->pmax.as<float>();
+        a.list_off = ix->off.as<int64_t>();
+        a.list_ids = ix->ids.as<int32_t>();
+        a.codes = ix->codes.as<uint8_t>();
+        a.pair_q = ix->f_pairq.as<int32_t>();
+        a.list_pair_off = ix->f_poff.as<int>();
+        a.item_list = ix->f_ilist.as<int32_t>();
+        a.item_pair = ix->f_ipair.as<int32_t>();
+        a.total_items = ctrl;
+        a.next_item = ctrl + 1;
+        a.tau = ix->f_tau.as<float>();
+        a.cand = ix->f_cand.as<uint64_t>();
+        a.cand_cnt = ix->f_ccnt.as<int>();
+        a.capq = capq;
+        a.ks = ix->ks;
+        a.err = ix->errflag.as<int>();
+        int dev = 0, ncu = 256;
+        MIVF_CHECK_HIP(hipGetDevice(&dev));
+        MIVF_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        MIVF_CHECK_HIP(launch_fast_scan(a, ix->m, 2 * ncu, st));
+    }
+    MIVF_TRY(tm.mark());
+    // 5. shortlist and exact rerank
+    MIVF_TRY(ix->f_short.ensure(sizeof(int32_t) * size_t(nq) * kl));
+    MIVF_CHECK_HIP(launch_fast_select(ix->f_sids.as<int32_t>(), ix->f_sdis.as<float>(), kl, ix->f_cand.as<uint64_t>(),
+                                      ix->f_ccnt.as<int>(), capq, nq, ix->f_short.as<int32_t>(), st));
+    MIVF_TRY(tm.mark());
+    MIVF_CHECK_HIP(launch_rerank(d_base, nbase, d_q, nq, d, ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
+    MIVF_TRY(tm.mark());
+    MIVF_TRY(finish_timer(ix, tm));
+    if (!ix->scan_checked || ix->profile) {
+        int err = 0;
+        MIVF_CHECK_HIP(hipStreamSynchronize(st));
+        MIVF_CHECK_HIP(hipMemcpy(&err, ix->errflag.p, sizeof(int), hipMemcpyDeviceToHost));
+        MIVF_REQUIRE(err == 0, kHip, "kernel precondition failed: dynamic LDS block not at address 0");
+        ix->scan_checked = true;
+        std::vector<int> cc(nq);
+        MIVF_CHECK_HIP(hipMemcpy(cc.data(), ix->f_ccnt.p, sizeof(int) * nq, hipMemcpyDeviceToHost));
+        int64_t tot = 0, over = 0;
+        for (int v : cc) {
+            tot += v;
+            over += v > capq;
+        }
+        ix->f_last_cands = tot;
+        ix->f_last_overflow = over;
+    }
+    return kOk;
+}
+
+int mivf_fast_stats(mivf_index *ix, int64_t *candidates, int64_t *overflowed_queries) {
+    MIVF_TRY(check_index(ix));
+    if (candidates) *candidates = ix->f_last_cands;
+    if (overflowed_queries) *overflowed_queries = ix->f_last_overflow;
+    return kOk;
+}
+
 int mivf_set_option(mivf_index *ix, const char *name, int64_t value) {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(name, kInvalid, "set_option: null name");
@@ -731,6 +899,12 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) {
         ix->ppg_opt = int(value);
     } else if (n == "profile") {
         ix->profile = value != 0;
+    } else if (n == "fast_seed_lists") {
+        MIVF_REQUIRE(value >= 1 && value <= 1024, kInvalid, "fast_seed_lists must be in 1..1024");
+        ix->fast_seed = int(value);
+    } else if (n == "fast_capacity") {
+        MIVF_REQUIRE(value >= 1024 && value <= (1 << 20), kInvalid, "fast_capacity must be in 1024..2^20");
+        ix->fast_cap = int(value);
     } else {
         set_error("set_option: unknown option " + n);
         return kInvalid;

@@ -136,6 +136,44 @@ hipError_t launch_merge(const uint64_t *partial, int nq, int ngroups, int kstrid
                         const int32_t *kq, int k, int32_t *out_ids, float *out_dis,
                         hipStream_t s);
 
+// Batched reduced-precision scan (list-major work items of up to 16 queries,
+// 8-bit tables, candidates appended to per-query buffers).
+struct FastArgs {
+    const float *queries;       // [nq][d]
+    const float *coarse;        // [nlist][d]
+    const float *pq;            // [m][ks][ds]
+    const float *pmax;          // [m] max_c |p_sc|
+    const int64_t *list_off;    // [nlist+1]
+    const int32_t *list_ids;    // [N]
+    const uint8_t *codes;       // [N][m]
+    const int32_t *pair_q;      // query of each (list, query) pair, list-major
+    const int *list_pair_off;   // [nlist+1]
+    const int32_t *item_list;   // work item -> list
+    const int32_t *item_pair;   // work item -> first pair
+    const int *total_items;
+    int *next_item;             // work-queue counter (zeroed before launch)
+    const float *tau;           // [nq] seed threshold (exact ADC distance)
+    uint64_t *cand;             // [nq][capq] (approx key, id)
+    int *cand_cnt;              // [nq]
+    int capq;
+    int ks;
+    int *err;
+};
+int fast_group_size();
+// Seed lists per query: the smallest ns >= min_seed whose first ns probed
+// lists hold >= kl vectors (or ns = w).
+hipError_t launch_seed_count(const int32_t *probes, int nq, int w, const int64_t *list_off, int nlist, int kl,
+                             int min_seed, int32_t *ns, hipStream_t s);
+// (list, query) pairs of probe ranks >= rank0[q], grouped by list, and the
+// work items (list, first pair) of at most fast_group_size() pairs.
+hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *rank0, int nlist, int *cnt,
+                        int *pair_off, int *item_off, int *total_items, int *fill, int32_t *pair_q,
+                        int32_t *item_list, int32_t *item_pair, hipStream_t s);
+hipError_t launch_seed_tau(const float *sdis, int nq, int kl, float *tau, hipStream_t s);
+hipError_t launch_fast_scan(const FastArgs &a, int m, int grid, hipStream_t s);
+hipError_t launch_fast_select(const int32_t *sids, const float *sdis, int kl, const uint64_t *cand,
+                              const int *cand_cnt, int capq, int nq, int32_t *short_ids, hipStream_t s);
+
 // Exact rerank of a shortlist against raw vectors (a = base row, b = query).
 hipError_t launch_rerank(const float *base, int64_t nbase, const float *queries, int nq,
                          int d, const int32_t *short_ids, int kl, int k,

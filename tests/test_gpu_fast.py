@@ -1,0 +1,128 @@
+"""GPU tests of the batched reduced-precision form (mivf_search_fast).
+
+Its shortlist comes from 8-bit tables, so the shortlist itself is not the
+reference's; what is pinned exactly is everything the caller sees: every
+returned distance is the exact FP32 rerank distance of the returned id (the
+oracle's distance arithmetic, bit for bit), rows are sorted by (distance, id),
+ids are unique and come from the query's probed lists, results are
+deterministic, and recall against FP64 brute force matches the exact RERANK
+path's (sequential/ivf_pq/ivf_search.cpp:98-134) within a stated tolerance."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def check_exact_rows(orc, ids, dis, base, query, coarse_ids, list_of):
+    nq, k = ids.shape
+    for q in range(nq):
+        row = ids[q]
+        n = int((row >= 0).sum())
+        assert (row[:n] >= 0).all() and (row[n:] == -1).all(), f"q={q}: padding not at the end"
+        assert np.isinf(dis[q, n:]).all()
+        assert len(set(row[:n].tolist())) == n, f"q={q}: duplicate ids"
+        if n == 0:
+            continue
+        ex = orc.cross_dist(base[row[:n]], query[q:q + 1])[0]
+        assert np.array_equal(ex.view(np.uint32), dis[q, :n].view(np.uint32)), \
+            f"q={q}: distances are not the exact rerank distances"
+        key = orc.ckey(dis[q, :n], row[:n])
+        assert (np.diff(key.astype(np.uint64)) > 0).all(), f"q={q}: not sorted by (distance, id)"
+        assert np.isin(list_of[row[:n]], coarse_ids[q]).all(), f"q={q}: id outside the probed lists"
+
+
+def list_of_ids(off, ids, nbase):
+    lo = np.full(nbase, -1, np.int64)
+    for l in range(len(off) - 1):
+        lo[ids[off[l]:off[l + 1]]] = l
+    return lo
+
+
+@pytest.fixture(scope="module")
+def zipf16_gt(orc, zipf16):
+    ix, index, query, base = zipf16
+    return orc.bruteforce_f64(base, query, 1)[0][:, 0]
+
+
+@pytest.mark.parametrize("w,kl,k,seed_lists", [(32, 256, 100, 1), (32, 128, 100, 2), (8, 1024, 10, 1),
+                                                (64, 256, 1, 1), (1, 256, 100, 1)])
+def test_fast_exact_rows_m16(mivf, orc, zipf16, zipf16_gt, w, kl, k, seed_lists):
+    ix, index, query, base = zipf16
+    gt = zipf16_gt
+    ix.set_option("fast_seed_lists", seed_lists)
+    ix.set_option("fast_capacity", 1 << 16)
+    ix.set_option("profile", 1)
+    try:
+        ids, dis = ix.search_fast(query, w, kl, k, base)
+        cands, over = ix.fast_stats()
+        ids2, dis2 = ix.search_fast(query, w, kl, k, base)
+    finally:
+        ix.set_option("fast_seed_lists", 1)
+        ix.set_option("fast_capacity", 8192)
+        ix.set_option("profile", 0)
+    assert over == 0, f"{over} queries overflowed the candidate buffer"
+    assert np.array_equal(ids, ids2) and np.array_equal(dis.view(np.uint32), dis2.view(np.uint32)), \
+        "not deterministic"
+    coarse_ids, _ = ix.coarse(query, w)
+    lo = list_of_ids(index[2], index[3], base.shape[0])
+    check_exact_rows(orc, ids, dis, base, query, coarse_ids, lo)
+    # recall against FP64 brute force: within 1 query of the exact RERANK path
+    rids, rdis = ix.search_rerank(query, w, kl, k, base)
+    R = min(k, 100)
+    rf = orc.recall_at(ids, gt[:, None], R)
+    rr = orc.recall_at(rids, gt[:, None], R)
+    assert rf >= rr - 1.0 / query.shape[0] - 1e-9, f"recall@{R}: fast {rf} vs rerank {rr}"
+    # the two paths agree on nearly every returned id
+    agree = np.mean([len(set(a[a >= 0]) & set(b[b >= 0])) / max(1, (b >= 0).sum()) for a, b in zip(ids, rids)])
+    assert agree >= 0.97, f"only {agree:.3f} of the exact RERANK ids returned"
+
+
+def test_fast_m8_and_empty_lists(mivf, orc, c1):
+    """m = 8, ragged lists with empty ones (seed lists shorter than kl keep
+    every vector), and k > available results (padding)."""
+    q = c1["gold_q"]
+    nlist = q["coarse"].shape[0]
+    base = c1["base"]
+    ix = mivf.Index(q["coarse"], q["pq"])
+    ix.add(base)
+    off, ids_all, codes = ix.get_lists()
+    # drop every third list's members to make empty lists
+    keep = np.ones(len(ids_all), bool)
+    for l in range(0, nlist, 3):
+        keep[off[l]:off[l + 1]] = False
+    sizes = np.diff(off)
+    sizes[::3] = 0
+    off2 = np.zeros(nlist + 1, np.int64)
+    off2[1:] = np.cumsum(sizes)
+    ix2 = mivf.Index(q["coarse"], q["pq"])
+    ix2.set_lists(off2, ids_all[keep], codes[keep])
+    lo = list_of_ids(off2, ids_all[keep], base.shape[0])
+    query = c1["query"]
+    gt = orc.bruteforce_f64(base, query, 1)[0][:, 0]
+    ix2.set_option("profile", 1)
+    for w, kl, k in ((8, 256, 100), (2, 64, 64), (256, 512, 100), (1, 1024, 1000)):
+        ids, dis = ix2.search_fast(query, w, kl, k, base)
+        assert ix2.fast_stats()[1] == 0
+        coarse_ids, _ = ix2.coarse(query, w)
+        check_exact_rows(orc, ids, dis, base, query, coarse_ids, lo)
+        # the shortlists differ from the exact RERANK path's only near the kl
+        # boundary: with kl >= 2k nearly the same ids, and always the same
+        # recall within one query
+        rids, rdis = ix2.search_rerank(query, w, kl, k, base)
+        agree = np.mean([len(set(a[a >= 0]) & set(b[b >= 0])) / max(1, (b >= 0).sum()) for a, b in zip(ids, rids)])
+        assert kl < 2 * k or agree >= 0.97, f"w={w} kl={kl}: only {agree:.3f} of the exact RERANK ids returned"
+        R = min(k, 100)
+        rf, rr = orc.recall_at(ids, gt[:, None], R), orc.recall_at(rids, gt[:, None], R)
+        assert rf >= rr - 1.0 / len(ids) - 1e-9, f"w={w} kl={kl}: recall@{R} fast {rf} vs rerank {rr}"
+
+
+def test_fast_rejects_bad_shapes(mivf, c1):
+    q = c1["gold_q"]
+    ix = mivf.Index(q["coarse"], q["pq"])
+    ix.add(c1["base"][:2000])
+    with pytest.raises(mivf.MivfError):
+        ix.search_fast(c1["query"], 8, 50, 100, c1["base"])   # k > kl
+    with pytest.raises(mivf.MivfError):
+        ix.search_fast(c1["query"], 8, 2048, 100, c1["base"])  # kl > 1024
+    with pytest.raises(mivf.MivfError):
+        ix.search_fast(c1["query"], 0, 64, 10, c1["base"])

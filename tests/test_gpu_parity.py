@@ -26,19 +26,6 @@ def assert_same_results(ids, dis, rids, rdis, what=""):
 
 
 @pytest.fixture(scope="module")
-def c1(orc):
-    c = WORKLOADS["ivfpq_10k_l256_m8"]
-    cent = orc.gen_centres(c["seed"], NCOMP, D)
-    return dict(
-        cfg=c, cent=cent,
-        train=orc.gen_rows(c["seed"], STREAM_TRAIN, 0, c["ntrain"], cent, None, SIGMA),
-        base=orc.gen_rows(c["seed"], STREAM_BASE, 0, c["nbase"], cent, None, SIGMA),
-        query=orc.gen_rows(c["seed"], STREAM_QUERY, 0, c["nquery"], cent, None, SIGMA),
-        gold_q=np.load(os.path.join(GOLD, "c1_quantizers.npz")),
-    )
-
-
-@pytest.fixture(scope="module")
 def c1_index(mivf, orc, c1):
     """GPU index over the C1 base with the golden quantizers."""
     q = c1["gold_q"]
@@ -226,25 +213,9 @@ def test_bruteforce_nn(mivf, orc, c1):
     assert np.array_equal(gd, rd[:, 0])
 
 
-@pytest.fixture(scope="module")
-def zipf16(mivf, orc):
-    """Headline-shaped (Zipf s=1.1 mixture, m=16) index at reduced size."""
-    seed = 103
-    cent = orc.gen_centres(seed, NCOMP, D)
-    thr = orc.zipf_thresholds(seed, NCOMP, 1.1)
-    train = orc.gen_rows(seed, STREAM_TRAIN, 0, 40_000, cent, thr, SIGMA)
-    base = mivf.synth_rows(seed, STREAM_BASE, 0, 300_000, cent, thr, SIGMA)
-    query = orc.gen_rows(seed, STREAM_QUERY, 0, 300, cent, thr, SIGMA)
-    coarse, pq = mivf.train(train, 512, 16, 256, 8, 8, seed)
-    ix = mivf.Index(coarse, pq)
-    ix.add(base)
-    index = (coarse, pq) + tuple(ix.get_lists())
-    return ix, index, query
-
-
 @pytest.mark.parametrize("w,k,ppg", [(32, 100, 0), (32, 100, 32), (64, 10, 4), (1, 1, 0), (200, 128, 0)])
 def test_search_m16_zipf(orc, zipf16, w, k, ppg):
-    ix, index, query = zipf16
+    ix, index, query, _ = zipf16
     ix.set_option("probes_per_group", ppg)
     try:
         ids, dis = ix.search(query, w, k)
@@ -255,7 +226,7 @@ def test_search_m16_zipf(orc, zipf16, w, k, ppg):
 
 
 def test_search_m16_rerank_and_var(orc, zipf16, mivf):
-    ix, index, query = zipf16
+    ix, index, query, _ = zipf16
     rng = np.random.default_rng(21)
     nq = query.shape[0]
     w = rng.integers(1, 65, nq).astype(np.int32)
