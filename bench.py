@@ -34,7 +34,7 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=400, help="queries in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gt-queries", type=int, default=1000)
-    ap.add_argument("--fast-kl", type=int, default=256, help="shortlist of the reduced-precision form")
+    ap.add_argument("--fast-kl", type=int, default=128, help="shortlist of the reduced-precision form")
     ap.add_argument("--fast-seed", type=int, default=1, help="lists scanned exactly to seed its filter")
     ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
@@ -102,14 +102,15 @@ def main():
         f_ids = M.DeviceArray((nq, k), np.int32)
         f_dis = M.DeviceArray((nq, k), np.float32)
         ix.set_option("fast_seed_lists", args.fast_seed)
+        ix.attach_base(base)  # list-ordered copy of the raw vectors (index build, untimed)
         for _ in range(args.warmup):
-            ix.search_fast(query, w, kl, k, base, out=(f_ids, f_dis))
+            ix.search_fast(query, w, kl, k, out=(f_ids, f_dis))
         ix.set_option("profile", 1)
         fstage = np.zeros(8)
         M.synchronize()
         t_f = time.perf_counter()
         for _ in range(args.steps):
-            ix.search_fast(query, w, kl, k, base, out=(f_ids, f_dis))
+            ix.search_fast(query, w, kl, k, out=(f_ids, f_dis))
             fstage += ix.timings()
         M.synchronize()
         f_elapsed = time.perf_counter() - t_f

@@ -128,6 +128,13 @@ int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
                        const float *d_base, int64_t nbase, int32_t *d_ids,
                        float *d_dis, void *stream);
 
+/* Keep a copy of the raw vectors d_base [nbase][d] in HBM in list order, so
+ * that the rerank of mivf_search_fast reads each query's shortlist from at
+ * most w contiguous regions (pass d_base = NULL there to use it).  The copy
+ * is dropped when the lists change (add / set_lists).  No reference
+ * counterpart: the reference reranks from vbase in host memory.             */
+int mivf_index_attach_base(mivf_index *ix, const float *d_base, int64_t nbase);
+
 /* Batched reduced-precision form with exact rerank (no reference
  * counterpart; the reference's RERANK path is mivf_search_rerank):
  * (1) coarse top-w, (2) an exact scan of each query's nearest lists (at least
@@ -136,16 +143,24 @@ int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
  * (list, query) pairs list-major, 16 queries per work item, with 8-bit
  * tables (one 16-byte LDS gather serves all 16 queries of a vector) and keeps
  * every vector whose approximate distance can be <= tau_q, (4) the kl best by
- * (approximate distance, id) are reranked exactly in FP32 against d_base.
- * Returned distances are exact; results sorted by (distance, id).          */
+ * (approximate distance, list position) are reranked exactly in FP32
+ * against d_base (or the attached copy when d_base is NULL).  d_q must be
+ * 16-byte aligned.  Returned distances are exact; results sorted by
+ * (distance, id).                                                          */
 int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, int k,
                      const float *d_base, int64_t nbase, int32_t *d_ids, float *d_dis,
                      void *stream);
 /* Candidates kept by the last profiled mivf_search_fast, and the number of
  * queries whose candidates exceeded the per-query capacity
- * max("fast_capacity" (default 8192), 64 kl); the excess is dropped, which
+ * max("fast_capacity" (default 16384), 64 kl); the excess is dropped, which
  * lowers recall and makes that query's shortlist depend on arrival order.  */
 int mivf_fast_stats(mivf_index *ix, int64_t *candidates, int64_t *overflowed_queries);
+
+/* The 8-bit tables of mivf_search_fast for queries d_q[0..n-1] (n <= 16)
+ * against list `list`: d_t8 [m][256][16] uint8 (entry [s][code][query]) and
+ * d_delta [16] (approximate distance = delta * sum of entries).  For tests. */
+int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t *d_t8,
+                     float *d_delta);
 
 /* ---- options and timing ---------------------------------------------------
  * "probes_per_group" (0 = automatic), "profile" (1 = record per-stage HIP

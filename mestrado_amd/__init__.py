@@ -57,6 +57,8 @@ _SIGS = {
     "mivf_search_rerank": [_vp, _vp, _i, _i, _i, _i, _vp, _i64, _vp, _vp, _vp],
     "mivf_search_fast": [_vp, _vp, _i, _i, _i, _i, _vp, _i64, _vp, _vp, _vp],
     "mivf_fast_stats": [_vp, _i64p, _i64p],
+    "mivf_index_attach_base": [_vp, _vp, _i64],
+    "mivf_fast_tables": [_vp, _vp, _i, _i, _vp, _vp],
     "mivf_set_option": [_vp, ctypes.c_char_p, _i64],
     "mivf_get_timings": [_vp, _f32p, _i],
     "mivf_synth_rows": [_u64, _u64, _i64, _i64, _i, _vp, _i, _vp, ctypes.c_float, _vp, _vp],
@@ -334,16 +336,33 @@ class Index:
         _check(lib().mivf_search_residual(self.h, rd.ptr, ld.ptr, nq, w, k, ids.ptr, dis.ptr, None))
         return ids.numpy(), dis.numpy()
 
-    def search_fast(self, q, w, kl, k, base, out=None):
-        """Batched reduced-precision search with exact FP32 rerank."""
-        qd, _ = _dev(q, np.float32)
+    def attach_base(self, base):
+        """Keep a list-ordered copy of the raw vectors for search_fast(base=None)."""
         bd, _ = _dev(base, np.float32)
+        _check(lib().mivf_index_attach_base(self.h, bd.ptr, bd.shape[0]))
+
+    def search_fast(self, q, w, kl, k, base=None, out=None):
+        """Batched reduced-precision search with exact FP32 rerank (against
+        `base`, or the attached copy when base is None)."""
+        qd, _ = _dev(q, np.float32)
         nq = qd.shape[0]
+        bptr, nb = None, 0
+        if base is not None:
+            bd, _ = _dev(base, np.float32)
+            bptr, nb = bd.ptr, bd.shape[0]
         ids, dis = out if out is not None else (DeviceArray((nq, k), np.int32), DeviceArray((nq, k), np.float32))
-        _check(lib().mivf_search_fast(self.h, qd.ptr, nq, w, kl, k, bd.ptr, bd.shape[0], ids.ptr, dis.ptr, None))
+        _check(lib().mivf_search_fast(self.h, qd.ptr, nq, w, kl, k, bptr, nb, ids.ptr, dis.ptr, None))
         if out is not None:
             return ids, dis
         return ids.numpy(), dis.numpy()
+
+    def fast_tables(self, q, list_no):
+        """8-bit tables [m][256][16] and deltas [16] of one reduced-precision work item."""
+        qd, _ = _dev(q, np.float32)
+        t8 = DeviceArray((self.m, 256, 16), np.uint8)
+        dl = DeviceArray((16,), np.float32)
+        _check(lib().mivf_fast_tables(self.h, qd.ptr, qd.shape[0], int(list_no), t8.ptr, dl.ptr))
+        return t8.numpy(), dl.numpy()
 
     def fast_stats(self):
         c = ctypes.c_int64(0)

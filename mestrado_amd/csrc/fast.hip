@@ -6,19 +6,25 @@
 // queries for a vector's code (the code belongs to the vector, not to the
 // query), and the list's codes are read from HBM once per 16 queries.
 //
-// Table entry (query g, subquantizer s, code c), with r = q - c_l:
-//   T = |r_s|^2 + (|p_sc|^2 + 2 c_ls.p_sc) - 2 q_s.p_sc          (= |r_s - p_sc|^2)
-//   t8 = round(T * 255 / bound_g), bound_g = max_s (|r_s| + max_c |p_sc|)^2 >= T
-// Approximate distance = delta_g * sum_s t8 with delta_g = bound_g / 255; each
-// term is within delta_g / 2 of T (plus FP32 rounding), so
-// |approx - ADC| <= 8 delta_g (+ slack) for m = 16.
+// Tables.  With r = q - c_l, sigma_s = max_c |p_sc| (1 if 0), bound_g =
+// max_s (|r_s| + sigma_s)^2 * 1.0001 and invd_g = 255 / bound_g, the entry
+// invd_g * T = invd_g * (|r_s|^2 + |p_sc|^2 - 2 r_s.p_sc) is one dot product
+//   a = [-2 invd sigma r_s, invd |r_s|^2, invd sigma^2] . b = [p_sc / sigma, 1, |p_sc|^2 / sigma^2]
+// computed on the matrix cores in FP16 with FP32 accumulation
+// (v_mfma_f32_16x16x16_f16: 16 query slots x 16 codes per instruction).
+// Every |a_k b_k| sums to <= invd (|r_s| + |p_sc|)^2 <= 255 and FP16 rounds
+// each operand to 2^-11, so the entry is within 255 * 2^-10 < 0.25 of
+// invd * T; rounding to 8 bits adds 0.5, so each table entry is within 0.75
+// quanta of invd * T and a vector's 8-bit sum within 0.75 m quanta of
+// invd * ADC.  B operands are prepared once per index (fast_prep_kernel).
 //
-// Candidate filter: a per-query threshold tau_q from a seed pass (the exact
-// ADC kl-th distance over the query's nearest list).  A vector passes if
-// sum <= tau_q / delta_g + m/2 + 2, which keeps every vector whose exact ADC
-// distance is <= tau_q.  Passing (approx key, id) pairs are appended to a
-// per-query global buffer; fast_select keeps the kl best (approx distance, id)
-// together with the seed's, and the exact FP32 rerank orders them.
+// Candidate filter: a per-query threshold tau_q >= the kl-th smallest ADC
+// distance over the query's seed lists (seed_tau_kernel).  A vector passes
+// if its sum <= tau_q * invd + 3m/4 + 2, which keeps every vector whose ADC
+// distance is <= tau_q (the +2 covers FP32 differences from the reference
+// table arithmetic).  Passing (approximate distance, list position) keys are
+// staged in LDS and flushed to per-query global buffers; fast_select keeps
+// the kl best and the exact FP32 rerank orders them by (distance, id).
 #include "select_dev.h"
 
 namespace mivf {
@@ -28,33 +34,43 @@ namespace {
 constexpr int D = 128;
 constexpr int FG = 16;    // queries per work item
 constexpr int FNT = 256;  // threads per workgroup
+constexpr int POOL = 1536;
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const u32x4 lds_u32x4;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
-// Candidate staging in LDS: entries (list position v, query slot g, sum) are
-// appended with one LDS atomic per wavefront and flushed to the per-query
-// global buffers with one global atomic per query slot.
-constexpr int POOL = 1536;
+// K of the table dot products, padded to whole 16-wide MFMA steps
+template <int M>
+struct FK {
+    static constexpr int DS = D / M;
+    static constexpr int KP = (DS + 2 + 15) / 16 * 16;
+    static constexpr int NMF = KP / 16;
+};
+
+__device__ inline float sigma_of(float pmax) { return pmax > 0.0f ? pmax : 1.0f; }
 
 template <int M>
 struct FLay {
+    static constexpr int KP = FK<M>::KP;
     static constexpr size_t t8 = 0;                            // uint8 [M][256][FG]
     static constexpr size_t pool = t8 + size_t(M) * 256 * FG;  // uint2 [POOL] (scan phase)
-    static constexpr size_t qv = pool;                         // float [FG][D]   (table phase)
-    static constexpr size_t A = qv + size_t(FG) * D * 4;       // float [FG][M]   (table phase)
-    static constexpr size_t cl = A + size_t(FG) * M * 4;       // float [D]       (table phase)
-    static_assert(cl + D * 4 <= pool + POOL * 8, "table-phase data must fit in the pool");
-    static constexpr size_t invd = pool + POOL * 8;            // float [FG]
-    static constexpr size_t delt = invd + FG * 4;              // float [FG]
-    static constexpr size_t sthr = delt + FG * 4;              // uint32 [FG/2] (u16 pairs of limits)
+    static constexpr size_t ao = pool;                         // fp16 [M][FG][KP] A operands (table phase)
+    static_assert(size_t(M) * FG * KP * 2 <= POOL * 8, "A operands must fit in the pool");
+    static constexpr size_t delt = pool + POOL * 8;            // float [FG]
+    static constexpr size_t pmx = delt + FG * 4;               // float [M]
+    static constexpr size_t sthr = pmx + M * 4;                // uint32 [FG/2] (u16 pairs of limits)
     static constexpr size_t stmp = sthr + FG / 2 * 4;          // uint32 [FG] unpacked limits
-    static constexpr size_t qid = stmp + FG * 4;               // int [FG]
+    static constexpr size_t qid = stmp + FG * 4;               // int [FG] query of each slot (-1 empty)
     static constexpr size_t gcnt = qid + FG * 4;               // int [FG]
     static constexpr size_t gbase = gcnt + FG * 4;             // int [FG]
-    static constexpr size_t pcnt = gbase + FG * 4;             // int
-    static constexpr size_t item = pcnt + 4;                   // int
-    static constexpr size_t total = item + 12;
+    static constexpr size_t rec = gbase + FG * 4;              // FastItem: the current item
+    static constexpr size_t nrec = rec + sizeof(FastItem);     // FastItem: the next item
+    static constexpr size_t pcnt = nrec + sizeof(FastItem);    // int
+    static constexpr size_t item = pcnt + 4;                   // int current item index
+    static constexpr size_t nidx = item + 4;                   // int next item index
+    static constexpr size_t total = nidx + 8;
 };
 
 // Move the staged candidates to the per-query global buffers (whole
@@ -87,7 +103,7 @@ __device__ void flush_pool(const FastArgs &a, unsigned char *smem, int64_t beg) 
         const uint32_t sum = (e.y >> 4) & 0xFFFu;
         const int pos = gbase[g] + int(e.y >> 16);
         if (pos < a.capq)
-            a.cand[int64_t(qid[g]) * a.capq + pos] = ckey(delt[g] * float(sum), a.list_ids[beg + e.x]);
+            a.cand[int64_t(qid[g]) * a.capq + pos] = ckey(delt[g] * float(sum), int32_t(beg + e.x));
     }
     __syncthreads();
     if (tid == 0) *pcnt = 0;
@@ -105,139 +121,237 @@ __device__ inline void load_codes(const uint8_t *lc, int v, uint32_t (&cw)[M / 4
     }
 }
 
+// Inputs of one work item's A side: thread (g, s) = (tid / M, tid % M),
+// tid < FG * M, holds the ds query and centroid values of its sub-vector.
+template <int M>
+struct ItemIn {
+    static constexpr int DS = D / M;
+    float qx[DS], cx[DS];
+    __device__ void load(const FastArgs &a, int l, int q, int s) {
+#pragma unroll
+        for (int t = 0; t < DS; t += 4) {
+            const float4 c4 = *reinterpret_cast<const float4 *>(a.coarse + int64_t(l) * D + s * DS + t);
+            cx[t] = c4.x; cx[t + 1] = c4.y; cx[t + 2] = c4.z; cx[t + 3] = c4.w;
+            float4 q4 = c4;  // empty slot: r = 0
+            if (q >= 0) q4 = *reinterpret_cast<const float4 *>(a.queries + int64_t(q) * D + s * DS + t);
+            qx[t] = q4.x; qx[t + 1] = q4.y; qx[t + 2] = q4.z; qx[t + 3] = q4.w;
+        }
+    }
+};
+
+// The work item's A operands, scales and limits (from the preloaded
+// inputs), then its 8-bit tables in LDS t8.  Uniform call; ends after a
+// barrier with the tables complete.
+template <int M>
+__device__ void build_item_tables(const FastArgs &a, unsigned char *smem, const ItemIn<M> &in,
+                                  const float *taus) {
+    constexpr int DS = D / M;
+    constexpr int KP = FK<M>::KP;
+    constexpr int NMF = FK<M>::NMF;
+    using L = FLay<M>;
+    _Float16 *ao = reinterpret_cast<_Float16 *>(smem + L::ao);
+    float *delt = reinterpret_cast<float *>(smem + L::delt);
+    const float *pmx = reinterpret_cast<const float *>(smem + L::pmx);
+    uint32_t *sthr = reinterpret_cast<uint32_t *>(smem + L::sthr);
+    uint32_t *stmp = reinterpret_cast<uint32_t *>(smem + L::stmp);
+    const int *qid = reinterpret_cast<const int *>(smem + L::qid);
+    const int tid = threadIdx.x;
+
+    if (tid < FG * M) {
+        const int g = tid / M, s = tid % M;
+        float r[DS];
+        float A = 0.0f;
+#pragma unroll
+        for (int t = 0; t < DS; ++t) {
+            r[t] = in.qx[t] - in.cx[t];
+            A = __builtin_fmaf(r[t], r[t], A);
+        }
+        const float sg = sigma_of(pmx[s]);
+        const float b = __builtin_sqrtf(A) + pmx[s];
+        float bound = b * b;
+#pragma unroll
+        for (int off = M / 2; off > 0; off >>= 1) bound = fmaxf(bound, __shfl_xor(bound, off, M));
+        bound = fmaxf(bound * 1.0001f, 1e-20f);
+        const float invd = 255.0f / bound;
+        _Float16 *row = ao + (size_t(s) * FG + g) * KP;
+        const float m2 = -2.0f * invd * sg;
+#pragma unroll
+        for (int t = 0; t < DS; ++t) row[t] = _Float16(m2 * r[t]);
+        row[DS] = _Float16(invd * A);
+        row[DS + 1] = _Float16(invd * sg * sg);
+#pragma unroll
+        for (int t = DS + 2; t < KP; ++t) row[t] = _Float16(0.0f);
+        if (s == 0) {
+            const float dlt = bound / 255.0f;
+            delt[g] = dlt;
+            const float tau = taus[g];
+            // stored limit = thr + 1 (a sum passes iff sum < limit)
+            uint32_t lim;
+            if (qid[g] < 0) lim = 0u;                     // empty slot: nothing passes
+            else if (!(tau < 3.0e38f)) lim = 0xFFFFu;     // no seed bound: keep everything
+            else lim = uint32_t(fminf(tau * invd + float(3 * M / 4 + 2), 65534.0f)) + 1u;
+            stmp[g] = lim;
+        }
+    }
+    __syncthreads();
+    if (tid < FG / 2) {
+        // sthr[k] holds the limits of the two queries of accumulator k:
+        // k = 2i + odd pairs queries 4i + odd (low half) and 4i + odd + 2 (high half)
+        const int i = tid >> 1, odd = tid & 1;
+        const int qa = 4 * i + odd, qb = qa + 2;
+        sthr[tid] = stmp[qa] | (stmp[qb] << 16);
+    }
+
+    // v_mfma_f32_16x16x16_f16: lane l supplies A[l%16][4(l/16) .. +3] and
+    // B[4(l/16) .. +3][l%16] and receives D[4(l/16) + r][l%16] in
+    // accumulator r, i.e. queries 4h..4h+3 of one code: one 32-bit word of
+    // the [s][code][16 queries] table row.
+    const int lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 4, c16 = lane & 15;
+    constexpr int TPW = 256 / 16 / (FNT / 64);  // code tiles per wave
+    const _Float16 *pb = reinterpret_cast<const _Float16 *>(a.pb);
+    f16x4 bn[TPW][NMF];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int f = 0; f < NMF; ++f)
+            bn[t][f] = *reinterpret_cast<const f16x4 *>(pb + (size_t((wave * TPW + t) * 16 + c16)) * KP + f * 16 + 4 * h);
+#pragma unroll 2
+    for (int s = 0; s < M; ++s) {
+        f16x4 b[TPW][NMF];
+#pragma unroll
+        for (int t = 0; t < TPW; ++t)
+#pragma unroll
+            for (int f = 0; f < NMF; ++f) b[t][f] = bn[t][f];
+        if (s + 1 < M) {
+#pragma unroll
+            for (int t = 0; t < TPW; ++t)
+#pragma unroll
+                for (int f = 0; f < NMF; ++f)
+                    bn[t][f] = *reinterpret_cast<const f16x4 *>(
+                        pb + (size_t(s + 1) * 256 + (wave * TPW + t) * 16 + c16) * KP + f * 16 + 4 * h);
+        }
+        f16x4 av[NMF];
+#pragma unroll
+        for (int f = 0; f < NMF; ++f)
+            av[f] = *reinterpret_cast<const f16x4 *>(ao + (size_t(s) * FG + c16) * KP + f * 16 + 4 * h);
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int f = 0; f < NMF; ++f) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(av[f], b[t][f], acc, 0, 0, 0);
+            uint32_t w = 0u;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                w = __builtin_amdgcn_cvt_pk_u8_f32(__builtin_amdgcn_fmed3f(acc[r], 0.0f, 255.0f), uint32_t(r), w);
+            const int c = (wave * TPW + t) * 16 + c16;
+            *reinterpret_cast<uint32_t *>(smem + L::t8 + (size_t(s) * 256 + c) * FG + 4 * h) = w;
+        }
+    }
+    __syncthreads();  // tables ready; the pool may now overwrite the A operands
+}
+
+// B operands of the table dot products, once per index:
+// pb[s][c][k] = p_sc[k] / sigma_s (k < ds), 1 (k = ds), |p_sc|^2 / sigma_s^2
+// (k = ds + 1), 0 after; rows of codes c >= ks are 0 (valid codes are < ks,
+// so those table entries are never indexed).
+template <int M>
+__global__ void fast_prep_kernel(const float *__restrict__ pq, const float *__restrict__ pmax, int ks,
+                                 _Float16 *__restrict__ pb) {
+    constexpr int DS = D / M;
+    constexpr int KP = FK<M>::KP;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // s * 256 + c
+    if (i >= M * 256) return;
+    const int s = i / 256, c = i % 256;
+    const float sg = sigma_of(pmax[s]);
+    _Float16 *row = pb + size_t(i) * KP;
+    float pn = 0.0f;
+    if (c < ks) {
+        const float *p = pq + (int64_t(s) * ks + c) * DS;
+        for (int t = 0; t < DS; ++t) {
+            pn = __builtin_fmaf(p[t], p[t], pn);
+            row[t] = _Float16(p[t] / sg);
+        }
+        row[DS] = _Float16(1.0f);
+        row[DS + 1] = _Float16(pn / (sg * sg));
+    } else {
+        for (int t = 0; t < DS + 2; ++t) row[t] = _Float16(0.0f);
+    }
+    for (int t = DS + 2; t < KP; ++t) row[t] = _Float16(0.0f);
+}
+
 // Accumulators: ae[i] holds the sums of queries 4i (low u16) and 4i+2 (high),
 // ao[i] those of 4i+1 and 4i+3; per query the sum is <= m * 255 < 2^16, so
 // the low half never carries into the high one.
+//
+// Persistent: workgroups take items from a global counter.  While item i is
+// scanned, the next item's index, its record and then its query and centroid
+// values are fetched one stage per scan step, so the table phase of item
+// i + 1 starts with its inputs in registers.
 template <int M>
 __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
-    constexpr int DS = D / M;
     using L = FLay<M>;
+    constexpr int RW = int(sizeof(FastItem) / 4);  // record words
+    static_assert(RW <= 64, "a record is loaded by one wavefront");
     extern __shared__ __align__(16) unsigned char smem[];
-    float *qv = reinterpret_cast<float *>(smem + L::qv);
-    float *A = reinterpret_cast<float *>(smem + L::A);
-    float *cl = reinterpret_cast<float *>(smem + L::cl);
-    float *invd = reinterpret_cast<float *>(smem + L::invd);
-    float *delt = reinterpret_cast<float *>(smem + L::delt);
-    uint32_t *sthr = reinterpret_cast<uint32_t *>(smem + L::sthr);
-    uint32_t *stmp = reinterpret_cast<uint32_t *>(smem + L::stmp);
+    float *pmx = reinterpret_cast<float *>(smem + L::pmx);
+    const float *delt = reinterpret_cast<const float *>(smem + L::delt);
+    const uint32_t *sthr = reinterpret_cast<const uint32_t *>(smem + L::sthr);
     int *qid = reinterpret_cast<int *>(smem + L::qid);
     int *pcnt = reinterpret_cast<int *>(smem + L::pcnt);
     int *item_sh = reinterpret_cast<int *>(smem + L::item);
+    int *nidx_sh = reinterpret_cast<int *>(smem + L::nidx);
+    FastItem *rec = reinterpret_cast<FastItem *>(smem + L::rec);
+    FastItem *nrec = reinterpret_cast<FastItem *>(smem + L::nrec);
     uint2 *pool = reinterpret_cast<uint2 *>(smem + L::pool);
     const int tid = threadIdx.x;
-    const int lane = __lane_id();
+    const int lane = tid & 63, wave = tid >> 6;
     if (uint32_t(uintptr_t((__attribute__((address_space(3))) unsigned char *)(smem))) != 0) {
         if (tid == 0) atomicOr(a.err, 2);
         return;
     }
     const int total = *a.total_items;
-    if (tid == 0) *pcnt = 0;
+    if (tid == 0) {
+        *pcnt = 0;
+        *item_sh = atomicAdd(a.next_item, 1);
+    }
+    if (tid < M) pmx[tid] = a.pmax[tid];
+    __syncthreads();
+    if (*item_sh < total && tid < RW)
+        reinterpret_cast<uint32_t *>(rec)[tid] = reinterpret_cast<const uint32_t *>(a.items + *item_sh)[tid];
+    __syncthreads();
+    ItemIn<M> in;
+    if (*item_sh < total && tid < FG * M) in.load(a, rec->l, rec->q[tid / M], tid % M);
 
     for (;;) {
-        if (tid == 0) *item_sh = atomicAdd(a.next_item, 1);
-        __syncthreads();
         const int item = *item_sh;
         if (item >= total) break;  // uniform
-        const int l = a.item_list[item];
-        const int pb = a.item_pair[item];
-        const int gc = min(FG, a.list_pair_off[l + 1] - pb);
-        const int64_t beg = a.list_off[l];
-        const int len = int(a.list_off[l + 1] - beg);
+        const int64_t beg = rec->beg;
+        const int len = rec->len;
+        if (tid < FG) qid[tid] = rec->q[tid];
+        __syncthreads();
+        build_item_tables<M>(a, smem, in, rec->tau);
 
-        if (tid < FG) qid[tid] = tid < gc ? a.pair_q[pb + tid] : -1;
-        if (tid < D) cl[tid] = a.coarse[int64_t(l) * D + tid];
-        __syncthreads();
-        for (int e = tid; e < FG * D; e += FNT) {
-            const int g = e / D;
-            const int q = qid[g];
-            qv[e] = q >= 0 ? a.queries[int64_t(q) * D + (e % D)] : 0.0f;
-        }
-        __syncthreads();
-        {   // |r_s|^2 for (g, s): FG*M <= 256 threads
-            const int g = tid / M, s = tid % M;
-            if (g < FG) {
-                float acc = 0.0f;
-#pragma unroll
-                for (int t = 0; t < DS; ++t) {
-                    const float r = qv[g * D + s * DS + t] - cl[s * DS + t];
-                    acc = __builtin_fmaf(r, r, acc);
+        // prefetch of the next item: 0 index wanted, 1 record in registers,
+        // 2 record in LDS, 3 inputs in registers (stage is uniform)
+        int nxt = 0;
+        if (tid == 0) nxt = atomicAdd(a.next_item, 1);
+        int stage = 0;
+        uint32_t rv = 0u;
+        auto advance = [&]() {
+            if (stage == 0) {
+                if (wave == 0) {
+                    const int nb = __shfl(nxt, 0);
+                    if (lane < RW && nb < total) rv = reinterpret_cast<const uint32_t *>(a.items + nb)[lane];
+                    if (lane == 0) *nidx_sh = nb;
                 }
-                A[g * M + s] = acc;
+            } else if (stage == 1) {
+                if (wave == 0 && lane < RW) reinterpret_cast<uint32_t *>(nrec)[lane] = rv;
+            } else if (stage == 2) {
+                if (*nidx_sh < total && tid < FG * M) in.load(a, nrec->l, nrec->q[tid / M], tid % M);
             }
-        }
-        __syncthreads();
-        if (tid < FG) {
-            const int g = tid;
-            float bound = 0.0f;
-            for (int s = 0; s < M; ++s) {
-                const float b = __builtin_sqrtf(A[g * M + s]) + a.pmax[s];
-                bound = fmaxf(bound, b * b);
-            }
-            bound = fmaxf(bound * 1.0001f, 1e-20f);
-            const float dlt = bound / 255.0f;
-            delt[g] = dlt;
-            invd[g] = 255.0f / bound;
-            const float tau = qid[g] >= 0 ? a.tau[qid[g]] : -1.0f;
-            // stored limit = thr + 1 (a sum passes iff sum < limit)
-            uint32_t lim;
-            if (qid[g] < 0) lim = 0u;                     // empty slot: nothing passes
-            else if (!(tau < 3.0e38f)) lim = 0xFFFFu;     // no seed bound: keep everything
-            else lim = uint32_t(fminf(tau / dlt + float(M / 2 + 2), 65534.0f)) + 1u;
-            stmp[g] = lim;
-        }
-        __syncthreads();
-        if (tid < FG / 2) {
-            // sthr[k] holds the limits of the two queries of accumulator k:
-            // k = 2i + odd pairs queries 4i + odd (low half) and 4i + odd + 2 (high half)
-            const int i = tid >> 1, odd = tid & 1;
-            const int qa = 4 * i + odd, qb = qa + 2;
-            sthr[tid] = stmp[qa] | (stmp[qb] << 16);
-        }
-
-        // ---- 16 quantized tables: thread c owns code c of every subquantizer
-        {
-            const int c = tid;  // FNT == 256 == ks
-            const bool cok = c < a.ks;
-            const float *src = a.pq + int64_t(cok ? c : 0) * DS;
-            const int64_t sstride = int64_t(a.ks) * DS;
-            float pnext[DS];
-#pragma unroll
-            for (int t = 0; t < DS; ++t) pnext[t] = src[t];
-#pragma unroll 1
-            for (int s = 0; s < M; ++s) {
-                float p[DS];
-#pragma unroll
-                for (int t = 0; t < DS; ++t) p[t] = pnext[t];
-                if (s + 1 < M) {
-#pragma unroll
-                    for (int t = 0; t < DS; ++t) pnext[t] = src[(s + 1) * sstride + t];
-                }
-                float pn = 0.0f, cp = 0.0f;
-#pragma unroll
-                for (int t = 0; t < DS; ++t) {
-                    pn = __builtin_fmaf(p[t], p[t], pn);
-                    cp = __builtin_fmaf(cl[s * DS + t], p[t], cp);
-                }
-                const float B = pn + 2.0f * cp;
-                uint32_t w4[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-                for (int g = 0; g < FG; ++g) {
-                    float qp = 0.0f;
-#pragma unroll
-                    for (int t = 0; t < DS; ++t) qp = __builtin_fmaf(qv[g * D + s * DS + t], p[t], qp);
-                    const float T = fmaxf(A[g * M + s] + B - 2.0f * qp, 0.0f);
-                    const float v = cok ? T * invd[g] : 255.0f;
-                    w4[g >> 2] = __builtin_amdgcn_cvt_pk_u8_f32(v, uint32_t(g & 3), w4[g >> 2]);
-                }
-                uint4 o;
-                o.x = w4[0];
-                o.y = w4[1];
-                o.z = w4[2];
-                o.w = w4[3];
-                *reinterpret_cast<uint4 *>(smem + L::t8 + (size_t(s) * 256 + c) * FG) = o;
-            }
-        }
-        __syncthreads();  // tables ready; the pool may now overwrite qv, A, cl
+            ++stage;
+        };
 
         // ---- scan: each lane one vector per step, codes loaded one step ahead
         const uint8_t *lc = a.codes + beg * M;
@@ -303,7 +417,7 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
                         const int q = qid[g];
                         const int pos = atomicAdd(a.cand_cnt + q, 1);
                         if (pos < a.capq)
-                            a.cand[int64_t(q) * a.capq + pos] = ckey(delt[g] * float(sum), a.list_ids[beg + v]);
+                            a.cand[int64_t(q) * a.capq + pos] = ckey(delt[g] * float(sum), int32_t(beg + v));
                     }
                     ++slot;
                 }
@@ -312,12 +426,44 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
             const int filled = *pcnt;
             __syncthreads();  // every thread has read the count before the next appends
             if (filled > POOL - FNT) flush_pool<M>(a, smem, beg);  // uniform
+            if (stage < 3) advance();
         }
         // the loop's last barrier pair ordered every append before this read,
         // and no thread appends again before the next item's barriers
         if (*pcnt > 0) flush_pool<M>(a, smem, beg);
-        __syncthreads();  // the next item overwrites qid, the tables and the pool
+        while (stage < 3) {
+            advance();
+            __syncthreads();
+        }
+        __syncthreads();  // rec, qid and the pool are free; nrec is complete
+        if (tid < RW) reinterpret_cast<uint32_t *>(rec)[tid] = reinterpret_cast<const uint32_t *>(nrec)[tid];
+        if (tid == 0) *item_sh = *nidx_sh;
+        __syncthreads();
     }
+}
+
+// One work item's tables copied out (test hook): queries 0..n-1, list l.
+template <int M>
+__global__ __launch_bounds__(FNT) void fast_tables_kernel(FastArgs a, int l, int n, uint8_t *t8, float *delta) {
+    using L = FLay<M>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *pmx = reinterpret_cast<float *>(smem + L::pmx);
+    int *qid = reinterpret_cast<int *>(smem + L::qid);
+    FastItem *rec = reinterpret_cast<FastItem *>(smem + L::rec);
+    const int tid = threadIdx.x;
+    if (tid < FG) {
+        qid[tid] = tid < n ? tid : -1;
+        rec->q[tid] = tid < n ? tid : -1;
+        rec->tau[tid] = __builtin_huge_valf();
+    }
+    if (tid < M) pmx[tid] = a.pmax[tid];
+    __syncthreads();
+    ItemIn<M> in;
+    if (tid < FG * M) in.load(a, l, rec->q[tid / M], tid % M);
+    build_item_tables<M>(a, smem, in, rec->tau);
+    const uint4 *src = reinterpret_cast<const uint4 *>(smem + L::t8);
+    for (int i = tid; i < M * 256; i += FNT) reinterpret_cast<uint4 *>(t8)[i] = src[i];
+    if (tid < FG) delta[tid] = reinterpret_cast<const float *>(smem + L::delt)[tid];
 }
 
 // ---- seed lists per query
@@ -343,7 +489,7 @@ __global__ void pair_count_kernel(const int32_t *__restrict__ probes, int nq, in
     if (e >= int64_t(nq) * w) return;
     const int r = int(e % w);
     const int l = probes[e];
-    if (r >= rank0[e / w] && l >= 0 && l < nlist) atomicAdd(cnt + l, 1);
+    if (r >= (rank0 ? rank0[e / w] : 0) && l >= 0 && l < nlist) atomicAdd(cnt + l, 1);
 }
 
 // exclusive scans of pair counts and of work items per list (one workgroup)
@@ -392,30 +538,177 @@ __global__ void pair_scatter_kernel(const int32_t *__restrict__ probes, int nq, 
     if (e >= int64_t(nq) * w) return;
     const int r = int(e % w);
     const int l = probes[e];
-    if (r >= rank0[e / w] && l >= 0 && l < nlist) pair_q[pair_off[l] + atomicAdd(fill + l, 1)] = int32_t(e / w);
+    if (r >= (rank0 ? rank0[e / w] : 0) && l >= 0 && l < nlist) pair_q[pair_off[l] + atomicAdd(fill + l, 1)] = int32_t(e / w);
 }
 
-__global__ void item_expand_kernel(const int *__restrict__ pair_off, const int *__restrict__ item_off, int nlist,
-                                   int32_t *__restrict__ item_list, int32_t *__restrict__ item_pair) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlist) return;
-    const int n = pair_off[l + 1] - pair_off[l];
-    for (int j = 0, it = item_off[l]; j * FG < n; ++j, ++it) {
-        item_list[it] = l;
-        item_pair[it] = pair_off[l] + j * FG;
+// One thread per (work item, query slot); items are numbered list by list
+// (item_off), so a binary search over item_off finds the item's list.
+__global__ void item_expand_kernel(const int *__restrict__ pair_off, const int *__restrict__ item_off,
+                                   const int64_t *__restrict__ list_off, const int32_t *__restrict__ pair_q,
+                                   const float *__restrict__ tau, int nlist, const int *__restrict__ total_items,
+                                   int64_t max_slots, FastItem *__restrict__ items) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= max_slots) return;
+    const int it = int(e / FG), slot = int(e % FG);
+    if (it >= *total_items) return;
+    int lo = 0, hi = nlist;  // last list with item_off[l] <= it (lists without items are skipped)
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (item_off[mid] <= it) lo = mid;
+        else hi = mid;
+    }
+    const int l = lo;
+    const int j = it - item_off[l];
+    const int pb = pair_off[l] + j * FG;
+    const int gc = min(FG, pair_off[l + 1] - pb);
+    FastItem &x = items[it];
+    const int q = slot < gc ? pair_q[pb + slot] : -1;
+    x.q[slot] = q;
+    x.tau[slot] = q >= 0 ? tau[q] : -1.0f;
+    if (slot == 0) {
+        x.l = l;
+        x.gc = gc;
+        x.len = int(list_off[l + 1] - list_off[l]);
+        x.pad = 0;
+        x.beg = list_off[l];
+        x.pad2 = 0;
     }
 }
 
-// ---- seed: tau_q = kl-th exact distance of the seed list (inf if fewer)
-__global__ void seed_tau_kernel(const float *__restrict__ sdis, int nq, int kl, float *__restrict__ tau) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nq) tau[q] = sdis[int64_t(q) * kl + kl - 1];
+// ---- seed threshold: an upper bound of the kl-th smallest ADC distance over
+// each query's seed lists.  Tables in FP32 from r = fl(q - c_l):
+// T = |r_s|^2 + |p|^2 - 2 r_s.p; the kl-th value is located by a three-pass
+// radix histogram (8-bit digits) of the order keys, and tau is the largest
+// float of the final 24-bit bucket plus a margin 2^-14 * sum_s (|r_s| +
+// max|p_s|)^2 that covers the FP32 rounding differences between these sums
+// and the reference table arithmetic (each at most ~40 eps of that sum).
+template <int M>
+__device__ inline float seed_adc(const float *tab, const uint8_t *lc, int v) {
+    uint32_t cw[M / 4];
+    load_codes<M>(lc, v, cw);
+    float acc = 0.0f;
+#pragma unroll
+    for (int s = 0; s < M; ++s) acc += tab[s * 256 + ((cw[s >> 2] >> (8 * (s & 3))) & 255u)];
+    return acc;
 }
 
-// ---- per query: the kl best (distance, id) keys of seed + candidates
-__global__ __launch_bounds__(256) void fast_select_kernel(const int32_t *__restrict__ sids,
-                                                          const float *__restrict__ sdis, int kl,
-                                                          const uint64_t *__restrict__ cand,
+// wave 0: the bin holding the need-th key (1-based) of hist[256]
+__device__ inline void find_bin(const uint32_t *hist, int need, int *digit, int *below) {
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
+        const uint32_t tot = h0 + h1 + h2 + h3;
+        uint32_t incl = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, 64);
+            if (tid >= off) incl += o;
+        }
+        const uint32_t excl = incl - tot;
+        if (excl < uint32_t(need) && incl >= uint32_t(need)) {
+            uint32_t c = excl;
+            int dg;
+            if (c + h0 >= uint32_t(need)) dg = 0;
+            else if (c + h0 + h1 >= uint32_t(need)) { c += h0; dg = 1; }
+            else if (c + h0 + h1 + h2 >= uint32_t(need)) { c += h0 + h1; dg = 2; }
+            else { c += h0 + h1 + h2; dg = 3; }
+            *digit = 4 * tid + dg;
+            *below = int(c);
+        }
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__ queries,
+                                                       const float *__restrict__ coarse,
+                                                       const float *__restrict__ pq, const float *__restrict__ pnorm,
+                                                       const float *__restrict__ pmax,
+                                                       const int64_t *__restrict__ list_off,
+                                                       const uint8_t *__restrict__ codes,
+                                                       const int32_t *__restrict__ probes, int w,
+                                                       const int32_t *__restrict__ nseed, int nlist, int ks, int kl,
+                                                       float *__restrict__ tau) {
+    constexpr int DS = D / M;
+    __shared__ float tab[M * 256];
+    __shared__ float r[D];
+    __shared__ float As[M];
+    __shared__ uint32_t hist[256];
+    __shared__ int sdig, sbelow;
+    __shared__ float smarg;
+    const int q = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int ns = nseed[q];
+    int64_t tot = 0;
+    for (int rr = 0; rr < ns; ++rr) {
+        const int l = probes[int64_t(q) * w + rr];
+        if (l >= 0 && l < nlist) tot += list_off[l + 1] - list_off[l];
+    }
+    if (tot < kl) {  // uniform
+        if (tid == 0) tau[q] = __builtin_huge_valf();
+        return;
+    }
+    if (tid == 0) smarg = 0.0f;
+    uint32_t prefix = 0;
+    int need = kl;
+    for (int pass = 0; pass < 3; ++pass) {
+        hist[tid] = 0u;  // 256 threads, 256 bins
+        const int sh = 24 - 8 * pass;
+        for (int rr = 0; rr < ns; ++rr) {
+            const int l = probes[int64_t(q) * w + rr];
+            if (l < 0 || l >= nlist) continue;  // uniform
+            if (pass == 0 || ns > 1) {
+                __syncthreads();  // previous list's scan is done with tab
+                if (tid < D) r[tid] = queries[int64_t(q) * D + tid] - coarse[int64_t(l) * D + tid];
+                __syncthreads();
+                if (tid < M) {
+                    float a = 0.0f;
+#pragma unroll
+                    for (int t = 0; t < DS; ++t) a = __builtin_fmaf(r[tid * DS + t], r[tid * DS + t], a);
+                    As[tid] = a;
+                }
+                __syncthreads();
+                if (tid == 0 && pass == 0) {
+                    float mg = 0.0f;
+                    for (int s2 = 0; s2 < M; ++s2) {
+                        const float b = __builtin_sqrtf(As[s2]) + pmax[s2];
+                        mg += b * b;
+                    }
+                    smarg = fmaxf(smarg, mg);
+                }
+                if (tid < ks) {
+                    for (int s2 = 0; s2 < M; ++s2) {
+                        const float *p = pq + (int64_t(s2) * ks + tid) * DS;
+                        float dot = 0.0f;
+#pragma unroll
+                        for (int t = 0; t < DS; ++t) dot = __builtin_fmaf(r[s2 * DS + t], p[t], dot);
+                        tab[s2 * 256 + tid] = As[s2] + pnorm[s2 * ks + tid] - 2.0f * dot;
+                    }
+                }
+            }
+            __syncthreads();
+            const int64_t beg = list_off[l];
+            const int len = int(list_off[l + 1] - beg);
+            const uint8_t *lc = codes + beg * M;
+            for (int v = tid; v < len; v += 256) {
+                const uint32_t key = fkey(seed_adc<M>(tab, lc, v));
+                if (pass == 0 || (key >> (sh + 8)) == (prefix >> (sh + 8))) atomicAdd(&hist[(key >> sh) & 255u], 1u);
+            }
+        }
+        __syncthreads();
+        find_bin(hist, need, &sdig, &sbelow);
+        __syncthreads();
+        prefix |= uint32_t(sdig) << sh;
+        need -= sbelow;
+        __syncthreads();  // sdig/sbelow and hist are rewritten by the next pass
+    }
+    if (tid == 0) {
+        const float top = key_dist(uint64_t(prefix | 0xFFu) << 32);
+        tau[q] = top + smarg * (1.0f / 16384.0f);
+    }
+}
+
+// ---- per query: the kl best (approximate distance, list position) keys of the candidates
+__global__ __launch_bounds__(256) void fast_select_kernel(int kl, const uint64_t *__restrict__ cand,
                                                           const int *__restrict__ cand_cnt, int capq,
                                                           int32_t *__restrict__ short_ids) {
     __shared__ SelectShared sh;
@@ -423,16 +716,9 @@ __global__ __launch_bounds__(256) void fast_select_kernel(const int32_t *__restr
     __shared__ int cnt, eqt;
     const int q = blockIdx.x;
     const int tid = threadIdx.x;
-    const int nc = min(cand_cnt[q], capq);
+    const int n = min(cand_cnt[q], capq);
     const uint64_t *C = cand + int64_t(q) * capq;
-    const int n = kl + nc;
-    auto keyfn = [&](int i) -> uint64_t {
-        if (i < kl) {
-            const int32_t id = sids[int64_t(q) * kl + i];
-            return id < 0 ? kNoKey : ckey(sdis[int64_t(q) * kl + i], id);
-        }
-        return C[i - kl];
-    };
+    auto keyfn = [&](int i) -> uint64_t { return C[i]; };
     SelectResult r = block_select<256>(keyfn, n, kl, sh);
     if (tid == 0) {
         cnt = 0;
@@ -440,8 +726,7 @@ __global__ __launch_bounds__(256) void fast_select_kernel(const int32_t *__restr
     }
     __syncthreads();
     for (int i = tid; i < n; i += 256) {
-        const uint64_t key = keyfn(i);
-        if (key == kNoKey) continue;
+        const uint64_t key = C[i];
         bool take = key < r.thr;
         if (!take && r.eq > 0 && key == r.thr) take = atomicAdd(&eqt, 1) < r.eq;
         if (take) {
@@ -457,16 +742,31 @@ __global__ __launch_bounds__(256) void fast_select_kernel(const int32_t *__restr
 inline unsigned nblk(int64_t n, int t) { return unsigned((n + t - 1) / t); }
 
 template <int M>
+hipError_t set_lds_attr(const void *fn) {
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(FLay<M>::total));
+}
+
+template <int M>
 hipError_t launch_fast_m(const FastArgs &a, int grid, hipStream_t s) {
-    constexpr size_t lds = FLay<M>::total;
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&fast_scan_kernel<M>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        hipError_t e = set_lds_attr<M>(reinterpret_cast<const void *>(&fast_scan_kernel<M>));
         if (e != hipSuccess) return e;
         attr = true;
     }
-    fast_scan_kernel<M><<<grid, FNT, lds, s>>>(a);
+    fast_scan_kernel<M><<<grid, FNT, FLay<M>::total, s>>>(a);
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_tables_m(const FastArgs &a, int l, int n, uint8_t *t8, float *delta, hipStream_t s) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = set_lds_attr<M>(reinterpret_cast<const void *>(&fast_tables_kernel<M>));
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    fast_tables_kernel<M><<<1, FNT, FLay<M>::total, s>>>(a, l, n, t8, delta);
     return hipGetLastError();
 }
 
@@ -474,27 +774,62 @@ hipError_t launch_fast_m(const FastArgs &a, int grid, hipStream_t s) {
 
 int fast_group_size() { return FG; }
 
+size_t fast_pb_bytes(int m) {
+    switch (m) {
+        case 8: return size_t(8) * 256 * FK<8>::KP * 2;
+        case 16: return size_t(16) * 256 * FK<16>::KP * 2;
+        default: return 0;
+    }
+}
+
+hipError_t launch_fast_prep(const float *pq, const float *pmax, int m, int ks, void *pb, hipStream_t s) {
+    switch (m) {
+        case 8:
+            fast_prep_kernel<8><<<nblk(8 * 256, 256), 256, 0, s>>>(pq, pmax, ks, static_cast<_Float16 *>(pb));
+            break;
+        case 16:
+            fast_prep_kernel<16><<<nblk(16 * 256, 256), 256, 0, s>>>(pq, pmax, ks, static_cast<_Float16 *>(pb));
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_seed_count(const int32_t *probes, int nq, int w, const int64_t *list_off, int nlist, int kl,
                              int min_seed, int32_t *ns, hipStream_t s) {
     seed_count_kernel<<<nblk(nq, 256), 256, 0, s>>>(probes, nq, w, list_off, nlist, kl, min_seed, ns);
     return hipGetLastError();
 }
 
-hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *rank0, int nlist, int *cnt,
-                        int *pair_off, int *item_off, int *total_items, int *fill, int32_t *pair_q,
-                        int32_t *item_list, int32_t *item_pair, hipStream_t s) {
+hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *rank0, const int64_t *list_off,
+                        const float *tau, int nlist, int *cnt, int *pair_off, int *item_off, int *total_items,
+                        int *fill, int32_t *pair_q, FastItem *items, hipStream_t s) {
     hipError_t e = hipMemsetAsync(cnt, 0, sizeof(int) * nlist, s);
     if (e != hipSuccess) return e;
     const int64_t np = int64_t(nq) * w;
     pair_count_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rank0, nlist, cnt);
     pair_scan_kernel<<<1, 1024, 0, s>>>(cnt, nlist, pair_off, item_off, total_items, fill);
     pair_scatter_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rank0, nlist, pair_off, fill, pair_q);
-    item_expand_kernel<<<nblk(nlist, 256), 256, 0, s>>>(pair_off, item_off, nlist, item_list, item_pair);
+    const int64_t max_slots = (np / FG + nlist) * FG;  // >= FG * total items
+    item_expand_kernel<<<nblk(max_slots, 256), 256, 0, s>>>(pair_off, item_off, list_off, pair_q, tau, nlist,
+                                                          total_items, max_slots, items);
     return hipGetLastError();
 }
 
-hipError_t launch_seed_tau(const float *sdis, int nq, int kl, float *tau, hipStream_t s) {
-    seed_tau_kernel<<<nblk(nq, 256), 256, 0, s>>>(sdis, nq, kl, tau);
+hipError_t launch_seed_tau(const SeedArgs &a, int m, hipStream_t s) {
+    if (a.nq <= 0) return hipSuccess;
+    if (a.kl < 1) return hipErrorInvalidValue;
+    switch (m) {
+        case 8:
+            seed_tau_kernel<8><<<a.nq, 256, 0, s>>>(a.queries, a.coarse, a.pq, a.pnorm, a.pmax, a.list_off, a.codes,
+                                                     a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.tau);
+            break;
+        case 16:
+            seed_tau_kernel<16><<<a.nq, 256, 0, s>>>(a.queries, a.coarse, a.pq, a.pnorm, a.pmax, a.list_off, a.codes,
+                                                      a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.tau);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -506,10 +841,20 @@ hipError_t launch_fast_scan(const FastArgs &a, int m, int grid, hipStream_t s) {
     }
 }
 
-hipError_t launch_fast_select(const int32_t *sids, const float *sdis, int kl, const uint64_t *cand,
-                              const int *cand_cnt, int capq, int nq, int32_t *short_ids, hipStream_t s) {
+hipError_t launch_fast_tables(const FastArgs &a, int m, int l, int n, uint8_t *t8, float *delta, hipStream_t s) {
+    if (n < 1 || n > FG) return hipErrorInvalidValue;
+    switch (m) {
+        case 8: return launch_tables_m<8>(a, l, n, t8, delta, s);
+        case 16: return launch_tables_m<16>(a, l, n, t8, delta, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fast_select(int kl, const uint64_t *cand, const int *cand_cnt, int capq, int nq,
+                              int32_t *short_ids, hipStream_t s) {
     if (kl < 1 || kl > 1024) return hipErrorInvalidValue;
-    fast_select_kernel<<<nq, 256, 0, s>>>(sids, sdis, kl, cand, cand_cnt, capq, short_ids);
+    if (nq <= 0) return hipSuccess;
+    fast_select_kernel<<<nq, 256, 0, s>>>(kl, cand, cand_cnt, capq, short_ids);
     return hipGetLastError();
 }
 

@@ -85,16 +85,19 @@ struct mivf_index {
     // scratch
     DevBuf qn, dmat, probes, coadis, partial, wq, kq, tmp_ids, tmp_dis, keys, res, resn, assign;
     DevBuf pmax;             // [m] max_c |p_sc| (reduced-precision table bound)
+    DevBuf fpb;              // fp16 B operands of the reduced-precision tables
+    DevBuf rows;             // raw vectors in list order (mivf_index_attach_base)
+    bool rows_ok = false;
     // batched reduced-precision form scratch
-    DevBuf f_sids, f_sdis, f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_ilist, f_ipair, f_ctrl, f_cand,
-        f_ccnt, f_short, f_ns;
+    DevBuf f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_items, f_ctrl, f_cand, f_ccnt,
+        f_short, f_ns;
     int64_t f_last_cands = 0, f_last_overflow = 0;
     DevBuf errflag;          // kernel-reported precondition failures (int)
     bool scan_checked = false;
     // options
     int ppg_opt = 0;
     int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
-    int fast_cap = 8192;      // candidate capacity per query of the reduced-precision form
+    int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
     bool profile = false;
     float timings[8] = {};
     hipEvent_t ev[8] = {};
@@ -129,6 +132,8 @@ int upload_lists(mivf_index *ix) {
     ix->h_off = off;
     ix->ntotal = n;
     ix->dirty = false;
+    ix->rows.release();  // list positions changed: an attached copy is stale
+    ix->rows_ok = false;
     return kOk;
 }
 
@@ -567,6 +572,14 @@ int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, cons
         set_error("index_create: HIP upload failed");
         return fail(kHip);
     }
+    if (fast_pb_bytes(m) > 0) {  // operands of the reduced-precision tables
+        if ((rc = ix->fpb.ensure(fast_pb_bytes(m)))) return fail(rc);
+        if (launch_fast_prep(ix->pq.as<float>(), ix->pmax.as<float>(), m, ks, ix->fpb.p, nullptr) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess) {
+            set_error("index_create: reduced-precision operand setup failed");
+            return fail(kHip);
+        }
+    }
     if ((rc = upload_lists(ix))) return fail(rc);
     *out = ix;
     return kOk;
@@ -753,8 +766,11 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_REQUIRE(w >= 1 && w <= ix->nlist && w <= 1024, kInvalid, "search_fast: w must be in 1..min(nlist,1024)");
     MIVF_REQUIRE(kl >= 1 && kl <= 1024 && k >= 1 && k <= kl, kInvalid, "search_fast: need 1 <= k <= kl <= 1024");
     if (nq <= 0) return kOk;
-    MIVF_REQUIRE(d_q && d_base && d_ids && d_dis && nbase >= 0, kInvalid, "search_fast: null pointer");
+    MIVF_REQUIRE(d_q && d_ids && d_dis && nbase >= 0, kInvalid, "search_fast: null pointer");
+    MIVF_REQUIRE((reinterpret_cast<uintptr_t>(d_q) & 15u) == 0, kInvalid, "search_fast: d_q must be 16-byte aligned");
     MIVF_TRY(upload_lists(ix));
+    MIVF_REQUIRE(d_base || ix->rows_ok, kInvalid,
+                 "search_fast: d_base is NULL and no base is attached (mivf_index_attach_base)");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int nlist = ix->nlist, d = ix->d;
     const int capq = std::max(ix->fast_cap, 64 * kl);
@@ -774,55 +790,45 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_CHECK_HIP(launch_row_topk(ix->dmat.as<float>(), nq, nlist, nlist, w, ix->probes.as<int32_t>(),
                                    ix->coadis.as<float>(), st));
     MIVF_TRY(tm.mark());
-    // 2. seed: exact scan of each query's ns[q] nearest lists (at least
-    // fast_seed, then as many as it takes to hold kl vectors), kl best -> tau
+    // 2. seed threshold from each query's ns[q] nearest lists (at least
+    // fast_seed, then as many as it takes to hold kl vectors)
     MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
+    MIVF_TRY(ix->f_tau.ensure(sizeof(float) * nq));
     MIVF_CHECK_HIP(launch_seed_count(ix->probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), nlist, kl,
                                      std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), st));
-    MIVF_TRY(ix->f_sids.ensure(sizeof(int32_t) * size_t(nq) * kl));
-    MIVF_TRY(ix->f_sdis.ensure(sizeof(float) * size_t(nq) * kl));
-    MIVF_TRY(ix->f_tau.ensure(sizeof(float) * nq));
     {
-        ScanArgs a{};
+        SeedArgs a{};
         a.queries = d_q;
         a.coarse = ix->coarse.as<float>();
         a.pq = ix->pq.as<float>();
-        a.pq_norm = ix->pq_n64.as<double>();
+        a.pnorm = ix->pq_n32.as<float>();
+        a.pmax = ix->pmax.as<float>();
         a.list_off = ix->off.as<int64_t>();
-        a.list_ids = ix->ids.as<int32_t>();
         a.codes = ix->codes.as<uint8_t>();
         a.probes = ix->probes.as<int32_t>();
-        a.wq = ix->f_ns.as<int32_t>();
+        a.nseed = ix->f_ns.as<int32_t>();
         a.nq = nq;
         a.w = w;
-        a.wstride = w;
-        a.k = kl;
-        a.kstride = kl;
-        a.ks = ix->ks;
         a.nlist = nlist;
-        a.ppg = w;
-        a.ngroups = 1;
-        a.err = ix->errflag.as<int>();
-        a.out_ids = ix->f_sids.as<int32_t>();
-        a.out_dis = ix->f_sdis.as<float>();
-        MIVF_CHECK_HIP(launch_scan(a, ix->m, st));
-        MIVF_CHECK_HIP(launch_seed_tau(ix->f_sdis.as<float>(), nq, kl, ix->f_tau.as<float>(), st));
+        a.ks = ix->ks;
+        a.kl = kl;
+        a.tau = ix->f_tau.as<float>();
+        MIVF_CHECK_HIP(launch_seed_tau(a, ix->m, st));
     }
     MIVF_TRY(tm.mark());
-    // 3. (list, query) pairs of ranks >= 1, grouped by list, and work items
+    // 3. all (list, query) pairs, grouped by list, and work items
     const int64_t np = int64_t(nq) * w;
     MIVF_TRY(ix->f_cnt.ensure(sizeof(int) * (nlist + 1)));
     MIVF_TRY(ix->f_poff.ensure(sizeof(int) * (nlist + 1)));
     MIVF_TRY(ix->f_ioff.ensure(sizeof(int) * (nlist + 1)));
     MIVF_TRY(ix->f_fill.ensure(sizeof(int) * (nlist + 1)));
     MIVF_TRY(ix->f_pairq.ensure(sizeof(int32_t) * std::max<int64_t>(np, 1)));
-    MIVF_TRY(ix->f_ilist.ensure(sizeof(int32_t) * std::max<int64_t>(np, 1)));
-    MIVF_TRY(ix->f_ipair.ensure(sizeof(int32_t) * std::max<int64_t>(np, 1)));
+    MIVF_TRY(ix->f_items.ensure(sizeof(FastItem) * std::max<int64_t>(np, 1)));
     MIVF_TRY(ix->f_ctrl.ensure(sizeof(int) * 4));
     int *ctrl = ix->f_ctrl.as<int>();  // [0] total items, [1] next item
-    MIVF_CHECK_HIP(launch_pairs(ix->probes.as<int32_t>(), nq, w, ix->f_ns.as<int32_t>(), nlist, ix->f_cnt.as<int>(), ix->f_poff.as<int>(),
-                                ix->f_ioff.as<int>(), ctrl, ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(),
-                                ix->f_ilist.as<int32_t>(), ix->f_ipair.as<int32_t>(), st));
+    MIVF_CHECK_HIP(launch_pairs(ix->probes.as<int32_t>(), nq, w, nullptr, ix->off.as<int64_t>(), ix->f_tau.as<float>(), nlist,
+                                ix->f_cnt.as<int>(), ix->f_poff.as<int>(), ix->f_ioff.as<int>(), ctrl,
+                                ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(), ix->f_items.as<FastItem>(), st));
     MIVF_TRY(ix->f_cand.ensure(sizeof(uint64_t) * size_t(nq) * capq));
     MIVF_TRY(ix->f_ccnt.ensure(sizeof(int) * nq));
     MIVF_CHECK_HIP(hipMemsetAsync(ix->f_ccnt.p, 0, sizeof(int) * nq, st));
@@ -833,18 +839,12 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         FastArgs a{};
         a.queries = d_q;
         a.coarse = ix->coarse.as<float>();
-        a.pq = ix->pq.as<float>();
+        a.pb = ix->fpb.p;
         a.pmax = ix->pmax.as<float>();
-        a.list_off = ix->off.as<int64_t>();
-        a.list_ids = ix->ids.as<int32_t>();
         a.codes = ix->codes.as<uint8_t>();
-        a.pair_q = ix->f_pairq.as<int32_t>();
-        a.list_pair_off = ix->f_poff.as<int>();
-        a.item_list = ix->f_ilist.as<int32_t>();
-        a.item_pair = ix->f_ipair.as<int32_t>();
+        a.items = ix->f_items.as<FastItem>();
         a.total_items = ctrl;
         a.next_item = ctrl + 1;
-        a.tau = ix->f_tau.as<float>();
         a.cand = ix->f_cand.as<uint64_t>();
         a.cand_cnt = ix->f_ccnt.as<int>();
         a.capq = capq;
@@ -858,10 +858,15 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(tm.mark());
     // 5. shortlist and exact rerank
     MIVF_TRY(ix->f_short.ensure(sizeof(int32_t) * size_t(nq) * kl));
-    MIVF_CHECK_HIP(launch_fast_select(ix->f_sids.as<int32_t>(), ix->f_sdis.as<float>(), kl, ix->f_cand.as<uint64_t>(),
-                                      ix->f_ccnt.as<int>(), capq, nq, ix->f_short.as<int32_t>(), st));
+    MIVF_CHECK_HIP(launch_fast_select(kl, ix->f_cand.as<uint64_t>(), ix->f_ccnt.as<int>(), capq, nq,
+                                      ix->f_short.as<int32_t>(), st));
     MIVF_TRY(tm.mark());
-    MIVF_CHECK_HIP(launch_rerank(d_base, nbase, d_q, nq, d, ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
+    if (d_base)
+        MIVF_CHECK_HIP(launch_rerank_pos(d_base, nbase, false, ix->ids.as<int32_t>(), d_q, nq, d,
+                                         ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
+    else
+        MIVF_CHECK_HIP(launch_rerank_pos(ix->rows.as<float>(), ix->ntotal, true, ix->ids.as<int32_t>(), d_q, nq, d,
+                                         ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
     MIVF_TRY(tm.mark());
     MIVF_TRY(finish_timer(ix, tm));
     if (!ix->scan_checked || ix->profile) {
@@ -880,6 +885,38 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         ix->f_last_cands = tot;
         ix->f_last_overflow = over;
     }
+    return kOk;
+}
+
+int mivf_index_attach_base(mivf_index *ix, const float *d_base, int64_t nbase) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(d_base && nbase >= 0, kInvalid, "attach_base: null base");
+    MIVF_TRY(upload_lists(ix));
+    int64_t maxid = -1;
+    for (const auto &v : ix->h_ids)
+        for (int32_t id : v) maxid = std::max<int64_t>(maxid, id);
+    MIVF_REQUIRE(maxid < nbase, kInvalid, "attach_base: the lists hold ids >= nbase");
+    MIVF_TRY(ix->rows.ensure(sizeof(float) * size_t(std::max<int64_t>(ix->ntotal, 1)) * ix->d));
+    MIVF_CHECK_HIP(launch_gather_rows(d_base, ix->d, ix->ids.as<int32_t>(), ix->ntotal, ix->rows.as<float>(), nullptr));
+    MIVF_CHECK_HIP(hipDeviceSynchronize());
+    ix->rows_ok = true;
+    return kOk;
+}
+
+int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t *d_t8, float *d_delta) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(ix->m == 8 || ix->m == 16, kUnsupported, "fast_tables: compiled for m in {8, 16}");
+    MIVF_REQUIRE(n >= 1 && n <= fast_group_size(), kInvalid, "fast_tables: need 1 <= n <= 16 queries");
+    MIVF_REQUIRE(list >= 0 && list < ix->nlist, kInvalid, "fast_tables: list out of range");
+    MIVF_REQUIRE(d_q && d_t8 && d_delta, kInvalid, "fast_tables: null pointer");
+    FastArgs a{};
+    a.queries = d_q;
+    a.coarse = ix->coarse.as<float>();
+    a.pb = ix->fpb.p;
+    a.pmax = ix->pmax.as<float>();
+    a.ks = ix->ks;
+    MIVF_CHECK_HIP(launch_fast_tables(a, ix->m, list, n, d_t8, d_delta, nullptr));
+    MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
 }
 

@@ -138,46 +138,72 @@ hipError_t launch_merge(const uint64_t *partial, int nq, int ngroups, int kstrid
 
 // Batched reduced-precision scan (list-major work items of up to 16 queries,
 // 8-bit tables, candidates appended to per-query buffers).
+// Work item of the reduced-precision scan: list l with gc <= 16 of the
+// queries that probe it (q[i] = -1 past gc) and their seed thresholds.
+struct FastItem {
+    int32_t l, gc, len, pad;
+    int64_t beg, pad2;
+    int32_t q[16];
+    float tau[16];
+};
 struct FastArgs {
     const float *queries;       // [nq][d]
     const float *coarse;        // [nlist][d]
-    const float *pq;            // [m][ks][ds]
+    const void *pb;             // fp16 [m][256][KP] table B operands (launch_fast_prep)
     const float *pmax;          // [m] max_c |p_sc|
-    const int64_t *list_off;    // [nlist+1]
-    const int32_t *list_ids;    // [N]
     const uint8_t *codes;       // [N][m]
-    const int32_t *pair_q;      // query of each (list, query) pair, list-major
-    const int *list_pair_off;   // [nlist+1]
-    const int32_t *item_list;   // work item -> list
-    const int32_t *item_pair;   // work item -> first pair
+    const FastItem *items;      // work items
     const int *total_items;
     int *next_item;             // work-queue counter (zeroed before launch)
-    const float *tau;           // [nq] seed threshold (exact ADC distance)
-    uint64_t *cand;             // [nq][capq] (approx key, id)
+    uint64_t *cand;             // [nq][capq] (approx key, list position)
     int *cand_cnt;              // [nq]
     int capq;
     int ks;
     int *err;
 };
 int fast_group_size();
+size_t fast_pb_bytes(int m);
+hipError_t launch_fast_prep(const float *pq, const float *pmax, int m, int ks, void *pb, hipStream_t s);
 // Seed lists per query: the smallest ns >= min_seed whose first ns probed
 // lists hold >= kl vectors (or ns = w).
 hipError_t launch_seed_count(const int32_t *probes, int nq, int w, const int64_t *list_off, int nlist, int kl,
                              int min_seed, int32_t *ns, hipStream_t s);
-// (list, query) pairs of probe ranks >= rank0[q], grouped by list, and the
+// (list, query) pairs of probe ranks >= rank0[q] (0 if rank0 is null), grouped by list, and the
 // work items (list, first pair) of at most fast_group_size() pairs.
-hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *rank0, int nlist, int *cnt,
-                        int *pair_off, int *item_off, int *total_items, int *fill, int32_t *pair_q,
-                        int32_t *item_list, int32_t *item_pair, hipStream_t s);
-hipError_t launch_seed_tau(const float *sdis, int nq, int kl, float *tau, hipStream_t s);
+hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *rank0, const int64_t *list_off,
+                        const float *tau, int nlist, int *cnt, int *pair_off, int *item_off, int *total_items,
+                        int *fill, int32_t *pair_q, FastItem *items, hipStream_t s);
+struct SeedArgs {
+    const float *queries, *coarse, *pq, *pnorm, *pmax;
+    const int64_t *list_off;
+    const uint8_t *codes;
+    const int32_t *probes;  // [nq][w]
+    const int32_t *nseed;   // [nq] seed lists per query
+    int nq, w, nlist, ks, kl;
+    float *tau;             // [nq]
+};
+// tau[q] >= the kl-th smallest ADC distance over q's seed lists (+inf when
+// they hold fewer than kl vectors).
+hipError_t launch_seed_tau(const SeedArgs &a, int m, hipStream_t s);
 hipError_t launch_fast_scan(const FastArgs &a, int m, int grid, hipStream_t s);
-hipError_t launch_fast_select(const int32_t *sids, const float *sdis, int kl, const uint64_t *cand,
-                              const int *cand_cnt, int capq, int nq, int32_t *short_ids, hipStream_t s);
+// The 8-bit tables and scales of one work item (queries 0..n-1 of a.queries,
+// n <= 16, list l): t8 [m][256][16] uint8, delta [16].  Test hook.
+hipError_t launch_fast_tables(const FastArgs &a, int m, int l, int n, uint8_t *t8, float *delta, hipStream_t s);
+hipError_t launch_fast_select(int kl, const uint64_t *cand, const int *cand_cnt, int capq, int nq,
+                              int32_t *short_ids, hipStream_t s);
 
 // Exact rerank of a shortlist against raw vectors (a = base row, b = query).
 hipError_t launch_rerank(const float *base, int64_t nbase, const float *queries, int nq,
                          int d, const int32_t *short_ids, int kl, int k,
                          int32_t *out_ids, float *out_dis, hipStream_t s);
+
+// Same with a shortlist of list positions (id = list_ids[pos]); rows are
+// base[id], or base[pos] of a list-ordered copy when list_ordered.
+hipError_t launch_rerank_pos(const float *rows, int64_t nrows, bool list_ordered, const int32_t *list_ids,
+                             const float *queries, int nq, int d, const int32_t *short_pos, int kl, int k,
+                             int32_t *out_ids, float *out_dis, hipStream_t s);
+// out[r] = base[ids[r]] (rows of d floats)
+hipError_t launch_gather_rows(const float *base, int d, const int32_t *ids, int64_t n, float *out, hipStream_t s);
 
 // Synthetic mixture rows (bit-identical to the CPU generator).
 uint64_t synth_stream_base(uint64_t seed, uint64_t stream);

@@ -58,7 +58,7 @@ def test_fast_exact_rows_m16(mivf, orc, zipf16, zipf16_gt, w, kl, k, seed_lists)
         ids2, dis2 = ix.search_fast(query, w, kl, k, base)
     finally:
         ix.set_option("fast_seed_lists", 1)
-        ix.set_option("fast_capacity", 8192)
+        ix.set_option("fast_capacity", 16384)
         ix.set_option("profile", 0)
     assert over == 0, f"{over} queries overflowed the candidate buffer"
     assert np.array_equal(ids, ids2) and np.array_equal(dis.view(np.uint32), dis2.view(np.uint32)), \
@@ -126,3 +126,65 @@ def test_fast_rejects_bad_shapes(mivf, c1):
         ix.search_fast(c1["query"], 8, 2048, 100, c1["base"])  # kl > 1024
     with pytest.raises(mivf.MivfError):
         ix.search_fast(c1["query"], 0, 64, 10, c1["base"])
+
+
+def expected_tables(orc, coarse, pq, q, l):
+    """Scaled reference tables x = T * 255 / bound (clipped to [0, 255]),
+    bound = max_s (|r_s| + max_c |p_sc|)^2 * 1.0001, and their rounding."""
+    m, ks, ds = pq.shape
+    pmax = np.sqrt((pq.astype(np.float64) ** 2).sum(-1)).max(1) * (1 + 1e-6)
+    x = np.zeros((m, ks, q.shape[0]), np.float64)
+    bounds = []
+    for g in range(q.shape[0]):
+        r = (q[g] - coarse[l]).astype(np.float32)
+        T = orc.tables(r, pq).astype(np.float64)  # [m][ks]
+        a = np.sqrt((r.astype(np.float64).reshape(m, ds) ** 2).sum(1))
+        bound = ((a + pmax) ** 2).max() * 1.0001
+        bounds.append(bound)
+        x[:, :, g] = np.clip(T * 255.0 / bound, 0, 255)
+    return x, np.rint(x).astype(np.int64), np.array(bounds) / 255.0
+
+
+@pytest.mark.parametrize("which", ["m16", "m8"])
+def test_fast_tables_match_reference_arithmetic(mivf, orc, zipf16, c1, which):
+    """The MFMA-built 8-bit tables: every entry within 0.75 quanta of the
+    scaled reference table value (0.5 rounding + 0.25 FP16 operand rounding,
+    the bound the candidate filter relies on), nearly all equal to its
+    rounding, and the deltas agree."""
+    if which == "m16":
+        ix, index, query, _ = zipf16
+        coarse, pq = index[0], index[1]
+    else:
+        q = c1["gold_q"]
+        coarse, pq = q["coarse"], q["pq"]
+        ix = mivf.Index(coarse, pq)
+        query = c1["query"]
+    probes, _ = ix.coarse(query[:16], 1)
+    for n, l in ((16, int(probes[0, 0])), (5, int(probes[3, 0])), (16, 0)):
+        t8, dl = ix.fast_tables(query[:n], l)
+        x, et8, edl = expected_tables(orc, coarse, pq, query[:n], l)
+        got = t8[:, :pq.shape[1], :n].astype(np.int64)
+        err = np.abs(got - x)
+        assert err.max() <= 0.75 + 1e-3, f"{which} list {l}: entry {err.max():.3f} quanta from the scaled table"
+        same = (got == et8).mean()
+        assert same >= 0.9, f"{which} list {l}: only {same:.4f} of the entries equal the rounded table"
+        np.testing.assert_allclose(dl[:n], edl, rtol=1e-5)
+
+
+def test_fast_attached_base(mivf, orc, c1):
+    """A list-ordered copy of the raw vectors gives the same results as
+    reranking against the caller's base; it is dropped when the lists change."""
+    q = c1["gold_q"]
+    base, query = c1["base"], c1["query"]
+    ix = mivf.Index(q["coarse"], q["pq"])
+    ix.add(base)
+    with pytest.raises(mivf.MivfError):
+        ix.search_fast(query, 8, 128, 100)          # nothing attached yet
+    ix.attach_base(base)
+    a_ids, a_dis = ix.search_fast(query, 8, 128, 100)
+    b_ids, b_dis = ix.search_fast(query, 8, 128, 100, base)
+    assert np.array_equal(a_ids, b_ids) and np.array_equal(a_dis.view(np.uint32), b_dis.view(np.uint32))
+    off, ids, codes = ix.get_lists()
+    ix.set_lists(off, ids, codes)
+    with pytest.raises(mivf.MivfError):
+        ix.search_fast(query, 8, 128, 100)          # stale copy dropped
