@@ -1,0 +1,392 @@
+// Coarse assignment (top-w lists per query) with the reference's exact FP32
+// arithmetic, computed in two steps:
+//
+// 1. coarse_approx_kernel: D'(q, c) = fl(|c|^2 + |q|^2) - 2 q.c with the dot
+//    products on the matrix cores (v_mfma_f32_16x16x4_f32, FP32 products,
+//    FP32 accumulation in MFMA order).
+// 2. coarse_select_kernel (one workgroup per query): v' = the w-th smallest
+//    D', then every centroid with D' <= v' + 2 M_q is re-evaluated with the
+//    exact reference arithmetic (sequential FP32 dot, a = centroid with its
+//    f32-accumulated norm, b = query with its f64-accumulated norm,
+//    D = fl(fl(-2 dot) + fl(|q|^2 + |c|^2)), parallel/ivf_pq/ivf_assign.cpp:28
+//    via knn_full) and the exact top-w by (distance, index) is kept.
+//
+// Why the recheck set holds the exact top-w: any two FP32 evaluations of a
+// 128-term dot product differ by at most ~2 * 128 * 2^-24 * sum|q_t c_t|
+// <= 1.6e-5 (|q|^2 + |c|^2) (sum|q_t c_t| <= (|q|^2 + |c|^2) / 2), and the
+// norm terms add a few ulps of |q|^2 + |c|^2, so |D' - D| <= M_q with
+// M_q = 1e-4 (|q|^2 + max_c |c|^2) (a factor ~5 of headroom).  w centroids
+// have D' <= v', hence D <= v' + M_q for them and the w-th exact distance is
+// <= v' + M_q; a centroid of the exact top-w therefore has D' <= D + M_q <=
+// v' + 2 M_q.  A centroid outside the set has D > v' + M_q >= the w-th exact
+// distance, so it loses to all of the top-w, ties included.
+#include "select_dev.h"
+
+namespace mivf {
+
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int CD = 128;       // dimension of the MFMA path
+constexpr int CP = CD + 4;    // padded LDS row (floats): conflict-free column reads
+constexpr int NTC = 256;
+constexpr int RCAP = 1024;    // recheck capacity per query
+
+// One 64-query x 256-centroid tile of D'.  The query tile sits in LDS; wave
+// wv owns centroids 64 wv .. 64 wv + 63 of the tile (four 16-wide column
+// tiles, operands read straight from global memory one k block ahead) and
+// all four 16-query row tiles.  MFMA step j of a 16-wide k block uses, for
+// lane group h = l/16, element k = k0 + 4h + j of both operands (a
+// permutation of the k sum), so one 16-byte read feeds four steps.
+constexpr int CQ = 64;   // queries per tile
+constexpr int CC = 256;  // centroids per tile
+
+__global__ __launch_bounds__(NTC) void coarse_approx_kernel(const float *__restrict__ coarse,
+                                                            const float *__restrict__ cn, int nlist,
+                                                            const float *__restrict__ queries,
+                                                            const double *__restrict__ qn, int nq,
+                                                            float *__restrict__ out, int64_t ldo) {
+    extern __shared__ __align__(16) float lds[];
+    float *Qs = lds;  // [CQ][CP]
+    const int tid = threadIdx.x;
+    const int64_t q0 = int64_t(blockIdx.y) * CQ;
+    for (int e = tid; e < CQ * CD / 4; e += NTC) {
+        const int row = e / (CD / 4), col = (e % (CD / 4)) * 4;
+        float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (q0 + row < nq) qv = *reinterpret_cast<const float4 *>(queries + (q0 + row) * CD + col);
+        *reinterpret_cast<float4 *>(Qs + row * CP + col) = qv;
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 4, i16 = lane & 15;
+    const int c0 = blockIdx.x * CC + wave * 64;
+    const float *crow[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) crow[t] = coarse + int64_t(min(c0 + 16 * t + i16, nlist - 1)) * CD + 4 * h;
+    f32x4 bn[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) bn[t] = *reinterpret_cast<const f32x4 *>(crow[t]);
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[a][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    __syncthreads();
+#pragma unroll 1
+    for (int k0 = 0; k0 < CD; k0 += 16) {
+        f32x4 b[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) b[t] = bn[t];
+        if (k0 + 16 < CD) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bn[t] = *reinterpret_cast<const f32x4 *>(crow[t] + k0 + 16);
+        }
+        f32x4 a[4];
+#pragma unroll
+        for (int qt = 0; qt < 4; ++qt) a[qt] = *reinterpret_cast<const f32x4 *>(Qs + (16 * qt + i16) * CP + k0 + 4 * h);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int qt = 0; qt < 4; ++qt)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[qt][j], b[t][j], acc[qt][t], 0, 0, 0);
+    }
+    // accumulator r of lane l: query row 16 qt + 4h + r, centroid column 16 t + l % 16
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int c = c0 + 16 * t + i16;
+        if (c >= nlist) continue;
+        const double cnc = double(cn[c]);
+#pragma unroll
+        for (int qt = 0; qt < 4; ++qt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t q = q0 + 16 * qt + 4 * h + r;
+                if (q < nq) out[q * ldo + c] = float(qn[q] + cnc) - 2.0f * acc[qt][t][r];
+            }
+    }
+}
+
+// Exact reference distance of centroid c to the query in LDS.
+__device__ inline float exact_dist(const float *__restrict__ crow, const float *qv, float an, double bn) {
+    float acc = 0.0f;
+#pragma unroll 4
+    for (int t = 0; t < CD; t += 4) {
+        const float4 c4 = *reinterpret_cast<const float4 *>(crow + t);
+        const float cc[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float p = cc[u] * qv[t + u];
+            acc = acc + p;
+        }
+    }
+    const float c = float(bn + double(an));
+    const float m2 = -2.0f * acc;
+    return m2 + c;
+}
+
+__global__ __launch_bounds__(NTC) void coarse_select_kernel(const float *__restrict__ coarse,
+                                                            const float *__restrict__ cn, float maxcn, int nlist,
+                                                            const float *__restrict__ queries,
+                                                            const double *__restrict__ qn,
+                                                            const float *__restrict__ approx, int64_t lda, int w,
+                                                            int32_t *__restrict__ probes,
+                                                            float *__restrict__ pdis,
+                                                            const int32_t *__restrict__ only,
+                                                            const int *__restrict__ only_cnt) {
+    if (only && int(blockIdx.x) >= *only_cnt) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *row = reinterpret_cast<float *>(smem);                        // [nlist] D', then exact D (fallback)
+    uint64_t *ck = reinterpret_cast<uint64_t *>(smem + ((size_t(nlist) * 4 + 15) & ~size_t(15)));  // [RCAP]
+    __shared__ SelectShared sh;
+    __shared__ uint64_t buf[RCAP];
+    __shared__ float qv[CD];
+    __shared__ int ncand, cnt, eqt;
+    const int tid = threadIdx.x;
+    const int64_t q = only ? only[blockIdx.x] : int64_t(blockIdx.x);
+    const double bn = qn[q];
+    if (tid < CD) qv[tid] = queries[q * CD + tid];
+    if (tid == 0) ncand = 0;
+    for (int c = tid; c < nlist; c += NTC) row[c] = approx[q * lda + c];
+    // An upper bound V of the w-th smallest D': a 256-bin histogram over
+    // [min, max], then V = the largest D' in the bins up to the one where the
+    // count reaches w (at least w values are <= V).
+    float mn = __builtin_huge_valf(), mx = -__builtin_huge_valf();
+    for (int c = tid; c < nlist; c += NTC) {
+        mn = fminf(mn, row[c]);
+        mx = fmaxf(mx, row[c]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, off));
+        mx = fmaxf(mx, __shfl_xor(mx, off));
+    }
+    __shared__ float wmn[NTC / 64], wmx[NTC / 64];
+    __shared__ uint32_t hist[256];
+    __shared__ int sdig, sbelow;
+    __shared__ uint32_t vmax;
+    if ((tid & 63) == 0) {
+        wmn[tid >> 6] = mn;
+        wmx[tid >> 6] = mx;
+    }
+    hist[tid] = 0u;
+    if (tid == 0) vmax = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NTC / 64; ++i) {
+        mn = fminf(mn, wmn[i]);
+        mx = fmaxf(mx, wmx[i]);
+    }
+    const float scale = mx > mn ? 256.0f / (mx - mn) : 0.0f;
+    auto bin = [&](float v) { return min(255, max(0, int((v - mn) * scale))); };
+    for (int c = tid; c < nlist; c += NTC) atomicAdd(&hist[bin(row[c])], 1u);
+    __syncthreads();
+    if (tid < 64) {
+        const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
+        const uint32_t tot = h0 + h1 + h2 + h3;
+        uint32_t incl = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, 64);
+            if (tid >= off) incl += o;
+        }
+        const uint32_t excl = incl - tot;
+        const uint32_t need = uint32_t(min(w, nlist));
+        if (excl < need && incl >= need) {
+            uint32_t c = excl;
+            int dg;
+            if (c + h0 >= need) dg = 0;
+            else if (c + h0 + h1 >= need) { c += h0; dg = 1; }
+            else if (c + h0 + h1 + h2 >= need) { c += h0 + h1; dg = 2; }
+            else { c += h0 + h1 + h2; dg = 3; }
+            sdig = 4 * tid + dg;
+            sbelow = int(c);
+        }
+    }
+    __syncthreads();
+    const int bmax = sdig;
+    uint32_t mine = 0u;
+    for (int c = tid; c < nlist; c += NTC)
+        if (bin(row[c]) <= bmax) mine = max(mine, fkey(row[c]));
+    atomicMax(&vmax, mine);
+    __syncthreads();
+    const float vw = key_dist(uint64_t(vmax) << 32);
+    auto akey = [&](int i) -> uint64_t { return ckey(row[i], i); };
+    const float lim = vw + 2.0e-4f * float(bn + double(maxcn));
+    // recheck set
+    for (int i = tid; i < nlist; i += NTC) {
+        if (row[i] <= lim) {
+            const int p = atomicAdd(&ncand, 1);
+            if (p < RCAP) ck[p] = uint64_t(i);
+        }
+    }
+    __syncthreads();
+    const int n = ncand;
+    SelectResult r;
+    int nk;
+    if (n <= RCAP) {
+        for (int i = tid; i < n; i += NTC) {
+            const int c = int(ck[i]);
+            ck[i] = ckey(exact_dist(coarse + int64_t(c) * CD, qv, cn[c], bn), c);
+        }
+        __syncthreads();
+        auto kf = [&](int i) { return ck[i]; };
+        nk = n;
+        r = block_select<NTC>(kf, nk, w, sh);
+    } else {
+        // recheck set too large: exact distances of every centroid
+        __syncthreads();
+        for (int c = tid; c < nlist; c += NTC) row[c] = exact_dist(coarse + int64_t(c) * CD, qv, cn[c], bn);
+        __syncthreads();
+        nk = nlist;
+        r = block_select<NTC>(akey, nk, w, sh);
+    }
+    // collect the w best into buf and sort
+    if (tid == 0) {
+        cnt = 0;
+        eqt = 0;
+    }
+    __syncthreads();
+    for (int i = tid; i < nk; i += NTC) {
+        const uint64_t k = (n <= RCAP) ? ck[i] : akey(i);
+        bool take = k < r.thr;
+        if (!take && r.eq > 0 && k == r.thr) take = atomicAdd(&eqt, 1) < r.eq;
+        if (take) {
+            const int p = atomicAdd(&cnt, 1);
+            if (p < RCAP) buf[p] = k;
+        }
+    }
+    __syncthreads();
+    const int c = min(cnt, RCAP);
+    const int np = next_pow2(c > 0 ? c : 1);
+    for (int i = c + tid; i < np; i += NTC) buf[i] = kNoKey;
+    __syncthreads();
+    block_bitonic_sort<NTC>(buf, np);
+    for (int i = tid; i < w; i += NTC) {
+        probes[q * w + i] = i < c ? key_id(buf[i]) : -1;
+        pdis[q * w + i] = i < c ? key_dist(buf[i]) : __builtin_huge_valf();
+    }
+}
+
+// One wavefront per query (nlist <= 64 * NPL, w <= 64): lane l holds the D'
+// of centroids l, l + 64, ...; the w-th smallest of the 64 lane minima is an
+// upper bound V of the w-th smallest D' (the minima are w distinct values).
+// The recheck set {D' <= V + 2 M_q} is compacted by ballots, evaluated with
+// the exact arithmetic, and its w best found by rank counting (keys are
+// distinct).  A query whose recheck set exceeds WCAP (or is short of w, which
+// only non-finite distances cause) goes to `redo` for the workgroup kernel.
+constexpr int NPL = 64;
+constexpr int WCAP = 256;
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(NTC) void coarse_select_wave_kernel(const float *__restrict__ coarse,
+                                                                 const float *__restrict__ cn, float maxcn,
+                                                                 int nlist, const float *__restrict__ queries,
+                                                                 const double *__restrict__ qn, int nq,
+                                                                 const float *__restrict__ approx, int64_t lda,
+                                                                 int w, int32_t *__restrict__ probes,
+                                                                 float *__restrict__ pdis,
+                                                                 int32_t *__restrict__ redo, int *__restrict__ redo_cnt) {
+    __shared__ uint32_t cidx_all[NTC / 64][WCAP];
+    __shared__ uint64_t keys_all[NTC / 64][WCAP];
+    __shared__ float qv_all[NTC / 64][CD];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t q = int64_t(blockIdx.x) * (NTC / 64) + wv;
+    if (q >= nq) return;  // wave-uniform; no workgroup barriers below
+    uint32_t *cidx = cidx_all[wv];
+    uint64_t *keys = keys_all[wv];
+    float *qv = qv_all[wv];
+    const double bn = qn[q];
+    qv[lane] = queries[q * CD + lane];
+    qv[lane + 64] = queries[q * CD + lane + 64];
+    float v[NPL];
+    float mn = __builtin_huge_valf();
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+        const int c = i * 64 + lane;
+        v[i] = c < nlist ? approx[q * lda + c] : __builtin_huge_valf();
+        mn = fminf(mn, v[i]);
+    }
+    // bitonic sort of the 64 lane minima across the wavefront (ascending)
+    float x = mn;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const float y = __shfl_xor(x, j);
+            const bool up = (lane & k) == 0;
+            const bool lower = (lane & j) == 0;
+            x = (lower == up) ? fminf(x, y) : fmaxf(x, y);
+        }
+    }
+    const float vw = __shfl(x, w - 1);
+    const float lim = vw + 2.0e-4f * float(bn + double(maxcn));
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+        const bool pass = v[i] <= lim;  // +inf padding never passes a finite limit
+        const uint64_t bal = __ballot(pass);
+        const int pos = n + int(__builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u)));
+        if (pass && pos < WCAP) cidx[pos] = uint32_t(i * 64 + lane);
+        n += __popcll(bal);
+    }
+    if (n > WCAP || n < w) {
+        if (lane == 0) redo[atomicAdd(redo_cnt, 1)] = int32_t(q);
+        return;
+    }
+    wave_sync();
+    for (int j = lane; j < n; j += 64) {
+        const int c = int(cidx[j]);
+        keys[j] = ckey(exact_dist(coarse + int64_t(c) * CD, qv, cn[c], bn), c);
+    }
+    wave_sync();
+    for (int j = lane; j < n; j += 64) {
+        const uint64_t k = keys[j];
+        int rank = 0;
+        for (int i = 0; i < n; ++i) rank += keys[i] < k;
+        if (rank < w) {
+            probes[q * w + rank] = key_id(k);
+            pdis[q * w + rank] = key_dist(k);
+        }
+    }
+}
+
+}  // namespace
+
+bool coarse_mfma_ok(int d, int nlist, int w) { return d == CD && nlist >= 1 && nlist <= 8192 && w >= 1 && w <= RCAP; }
+
+size_t coarse_select_lds(int nlist) { return ((size_t(nlist) * 4 + 15) & ~size_t(15)) + RCAP * 8; }
+
+hipError_t launch_coarse_approx(const float *coarse, const float *cn, int nlist, const float *queries,
+                                const double *qn, int nq, float *approx, int64_t lda, hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    dim3 grid(unsigned((nlist + CC - 1) / CC), unsigned((nq + CQ - 1) / CQ));
+    coarse_approx_kernel<<<grid, NTC, CQ * CP * 4, s>>>(coarse, cn, nlist, queries, qn, nq, approx, lda);
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxcn, int nlist, const float *queries,
+                                const double *qn, int nq, int w, const float *approx, int64_t lda, int32_t *probes,
+                                float *pdis, int32_t *redo, int *redo_cnt, hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    if (nlist >= 64 && nlist <= 64 * NPL && w <= 64 && redo && redo_cnt) {
+        hipError_t e = hipMemsetAsync(redo_cnt, 0, sizeof(int), s);
+        if (e != hipSuccess) return e;
+        coarse_select_wave_kernel<<<unsigned((nq + NTC / 64 - 1) / (NTC / 64)), NTC, 0, s>>>(
+            coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda, w, probes, pdis, redo, redo_cnt);
+        coarse_select_kernel<<<unsigned(nq), NTC, coarse_select_lds(nlist), s>>>(
+            coarse, cn, maxcn, nlist, queries, qn, approx, lda, w, probes, pdis, redo, redo_cnt);
+    } else {
+        coarse_select_kernel<<<unsigned(nq), NTC, coarse_select_lds(nlist), s>>>(
+            coarse, cn, maxcn, nlist, queries, qn, approx, lda, w, probes, pdis, nullptr, nullptr);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace mivf

@@ -75,6 +75,7 @@ struct mivf_index {
     int d = 0, nlist = 0, m = 0, ks = 0, ds = 0;
     // quantizers (device)
     DevBuf coarse, coarse_n32, pq, pq_n64, pq_n32;
+    float maxcn = 0.0f;      // max_c |c|^2 (margin of the coarse recheck)
     // lists: host mirror (list-major) + device arrays
     std::vector<std::vector<int32_t>> h_ids;
     std::vector<std::vector<uint8_t>> h_codes;
@@ -84,6 +85,7 @@ struct mivf_index {
     DevBuf off, ids, codes;
     // scratch
     DevBuf qn, dmat, probes, coadis, partial, wq, kq, tmp_ids, tmp_dis, keys, res, resn, assign;
+    DevBuf redo;             // coarse recheck: [0] count, then query ids
     DevBuf pmax;             // [m] max_c |p_sc| (reduced-precision table bound)
     DevBuf fpb;              // fp16 B operands of the reduced-precision tables
     DevBuf rows;             // raw vectors in list order (mivf_index_attach_base)
@@ -324,6 +326,32 @@ int check_index(const mivf_index *ix) {
     return kOk;
 }
 
+
+// Coarse top-w of nq queries into probes/dis [nq][w] (needs ix->dmat sized
+// nq * nlist): query norms, distances (timer mark), selection.
+int coarse_stage(mivf_index *ix, const float *d_q, int nq, int w, int32_t *probes, float *dis, hipStream_t st,
+                 Timer *tm) {
+    const int d = ix->d, nlist = ix->nlist;
+    MIVF_TRY(ix->qn.ensure(sizeof(double) * nq));
+    MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ix->qn.as<double>(), st));
+    if (coarse_mfma_ok(d, nlist, w) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0) {
+        MIVF_CHECK_HIP(launch_coarse_approx(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), nlist, d_q,
+                                            ix->qn.as<double>(), nq, ix->dmat.as<float>(), nlist, st));
+        if (tm) MIVF_TRY(tm->mark());
+        MIVF_TRY(ix->redo.ensure(sizeof(int32_t) * (size_t(nq) + 1)));
+        int32_t *redo = ix->redo.as<int32_t>();
+        MIVF_CHECK_HIP(launch_coarse_select(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), ix->maxcn, nlist,
+                                            d_q, ix->qn.as<double>(), nq, w, ix->dmat.as<float>(), nlist, probes,
+                                            dis, redo + 1, redo, st));
+    } else {
+        MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), nlist, d_q, d,
+                                         ix->qn.as<double>(), nq, d, ix->dmat.as<float>(), nlist, st));
+        if (tm) MIVF_TRY(tm->mark());
+        MIVF_CHECK_HIP(launch_row_topk(ix->dmat.as<float>(), nq, nlist, nlist, w, probes, dis, st));
+    }
+    return kOk;
+}
+
 int choose_groups(const mivf_index *ix, int nq, int wmax, int *ppg) {
     int p = ix->ppg_opt;
     if (p <= 0) {
@@ -350,12 +378,7 @@ int search_core(mivf_index *ix, const float *d_q, int nq, int wmax, const int32_
         MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nq) * nlist));
         MIVF_TRY(ix->probes.ensure(sizeof(int32_t) * size_t(nq) * wmax));
         MIVF_TRY(ix->coadis.ensure(sizeof(float) * size_t(nq) * wmax));
-        MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ix->qn.as<double>(), st));
-        MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), nlist, d_q, d,
-                                         ix->qn.as<double>(), nq, d, ix->dmat.as<float>(), nlist, st));
-        MIVF_TRY(tm.mark());
-        MIVF_CHECK_HIP(launch_row_topk(ix->dmat.as<float>(), nq, nlist, nlist, wmax,
-                                       ix->probes.as<int32_t>(), ix->coadis.as<float>(), st));
+        MIVF_TRY(coarse_stage(ix, d_q, nq, wmax, ix->probes.as<int32_t>(), ix->coadis.as<float>(), st, &tm));
         MIVF_TRY(tm.mark());
         probes = ix->probes.as<int32_t>();
     } else {
@@ -572,6 +595,14 @@ int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, cons
         set_error("index_create: HIP upload failed");
         return fail(kHip);
     }
+    {
+        std::vector<float> cn(nlist);
+        if (hipMemcpy(cn.data(), ix->coarse_n32.p, sizeof(float) * nlist, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("index_create: readback failed");
+            return fail(kHip);
+        }
+        for (float v : cn) ix->maxcn = std::max(ix->maxcn, v);
+    }
     if (fast_pb_bytes(m) > 0) {  // operands of the reduced-precision tables
         if ((rc = ix->fpb.ensure(fast_pb_bytes(m)))) return fail(rc);
         if (launch_fast_prep(ix->pq.as<float>(), ix->pmax.as<float>(), m, ks, ix->fpb.p, nullptr) != hipSuccess ||
@@ -705,12 +736,7 @@ int mivf_coarse(mivf_index *ix, const float *d_q, int nq, int w, int32_t *d_idx,
         const int nc = int(std::min<int64_t>(maxq, nq - q0));
         MIVF_TRY(ix->qn.ensure(sizeof(double) * nc));
         MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nc) * ix->nlist));
-        MIVF_CHECK_HIP(launch_norm_f64(d_q + q0 * ix->d, nc, ix->d, ix->d, ix->qn.as<double>(), st));
-        MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), ix->d, ix->coarse_n32.as<float>(), ix->nlist,
-                                         d_q + q0 * ix->d, ix->d, ix->qn.as<double>(), nc, ix->d,
-                                         ix->dmat.as<float>(), ix->nlist, st));
-        MIVF_CHECK_HIP(launch_row_topk(ix->dmat.as<float>(), nc, ix->nlist, ix->nlist, w, d_idx + q0 * w,
-                                       d_dis + q0 * w, st));
+        MIVF_TRY(coarse_stage(ix, d_q + q0 * ix->d, nc, w, d_idx + q0 * w, d_dis + q0 * w, st, nullptr));
     }
     return kOk;
 }
@@ -784,11 +810,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nq) * nlist));
     MIVF_TRY(ix->probes.ensure(sizeof(int32_t) * size_t(nq) * w));
     MIVF_TRY(ix->coadis.ensure(sizeof(float) * size_t(nq) * w));
-    MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ix->qn.as<double>(), st));
-    MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), nlist, d_q, d,
-                                     ix->qn.as<double>(), nq, d, ix->dmat.as<float>(), nlist, st));
-    MIVF_CHECK_HIP(launch_row_topk(ix->dmat.as<float>(), nq, nlist, nlist, w, ix->probes.as<int32_t>(),
-                                   ix->coadis.as<float>(), st));
+    MIVF_TRY(coarse_stage(ix, d_q, nq, w, ix->probes.as<int32_t>(), ix->coadis.as<float>(), st, nullptr));
     MIVF_TRY(tm.mark());
     // 2. seed threshold from each query's ns[q] nearest lists (at least
     // fast_seed, then as many as it takes to hold kl vectors)
