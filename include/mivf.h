@@ -162,6 +162,11 @@ int mivf_fast_stats(mivf_index *ix, int64_t *candidates, int64_t *overflowed_que
 int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t *d_t8,
                      float *d_delta);
 
+/* The seed thresholds of mivf_search_fast: d_tau [nq] >= the kl-th smallest
+ * ADC distance over each query's seed lists (+inf when they hold fewer than
+ * kl vectors).  For tests.                                                  */
+int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, float *d_tau);
+
 /* ---- options and timing ---------------------------------------------------
  * "probes_per_group" (0 = automatic), "profile" (1 = record per-stage HIP
  * event timings of the next searches), "fast_seed_lists" (lists per query

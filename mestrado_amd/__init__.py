@@ -59,6 +59,7 @@ _SIGS = {
     "mivf_fast_stats": [_vp, _i64p, _i64p],
     "mivf_index_attach_base": [_vp, _vp, _i64],
     "mivf_fast_tables": [_vp, _vp, _i, _i, _vp, _vp],
+    "mivf_fast_seed_tau": [_vp, _vp, _i, _i, _i, _vp],
     "mivf_set_option": [_vp, ctypes.c_char_p, _i64],
     "mivf_get_timings": [_vp, _f32p, _i],
     "mivf_synth_rows": [_u64, _u64, _i64, _i64, _i, _vp, _i, _vp, ctypes.c_float, _vp, _vp],
@@ -363,6 +364,13 @@ class Index:
         dl = DeviceArray((16,), np.float32)
         _check(lib().mivf_fast_tables(self.h, qd.ptr, qd.shape[0], int(list_no), t8.ptr, dl.ptr))
         return t8.numpy(), dl.numpy()
+
+    def fast_seed_tau(self, q, w, kl):
+        """Seed thresholds [nq] of the reduced-precision form."""
+        qd, _ = _dev(q, np.float32)
+        t = DeviceArray((qd.shape[0],), np.float32)
+        _check(lib().mivf_fast_seed_tau(self.h, qd.ptr, qd.shape[0], w, kl, t.ptr))
+        return t.numpy()
 
     def fast_stats(self):
         c = ctypes.c_int64(0)

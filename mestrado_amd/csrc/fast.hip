@@ -482,14 +482,36 @@ __global__ void seed_count_kernel(const int32_t *__restrict__ probes, int nq, in
     ns[q] = r;
 }
 
-// ---- list-major grouping of (query, probe) pairs with rank >= rank0[q]
-__global__ void pair_count_kernel(const int32_t *__restrict__ probes, int nq, int w,
-                                  const int32_t *__restrict__ rank0, int nlist, int *__restrict__ cnt) {
-    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= int64_t(nq) * w) return;
+// ---- list-major grouping of (query, probe) pairs with rank >= rank0[q].
+// Each workgroup aggregates its PCH pairs in an LDS histogram over the lists
+// (nlist <= LBINS) and issues one global atomic per list it touches, so
+// popular lists do not serialise thousands of atomics.
+constexpr int LBINS = 8192;
+constexpr int PCH = 2048;
+
+__device__ inline bool pair_ok(const int32_t *probes, int64_t e, int w, const int32_t *rank0, int nlist, int *l) {
     const int r = int(e % w);
-    const int l = probes[e];
-    if (r >= (rank0 ? rank0[e / w] : 0) && l >= 0 && l < nlist) atomicAdd(cnt + l, 1);
+    *l = probes[e];
+    return r >= (rank0 ? rank0[e / w] : 0) && *l >= 0 && *l < nlist;
+}
+
+__global__ __launch_bounds__(256) void pair_count_kernel(const int32_t *__restrict__ probes, int nq, int w,
+                                                         const int32_t *__restrict__ rank0, int nlist,
+                                                         int *__restrict__ cnt) {
+    __shared__ int hc[LBINS];
+    const int tid = threadIdx.x;
+    const int64_t np = int64_t(nq) * w;
+    const int64_t e0 = int64_t(blockIdx.x) * PCH;
+    for (int i = tid; i < nlist; i += 256) hc[i] = 0;
+    __syncthreads();
+    for (int i = tid; i < PCH; i += 256) {
+        const int64_t e = e0 + i;
+        int l;
+        if (e < np && pair_ok(probes, e, w, rank0, nlist, &l)) atomicAdd(&hc[l], 1);
+    }
+    __syncthreads();
+    for (int i = tid; i < nlist; i += 256)
+        if (hc[i]) atomicAdd(cnt + i, hc[i]);
 }
 
 // exclusive scans of pair counts and of work items per list (one workgroup)
@@ -531,14 +553,31 @@ __global__ void pair_scan_kernel(const int *__restrict__ cnt, int nlist, int *__
     }
 }
 
-__global__ void pair_scatter_kernel(const int32_t *__restrict__ probes, int nq, int w,
-                                    const int32_t *__restrict__ rank0, int nlist, const int *__restrict__ pair_off,
-                                    int *__restrict__ fill, int32_t *__restrict__ pair_q) {
-    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= int64_t(nq) * w) return;
-    const int r = int(e % w);
-    const int l = probes[e];
-    if (r >= (rank0 ? rank0[e / w] : 0) && l >= 0 && l < nlist) pair_q[pair_off[l] + atomicAdd(fill + l, 1)] = int32_t(e / w);
+__global__ __launch_bounds__(256) void pair_scatter_kernel(const int32_t *__restrict__ probes, int nq, int w,
+                                                           const int32_t *__restrict__ rank0, int nlist,
+                                                           const int *__restrict__ pair_off, int *__restrict__ fill,
+                                                           int32_t *__restrict__ pair_q) {
+    __shared__ int hc[LBINS];
+    __shared__ int rk[PCH];
+    const int tid = threadIdx.x;
+    const int64_t np = int64_t(nq) * w;
+    const int64_t e0 = int64_t(blockIdx.x) * PCH;
+    for (int i = tid; i < nlist; i += 256) hc[i] = 0;
+    __syncthreads();
+    for (int i = tid; i < PCH; i += 256) {
+        const int64_t e = e0 + i;
+        int l;
+        rk[i] = (e < np && pair_ok(probes, e, w, rank0, nlist, &l)) ? atomicAdd(&hc[l], 1) : -1;
+    }
+    __syncthreads();
+    for (int i = tid; i < nlist; i += 256)
+        if (hc[i]) hc[i] = pair_off[i] + atomicAdd(fill + i, hc[i]);  // this workgroup's range start
+    __syncthreads();
+    for (int i = tid; i < PCH; i += 256) {
+        if (rk[i] < 0) continue;
+        const int64_t e = e0 + i;
+        pair_q[hc[probes[e]] + rk[i]] = int32_t(e / w);
+    }
 }
 
 // One thread per (work item, query slot); items are numbered list by list
@@ -618,6 +657,62 @@ __device__ inline void find_bin(const uint32_t *hist, int need, int *digit, int 
     }
 }
 
+// FP32 table of query q against list l into tab (256 threads; uniform call,
+// ends with a barrier); also returns sum_s (|r_s| + max|p_s|)^2 to thread 0.
+template <int M>
+__device__ float seed_table(const float *__restrict__ queries, const float *__restrict__ coarse,
+                            const float *__restrict__ pq, const float *__restrict__ pnorm,
+                            const float *__restrict__ pmax, int64_t q, int l, int ks, float *tab, float *r,
+                            float *As) {
+    constexpr int DS = D / M;
+    const int tid = threadIdx.x;
+    __syncthreads();  // the previous table is no longer read
+    if (tid < D) r[tid] = queries[q * D + tid] - coarse[int64_t(l) * D + tid];
+    __syncthreads();
+    if (tid < M) {
+        float a = 0.0f;
+#pragma unroll
+        for (int t = 0; t < DS; ++t) a = __builtin_fmaf(r[tid * DS + t], r[tid * DS + t], a);
+        As[tid] = a;
+    }
+    __syncthreads();
+    float mg = 0.0f;
+    if (tid == 0)
+        for (int s2 = 0; s2 < M; ++s2) {
+            const float b = __builtin_sqrtf(As[s2]) + pmax[s2];
+            mg += b * b;
+        }
+    if (tid < ks) {
+        // four subquantizers per batch: their loads are issued together
+#pragma unroll 1
+        for (int s0 = 0; s0 < M; s0 += 4) {
+            float p[4][DS], pn[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float4 *p4 = reinterpret_cast<const float4 *>(pq + (int64_t(s0 + u) * ks + tid) * DS);
+#pragma unroll
+                for (int t = 0; t < DS / 4; ++t) {
+                    const float4 x = p4[t];
+                    p[u][4 * t] = x.x; p[u][4 * t + 1] = x.y; p[u][4 * t + 2] = x.z; p[u][4 * t + 3] = x.w;
+                }
+                pn[u] = pnorm[(s0 + u) * ks + tid];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int s2 = s0 + u;
+                float dot = 0.0f;
+#pragma unroll
+                for (int t = 0; t < DS; ++t) dot = __builtin_fmaf(r[s2 * DS + t], p[u][t], dot);
+                tab[s2 * 256 + tid] = As[s2] + pn[u] - 2.0f * dot;
+            }
+        }
+    }
+    __syncthreads();
+    return mg;
+}
+
+constexpr int SCAP = 8192;  // seed keys held in LDS
+
 template <int M>
 __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__ queries,
                                                        const float *__restrict__ coarse,
@@ -628,82 +723,116 @@ __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__
                                                        const int32_t *__restrict__ probes, int w,
                                                        const int32_t *__restrict__ nseed, int nlist, int ks, int kl,
                                                        float *__restrict__ tau) {
-    constexpr int DS = D / M;
     __shared__ float tab[M * 256];
     __shared__ float r[D];
     __shared__ float As[M];
     __shared__ uint32_t hist[256];
+    __shared__ uint32_t skeys[SCAP];
     __shared__ int sdig, sbelow;
-    __shared__ float smarg;
-    const int q = blockIdx.x;
+    const int64_t q = blockIdx.x;
     const int tid = threadIdx.x;
     const int ns = nseed[q];
     int64_t tot = 0;
     for (int rr = 0; rr < ns; ++rr) {
-        const int l = probes[int64_t(q) * w + rr];
+        const int l = probes[q * w + rr];
         if (l >= 0 && l < nlist) tot += list_off[l + 1] - list_off[l];
     }
     if (tot < kl) {  // uniform
         if (tid == 0) tau[q] = __builtin_huge_valf();
         return;
     }
-    if (tid == 0) smarg = 0.0f;
-    uint32_t prefix = 0;
-    int need = kl;
-    for (int pass = 0; pass < 3; ++pass) {
-        hist[tid] = 0u;  // 256 threads, 256 bins
-        const int sh = 24 - 8 * pass;
+    const bool inl = tot <= SCAP;  // keys computed once, kept in LDS
+    float marg = 0.0f;
+    if (inl) {
+        int off = 0;
         for (int rr = 0; rr < ns; ++rr) {
-            const int l = probes[int64_t(q) * w + rr];
+            const int l = probes[q * w + rr];
             if (l < 0 || l >= nlist) continue;  // uniform
-            if (pass == 0 || ns > 1) {
-                __syncthreads();  // previous list's scan is done with tab
-                if (tid < D) r[tid] = queries[int64_t(q) * D + tid] - coarse[int64_t(l) * D + tid];
-                __syncthreads();
-                if (tid < M) {
-                    float a = 0.0f;
-#pragma unroll
-                    for (int t = 0; t < DS; ++t) a = __builtin_fmaf(r[tid * DS + t], r[tid * DS + t], a);
-                    As[tid] = a;
-                }
-                __syncthreads();
-                if (tid == 0 && pass == 0) {
-                    float mg = 0.0f;
-                    for (int s2 = 0; s2 < M; ++s2) {
-                        const float b = __builtin_sqrtf(As[s2]) + pmax[s2];
-                        mg += b * b;
-                    }
-                    smarg = fmaxf(smarg, mg);
-                }
-                if (tid < ks) {
-                    for (int s2 = 0; s2 < M; ++s2) {
-                        const float *p = pq + (int64_t(s2) * ks + tid) * DS;
-                        float dot = 0.0f;
-#pragma unroll
-                        for (int t = 0; t < DS; ++t) dot = __builtin_fmaf(r[s2 * DS + t], p[t], dot);
-                        tab[s2 * 256 + tid] = As[s2] + pnorm[s2 * ks + tid] - 2.0f * dot;
-                    }
-                }
-            }
-            __syncthreads();
+            marg = fmaxf(marg, seed_table<M>(queries, coarse, pq, pnorm, pmax, q, l, ks, tab, r, As));
             const int64_t beg = list_off[l];
             const int len = int(list_off[l + 1] - beg);
             const uint8_t *lc = codes + beg * M;
-            for (int v = tid; v < len; v += 256) {
-                const uint32_t key = fkey(seed_adc<M>(tab, lc, v));
-                if (pass == 0 || (key >> (sh + 8)) == (prefix >> (sh + 8))) atomicAdd(&hist[(key >> sh) & 255u], 1u);
+            // four vectors per thread per round: their code loads are issued together
+            for (int v0 = tid; v0 < len; v0 += 1024) {
+                uint32_t cw[4][M / 4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) load_codes<M>(lc, min(v0 + 256 * u, len - 1), cw[u]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int s2 = 0; s2 < M; ++s2) acc += tab[s2 * 256 + ((cw[u][s2 >> 2] >> (8 * (s2 & 3))) & 255u)];
+                    if (v0 + 256 * u < len) skeys[off + v0 + 256 * u] = fkey(acc);
+                }
+            }
+            off += len;
+        }
+    }
+    uint32_t prefix = 0;
+    int need = kl;
+    int sh0 = 24, npass = 3;
+    if (inl) {
+        // the radix passes start below the bits that all keys share
+        __shared__ uint32_t kmn_w[4], kmx_w[4];
+        __syncthreads();
+        uint32_t kmn = 0xFFFFFFFFu, kmx = 0u;
+        for (int i = tid; i < int(tot); i += 256) {
+            kmn = min(kmn, skeys[i]);
+            kmx = max(kmx, skeys[i]);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            kmn = min(kmn, uint32_t(__shfl_xor(int(kmn), off)));
+            kmx = max(kmx, uint32_t(__shfl_xor(int(kmx), off)));
+        }
+        if ((tid & 63) == 0) {
+            kmn_w[tid >> 6] = kmn;
+            kmx_w[tid >> 6] = kmx;
+        }
+        __syncthreads();
+        kmn = min(min(kmn_w[0], kmn_w[1]), min(kmn_w[2], kmn_w[3]));
+        kmx = max(max(kmx_w[0], kmx_w[1]), max(kmx_w[2], kmx_w[3]));
+        const int cp = kmn == kmx ? 32 : __clz(int(kmn ^ kmx));  // shared leading bits
+        prefix = cp == 32 ? kmn : (cp == 0 ? 0u : (kmn >> (32 - cp)) << (32 - cp));
+        sh0 = max(0, 24 - cp);  // digit = bits [sh0, sh0 + 8) sit right below the shared ones
+        npass = cp == 32 ? 0 : min(3, sh0 / 8 + 1);
+    }
+    int sh = sh0;
+    for (int pass = 0; pass < npass; ++pass, sh -= 8) {
+        __syncthreads();  // keys written / previous pass's bin read
+        hist[tid] = 0u;   // 256 threads, 256 bins
+        __syncthreads();
+        if (inl) {
+            for (int i = tid; i < int(tot); i += 256) {
+                const uint32_t key = skeys[i];
+                if (sh + 8 >= 32 || (key >> (sh + 8)) == (prefix >> (sh + 8))) atomicAdd(&hist[(key >> sh) & 255u], 1u);
+            }
+        } else {
+            for (int rr = 0; rr < ns; ++rr) {
+                const int l = probes[q * w + rr];
+                if (l < 0 || l >= nlist) continue;  // uniform
+                const float mg = seed_table<M>(queries, coarse, pq, pnorm, pmax, q, l, ks, tab, r, As);
+                if (pass == 0) marg = fmaxf(marg, mg);
+                const int64_t beg = list_off[l];
+                const int len = int(list_off[l + 1] - beg);
+                const uint8_t *lc = codes + beg * M;
+                for (int v = tid; v < len; v += 256) {
+                    const uint32_t key = fkey(seed_adc<M>(tab, lc, v));
+                    if (sh + 8 >= 32 || (key >> (sh + 8)) == (prefix >> (sh + 8))) atomicAdd(&hist[(key >> sh) & 255u], 1u);
+                }
             }
         }
         __syncthreads();
         find_bin(hist, need, &sdig, &sbelow);
         __syncthreads();
-        prefix |= uint32_t(sdig) << sh;
+        prefix = (prefix & ~(0xFFu << sh)) | (uint32_t(sdig) << sh);
         need -= sbelow;
-        __syncthreads();  // sdig/sbelow and hist are rewritten by the next pass
     }
     if (tid == 0) {
-        const float top = key_dist(uint64_t(prefix | 0xFFu) << 32);
-        tau[q] = top + smarg * (1.0f / 16384.0f);
+        // the largest key of the final bucket (all bits below the last digit set)
+        const uint32_t low = npass == 0 ? 0u : ((sh + 8) >= 32 ? 0xFFFFFFFFu : ((1u << (sh + 8)) - 1u));
+        const float top = key_dist(uint64_t(prefix | low) << 32);
+        tau[q] = top + marg * (1.0f / 16384.0f);
     }
 }
 
@@ -807,9 +936,10 @@ hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *ran
     hipError_t e = hipMemsetAsync(cnt, 0, sizeof(int) * nlist, s);
     if (e != hipSuccess) return e;
     const int64_t np = int64_t(nq) * w;
-    pair_count_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rank0, nlist, cnt);
+    if (nlist > LBINS) return hipErrorInvalidValue;
+    pair_count_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rank0, nlist, cnt);
     pair_scan_kernel<<<1, 1024, 0, s>>>(cnt, nlist, pair_off, item_off, total_items, fill);
-    pair_scatter_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rank0, nlist, pair_off, fill, pair_q);
+    pair_scatter_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rank0, nlist, pair_off, fill, pair_q);
     const int64_t max_slots = (np / FG + nlist) * FG;  // >= FG * total items
     item_expand_kernel<<<nblk(max_slots, 256), 256, 0, s>>>(pair_off, item_off, list_off, pair_q, tau, nlist,
                                                           total_items, max_slots, items);

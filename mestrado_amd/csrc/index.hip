@@ -789,6 +789,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
                      int64_t nbase, int32_t *d_ids, float *d_dis, void *stream) {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(ix->m == 8 || ix->m == 16, kUnsupported, "search_fast: compiled for m in {8, 16}");
+    MIVF_REQUIRE(ix->nlist <= 8192, kUnsupported, "search_fast: compiled for nlist <= 8192");
     MIVF_REQUIRE(w >= 1 && w <= ix->nlist && w <= 1024, kInvalid, "search_fast: w must be in 1..min(nlist,1024)");
     MIVF_REQUIRE(kl >= 1 && kl <= 1024 && k >= 1 && k <= kl, kInvalid, "search_fast: need 1 <= k <= kl <= 1024");
     if (nq <= 0) return kOk;
@@ -938,6 +939,42 @@ int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t 
     a.pmax = ix->pmax.as<float>();
     a.ks = ix->ks;
     MIVF_CHECK_HIP(launch_fast_tables(a, ix->m, list, n, d_t8, d_delta, nullptr));
+    MIVF_CHECK_HIP(hipDeviceSynchronize());
+    return kOk;
+}
+
+int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, float *d_tau) {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(ix->m == 8 || ix->m == 16, kUnsupported, "fast_seed_tau: compiled for m in {8, 16}");
+    MIVF_REQUIRE(w >= 1 && w <= ix->nlist && w <= 1024 && kl >= 1, kInvalid, "fast_seed_tau: bad w or kl");
+    MIVF_REQUIRE(d_q && d_tau && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0, kInvalid,
+                 "fast_seed_tau: null or unaligned pointer");
+    if (nq <= 0) return kOk;
+    MIVF_TRY(upload_lists(ix));
+    MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nq) * ix->nlist));
+    MIVF_TRY(ix->probes.ensure(sizeof(int32_t) * size_t(nq) * w));
+    MIVF_TRY(ix->coadis.ensure(sizeof(float) * size_t(nq) * w));
+    MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
+    MIVF_TRY(coarse_stage(ix, d_q, nq, w, ix->probes.as<int32_t>(), ix->coadis.as<float>(), nullptr, nullptr));
+    MIVF_CHECK_HIP(launch_seed_count(ix->probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), ix->nlist, kl,
+                                     std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), nullptr));
+    SeedArgs a{};
+    a.queries = d_q;
+    a.coarse = ix->coarse.as<float>();
+    a.pq = ix->pq.as<float>();
+    a.pnorm = ix->pq_n32.as<float>();
+    a.pmax = ix->pmax.as<float>();
+    a.list_off = ix->off.as<int64_t>();
+    a.codes = ix->codes.as<uint8_t>();
+    a.probes = ix->probes.as<int32_t>();
+    a.nseed = ix->f_ns.as<int32_t>();
+    a.nq = nq;
+    a.w = w;
+    a.nlist = ix->nlist;
+    a.ks = ix->ks;
+    a.kl = kl;
+    a.tau = d_tau;
+    MIVF_CHECK_HIP(launch_seed_tau(a, ix->m, nullptr));
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
 }

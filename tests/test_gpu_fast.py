@@ -188,3 +188,47 @@ def test_fast_attached_base(mivf, orc, c1):
     ix.set_lists(off, ids, codes)
     with pytest.raises(mivf.MivfError):
         ix.search_fast(query, 8, 128, 100)          # stale copy dropped
+
+
+@pytest.mark.parametrize("which,kl,seed_lists", [("m16", 256, 1), ("m16", 64, 2), ("m8", 128, 1), ("m8", 1000, 1)])
+def test_fast_seed_threshold(mivf, orc, zipf16, c1, which, kl, seed_lists):
+    """tau_q bounds the kl-th smallest reference ADC distance over the query's
+    seed lists from above (the filter keeps everything up to it) and is
+    tight (within 0.1%)."""
+    if which == "m16":
+        ix, index, query, _ = zipf16
+    else:
+        q = c1["gold_q"]
+        ix = mivf.Index(q["coarse"], q["pq"])
+        ix.add(c1["base"])
+        index = (q["coarse"], q["pq"]) + tuple(ix.get_lists())
+        query = c1["query"]
+    coarse, pq, off, ids, codes = index
+    w = 16
+    query = query[:40]
+    ix.set_option("fast_seed_lists", seed_lists)
+    try:
+        tau = ix.fast_seed_tau(query, w, kl)
+    finally:
+        ix.set_option("fast_seed_lists", 1)
+    probes, _ = ix.coarse(query, w)
+    lens = np.diff(off)
+    for qi in range(query.shape[0]):
+        have, ns = 0, 0
+        while ns < w and (ns < seed_lists or have < kl):
+            have += lens[probes[qi, ns]]
+            ns += 1
+        vals = []
+        for r in range(ns):
+            l = probes[qi, r]
+            if lens[l] == 0:
+                continue
+            tab = orc.tables((query[qi] - coarse[l]).astype(np.float32), pq)
+            vals.append(orc.adc(tab, codes[off[l]:off[l + 1]]))
+        vals = np.sort(np.concatenate(vals)) if vals else np.zeros(0, np.float32)
+        if len(vals) < kl:
+            assert np.isinf(tau[qi]), f"q={qi}: fewer than kl seed vectors but finite tau"
+            continue
+        kth = float(vals[kl - 1])
+        assert tau[qi] >= kth, f"q={qi}: tau {tau[qi]} below the kl-th seed distance {kth}"
+        assert tau[qi] <= kth * 1.001 + 1.0, f"q={qi}: tau {tau[qi]} loose against {kth}"
