@@ -37,6 +37,9 @@ def main():
     ap.add_argument("--fast-kl", type=int, default=128, help="shortlist of the reduced-precision form")
     ap.add_argument("--fast-seed", type=int, default=1, help="lists scanned exactly to seed its filter")
     ap.add_argument("--no-fast", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-fracs", default="0.25,0.5,0.75", help="offered load as fractions of the batch QPS")
+    ap.add_argument("--serve-reqs", type=int, default=12000)
     ap.add_argument("--profile-only", action="store_true",
                     help="build, run the timed steps and a 1 GiB device copy (counter calibration), then exit")
     args = ap.parse_args()
@@ -123,6 +126,23 @@ def main():
                 "stages_ms": {n: round(float(v), 4) for n, v in
                               zip(("coarse", "seed", "pairs", "scan8", "select", "rerank"), fstage)}}
         log(f"fast form: {fast}")
+    serving = None
+    if not args.no_serve and not args.profile_only:
+        # open-loop Poisson requests (nq log-uniform 1..1024, mean ~148) against the
+        # continuous-batching server, exact form, w and k of the headline
+        qps_peak = nq * args.steps / elapsed
+        mean_nq = 147.6
+        pts = []
+        for frac in [float(x) for x in args.serve_fracs.split(",")]:
+            st = M.serve_bench(ix, query, rate=frac * qps_peak / mean_nq, nreq=args.serve_reqs, seed=11,
+                               bmax=1024, w_choices=(w,), k_choices=(k,), slots=4, max_batch=2048)
+            pts.append({"load": frac, "offered_qps": round(st["offered_qps"], 1), "qps": round(st["qps"], 1),
+                        "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                        "mean_ms": round(st["mean_ms"], 3), "batches": int(st["batches"])})
+            log(f"serving at {frac:.2f} of peak: {pts[-1]}")
+        serving = {"form": "plain exact FP32 (per-request w, k)", "w": w, "k": k, "slots": 4, "max_batch": 2048,
+                   "requests_per_point": args.serve_reqs, "batch_sizes": "log-uniform 1..1024",
+                   "arrivals": "Poisson (open loop)", "points": pts}
     if args.profile_only:
         # a plain device-to-device copy of known size: the reference point
         # that scales the HBM byte counters of the search kernels
@@ -191,6 +211,7 @@ def main():
                      "kernel": "scan_kernel", "algorithmic_bytes_per_launch": scan_bytes},
         "cpu_baseline": cpu,
         "fast_form": fast,
+        "serving": serving,
     }
     print(json.dumps(res), flush=True)
 

@@ -178,6 +178,35 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value);
  * [5] rerank (mivf_search_fast).                                            */
 int mivf_get_timings(mivf_index *ix, float *ms, int n);
 
+/* ---- serving loop (continuous batching) -----------------------------------
+ * Replaces the search stage of the reference's MPI query pipeline (query
+ * sender -> search ranks -> aggregator, parallel/ivf_test.cpp) on one GPU.
+ * Requests (device queries d_q [nq][d] with their own w and k, device outputs
+ * [nq][k]) wait in arrival order; up to `slots` micro-batches run at once,
+ * each on its own stream with its own workspace, packed from the waiting
+ * requests up to max_batch queries and within mem_budget bytes of per-request
+ * device state (0 = 4 GiB).  Searches are exact (mivf_search_var kernels), so
+ * a request's results are the same alone and in any mix.  The server is driven
+ * by the calling thread: submit, then progress / wait / drain.                */
+typedef struct mivf_server mivf_server;
+int mivf_server_create(mivf_index *ix, int slots, int max_batch, int64_t mem_budget, mivf_server **out);
+int mivf_server_free(mivf_server *sv);
+int mivf_server_submit(mivf_server *sv, const float *d_q, int nq, int w, int k, int32_t *d_ids,
+                       float *d_dis, int64_t *ticket);
+int mivf_server_progress(mivf_server *sv, int *completed);
+int mivf_server_wait(mivf_server *sv, int64_t ticket);
+int mivf_server_drain(mivf_server *sv);
+/* Submission-to-completion time of a finished request, milliseconds.       */
+int mivf_server_latency(mivf_server *sv, int64_t ticket, double *ms);
+/* Open-loop benchmark: nreq requests, Poisson arrivals at `rate` requests/s,
+ * nq log-uniform in 1..bmax, w and k drawn from the given choices, queries at
+ * random rows of d_pool [pool_n][d].  stats[8] = achieved queries/s, p50 ms,
+ * p99 ms, mean ms, queries served, makespan ms, micro-batches, offered
+ * queries/s (latency counted from each request's scheduled arrival).       */
+int mivf_serve_bench(mivf_index *ix, int slots, int max_batch, int64_t mem_budget, const float *d_pool,
+                     int64_t pool_n, double rate, int nreq, uint64_t seed, int bmax,
+                     const int32_t *w_choices, int nw, const int32_t *k_choices, int nk, double *stats);
+
 /* ---- synthetic data and ground truth ------------------------------------ */
 /* Seeded Gaussian mixture rows; bit-identical to the CPU generator.
  * d_centres [ncomp][d]; d_thr NULL (uniform components) or [ncomp] Zipf

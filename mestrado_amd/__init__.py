@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-__all__ = ["lib", "MivfError", "DeviceArray", "Index", "train", "kmeans",
+__all__ = ["lib", "MivfError", "DeviceArray", "Index", "Server", "serve_bench", "train", "kmeans",
            "synth_centres", "zipf_thresholds", "synth_rows", "bruteforce_nn",
            "device_count", "synchronize", "LIB_PATH"]
 
@@ -60,6 +60,15 @@ _SIGS = {
     "mivf_index_attach_base": [_vp, _vp, _i64],
     "mivf_fast_tables": [_vp, _vp, _i, _i, _vp, _vp],
     "mivf_fast_seed_tau": [_vp, _vp, _i, _i, _i, _vp],
+    "mivf_server_create": [_vp, _i, _i, _i64, ctypes.POINTER(_vp)],
+    "mivf_server_free": [_vp],
+    "mivf_server_submit": [_vp, _vp, _i, _i, _i, _vp, _vp, _i64p],
+    "mivf_server_progress": [_vp, ctypes.POINTER(_i)],
+    "mivf_server_wait": [_vp, _i64],
+    "mivf_server_drain": [_vp],
+    "mivf_server_latency": [_vp, _i64, ctypes.POINTER(ctypes.c_double)],
+    "mivf_serve_bench": [_vp, _i, _i, _i64, _vp, _i64, ctypes.c_double, _i, _u64, _i, _vp, _i, _vp, _i,
+                         ctypes.POINTER(ctypes.c_double)],
     "mivf_set_option": [_vp, ctypes.c_char_p, _i64],
     "mivf_get_timings": [_vp, _f32p, _i],
     "mivf_synth_rows": [_u64, _u64, _i64, _i64, _i, _vp, _i, _vp, ctypes.c_float, _vp, _vp],
@@ -387,3 +396,66 @@ class Index:
         _check(lib().mivf_search_rerank(self.h, qd.ptr, nq, w, kl, k, bd.ptr, bd.shape[0], ids.ptr,
                                         dis.ptr, None))
         return ids.numpy(), dis.numpy()
+
+
+class Server:
+    """Continuous-batching request loop over one index (exact search, per-request w and k)."""
+
+    def __init__(self, index, slots=4, max_batch=2048, mem_budget=0):
+        h = ctypes.c_void_p()
+        _check(lib().mivf_server_create(index.h, slots, max_batch, mem_budget, ctypes.byref(h)))
+        self.h, self.index = h, index
+        self._keep = {}
+
+    def submit(self, q, w, k):
+        """Queue a request; returns a ticket.  q: DeviceArray [nq][d]."""
+        nq = q.shape[0]
+        ids = DeviceArray((nq, k), np.int32)
+        dis = DeviceArray((nq, k), np.float32)
+        t = ctypes.c_int64(0)
+        _check(lib().mivf_server_submit(self.h, q.ptr, nq, w, k, ids.ptr, dis.ptr, ctypes.byref(t)))
+        self._keep[t.value] = (q, ids, dis)
+        return t.value
+
+    def progress(self):
+        c = ctypes.c_int(0)
+        _check(lib().mivf_server_progress(self.h, ctypes.byref(c)))
+        return c.value
+
+    def drain(self):
+        _check(lib().mivf_server_drain(self.h))
+
+    def result(self, ticket):
+        _check(lib().mivf_server_wait(self.h, ticket))
+        _, ids, dis = self._keep[ticket]
+        return ids.numpy(), dis.numpy()
+
+    def latency_ms(self, ticket):
+        v = ctypes.c_double(0)
+        _check(lib().mivf_server_latency(self.h, ticket, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h is not None:
+            lib().mivf_server_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve_bench(index, pool, rate, nreq, seed=1, bmax=1024, w_choices=(32,), k_choices=(100,), slots=4,
+                max_batch=2048, mem_budget=0):
+    """Open-loop Poisson load on a Server; returns a dict of QPS and latency percentiles."""
+    pd, _ = _dev(pool, np.float32)
+    wc = np.ascontiguousarray(w_choices, dtype=np.int32)
+    kc = np.ascontiguousarray(k_choices, dtype=np.int32)
+    st = (ctypes.c_double * 8)()
+    _check(lib().mivf_serve_bench(index.h, slots, max_batch, mem_budget, pd.ptr, pd.shape[0], float(rate), nreq,
+                                  seed, bmax, _ptr(wc), len(wc), _ptr(kc), len(kc), st))
+    keys = ("qps", "p50_ms", "p99_ms", "mean_ms", "queries", "makespan_ms", "batches", "offered_qps")
+    return {k: float(v) for k, v in zip(keys, st)}
+

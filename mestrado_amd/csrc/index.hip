@@ -9,7 +9,7 @@
 #include <vector>
 
 #include "../../include/mivf.h"
-#include "mivf_internal.h"
+#include "index_impl.h"
 
 namespace mivf {
 
@@ -17,99 +17,11 @@ static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 const char *last_error() { return g_err.c_str(); }
 
-// Grow-only device buffer.
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    int ensure(size_t want) {
-        if (want <= bytes) return kOk;
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, want);
-        if (e != hipSuccess) {
-            set_error(std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
-            return kNoMem;
-        }
-        bytes = want;
-        return kOk;
-    }
-    template <typename T>
-    T *as() const { return static_cast<T *>(p); }
-    void release() {
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    ~DevBuf() { release(); }
-};
-
-// Per-stage HIP event timing on the launch stream (events owned by the index).
-struct Timer {
-    bool on = false;
-    hipStream_t s = nullptr;
-    hipEvent_t *ev = nullptr;
-    int n = 0;
-    int start(bool enable, hipStream_t st, hipEvent_t *evs) {
-        on = enable && evs;
-        s = st;
-        n = 0;
-        ev = evs;
-        if (!on) return kOk;
-        for (int i = 0; i < 8; ++i)
-            if (!ev[i]) MIVF_CHECK_HIP(hipEventCreate(&ev[i]));
-        return mark();
-    }
-    int mark() {
-        if (!on) return kOk;
-        MIVF_CHECK_HIP(hipEventRecord(ev[n++], s));
-        return kOk;
-    }
-};
-
 }  // namespace mivf
 
 using namespace mivf;
 
-struct mivf_index {
-    int d = 0, nlist = 0, m = 0, ks = 0, ds = 0;
-    // quantizers (device)
-    DevBuf coarse, coarse_n32, pq, pq_n64, pq_n32;
-    float maxcn = 0.0f;      // max_c |c|^2 (margin of the coarse recheck)
-    // lists: host mirror (list-major) + device arrays
-    std::vector<std::vector<int32_t>> h_ids;
-    std::vector<std::vector<uint8_t>> h_codes;
-    bool dirty = true;
-    int64_t ntotal = 0;
-    std::vector<int64_t> h_off;
-    DevBuf off, ids, codes;
-    // scratch
-    DevBuf qn, dmat, probes, coadis, partial, wq, kq, tmp_ids, tmp_dis, keys, res, resn, assign;
-    DevBuf redo;             // coarse recheck: [0] count, then query ids
-    DevBuf pmax;             // [m] max_c |p_sc| (reduced-precision table bound)
-    DevBuf fpb;              // fp16 B operands of the reduced-precision tables
-    DevBuf rows;             // raw vectors in list order (mivf_index_attach_base)
-    bool rows_ok = false;
-    // batched reduced-precision form scratch
-    DevBuf f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_items, f_ctrl, f_cand, f_ccnt,
-        f_short, f_ns;
-    int64_t f_last_cands = 0, f_last_overflow = 0;
-    DevBuf errflag;          // kernel-reported precondition failures (int)
-    bool scan_checked = false;
-    // options
-    int ppg_opt = 0;
-    int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
-    int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
-    bool profile = false;
-    float timings[8] = {};
-    hipEvent_t ev[8] = {};
-    ~mivf_index() {
-        for (auto &e : ev)
-            if (e) hipEventDestroy(e);
-    }
-};
-
-namespace {
+namespace mivf {
 
 int upload_lists(mivf_index *ix) {
     if (!ix->dirty) return kOk;
@@ -138,6 +50,10 @@ int upload_lists(mivf_index *ix) {
     ix->rows_ok = false;
     return kOk;
 }
+
+}  // namespace mivf
+
+namespace {
 
 // ---------------------------------------------------------------------------
 // k-means: GPU assignment with the reference distance arithmetic (a = centroid,
@@ -249,15 +165,15 @@ int train_impl(const float *d_x, const float *h_x, int64_t n, int d, int nlist, 
 int encode_impl(mivf_index *ix, const float *d_x, int64_t n, int32_t *d_assign, uint8_t *d_codes,
                 hipStream_t st) {
     const int d = ix->d, m = ix->m, ds = ix->ds;
-    MIVF_TRY(ix->qn.ensure(sizeof(double) * std::max<int64_t>(n, 1)));
+    MIVF_TRY(ix->ws.qn.ensure(sizeof(double) * std::max<int64_t>(n, 1)));
     MIVF_TRY(ix->keys.ensure(sizeof(uint64_t) * std::max<int64_t>(n, 1)));
     MIVF_TRY(ix->res.ensure(sizeof(float) * size_t(std::max<int64_t>(n, 1)) * d));
     MIVF_TRY(ix->resn.ensure(sizeof(double) * std::max<int64_t>(n, 1)));
     uint64_t *keys = ix->keys.as<uint64_t>();
-    MIVF_CHECK_HIP(launch_norm_f64(d_x, n, d, d, ix->qn.as<double>(), st));
+    MIVF_CHECK_HIP(launch_norm_f64(d_x, n, d, d, ix->ws.qn.as<double>(), st));
     MIVF_CHECK_HIP(launch_fill_u64(keys, n, kNoKey, st));
     MIVF_CHECK_HIP(launch_dist_argmin(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), ix->nlist,
-                                      d_x, d, ix->qn.as<double>(), n, d, keys, st));
+                                      d_x, d, ix->ws.qn.as<double>(), n, d, keys, st));
     MIVF_CHECK_HIP(launch_keys_to_idx(keys, n, d_assign, st));
     MIVF_CHECK_HIP(launch_residual(d_x, n, d, ix->coarse.as<float>(), d_assign, ix->res.as<float>(), st));
     for (int s = 0; s < m; ++s) {
@@ -327,27 +243,27 @@ int check_index(const mivf_index *ix) {
 }
 
 
-// Coarse top-w of nq queries into probes/dis [nq][w] (needs ix->dmat sized
+// Coarse top-w of nq queries into probes/dis [nq][w] (needs ix->ws.dmat sized
 // nq * nlist): query norms, distances (timer mark), selection.
-int coarse_stage(mivf_index *ix, const float *d_q, int nq, int w, int32_t *probes, float *dis, hipStream_t st,
+int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_t *probes, float *dis, hipStream_t st,
                  Timer *tm) {
     const int d = ix->d, nlist = ix->nlist;
-    MIVF_TRY(ix->qn.ensure(sizeof(double) * nq));
-    MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ix->qn.as<double>(), st));
+    MIVF_TRY(ws.qn.ensure(sizeof(double) * nq));
+    MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ws.qn.as<double>(), st));
     if (coarse_mfma_ok(d, nlist, w) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0) {
         MIVF_CHECK_HIP(launch_coarse_approx(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), nlist, d_q,
-                                            ix->qn.as<double>(), nq, ix->dmat.as<float>(), nlist, st));
+                                            ws.qn.as<double>(), nq, ws.dmat.as<float>(), nlist, st));
         if (tm) MIVF_TRY(tm->mark());
-        MIVF_TRY(ix->redo.ensure(sizeof(int32_t) * (size_t(nq) + 1)));
-        int32_t *redo = ix->redo.as<int32_t>();
+        MIVF_TRY(ws.redo.ensure(sizeof(int32_t) * (size_t(nq) + 1)));
+        int32_t *redo = ws.redo.as<int32_t>();
         MIVF_CHECK_HIP(launch_coarse_select(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), ix->maxcn, nlist,
-                                            d_q, ix->qn.as<double>(), nq, w, ix->dmat.as<float>(), nlist, probes,
+                                            d_q, ws.qn.as<double>(), nq, w, ws.dmat.as<float>(), nlist, probes,
                                             dis, redo + 1, redo, st));
     } else {
         MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), nlist, d_q, d,
-                                         ix->qn.as<double>(), nq, d, ix->dmat.as<float>(), nlist, st));
+                                         ws.qn.as<double>(), nq, d, ws.dmat.as<float>(), nlist, st));
         if (tm) MIVF_TRY(tm->mark());
-        MIVF_CHECK_HIP(launch_row_topk(ix->dmat.as<float>(), nq, nlist, nlist, w, probes, dis, st));
+        MIVF_CHECK_HIP(launch_row_topk(ws.dmat.as<float>(), nq, nlist, nlist, w, probes, dis, st));
     }
     return kOk;
 }
@@ -368,19 +284,19 @@ int choose_groups(const mivf_index *ix, int nq, int wmax, int *ppg) {
 // Core search over a chunk of queries already on the device.
 //   coarse (q) -> probes -> scan (-> merge)
 // resid_mode: d_q holds residuals [nq*w][d] and d_probes_in the list ids.
-int search_core(mivf_index *ix, const float *d_q, int nq, int wmax, const int32_t *d_wq,
+int search_core(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, const int32_t *d_wq,
                 const int32_t *d_kq, int kuni, int kstride, int32_t *d_ids, float *d_dis,
                 hipStream_t st, const int32_t *d_probes_in, Timer &tm) {
     const int nlist = ix->nlist, d = ix->d;
     const int32_t *probes = d_probes_in;
     if (!probes) {
-        MIVF_TRY(ix->qn.ensure(sizeof(double) * nq));
-        MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nq) * nlist));
-        MIVF_TRY(ix->probes.ensure(sizeof(int32_t) * size_t(nq) * wmax));
-        MIVF_TRY(ix->coadis.ensure(sizeof(float) * size_t(nq) * wmax));
-        MIVF_TRY(coarse_stage(ix, d_q, nq, wmax, ix->probes.as<int32_t>(), ix->coadis.as<float>(), st, &tm));
+        MIVF_TRY(ws.qn.ensure(sizeof(double) * nq));
+        MIVF_TRY(ws.dmat.ensure(sizeof(float) * size_t(nq) * nlist));
+        MIVF_TRY(ws.probes.ensure(sizeof(int32_t) * size_t(nq) * wmax));
+        MIVF_TRY(ws.coadis.ensure(sizeof(float) * size_t(nq) * wmax));
+        MIVF_TRY(coarse_stage(ix, ws, d_q, nq, wmax, ws.probes.as<int32_t>(), ws.coadis.as<float>(), st, &tm));
         MIVF_TRY(tm.mark());
-        probes = ix->probes.as<int32_t>();
+        probes = ws.probes.as<int32_t>();
     } else {
         MIVF_TRY(tm.mark());
         MIVF_TRY(tm.mark());
@@ -411,8 +327,8 @@ int search_core(mivf_index *ix, const float *d_q, int nq, int wmax, const int32_
     a.resid_mode = d_probes_in ? 1 : 0;
     a.err = ix->errflag.as<int>();
     if (ngroups > 1) {
-        MIVF_TRY(ix->partial.ensure(sizeof(uint64_t) * size_t(nq) * ngroups * kstride));
-        a.partial = ix->partial.as<uint64_t>();
+        MIVF_TRY(ws.partial.ensure(sizeof(uint64_t) * size_t(nq) * ngroups * kstride));
+        a.partial = ws.partial.as<uint64_t>();
     } else {
         a.out_ids = d_ids;
         a.out_dis = d_dis;
@@ -427,7 +343,7 @@ int search_core(mivf_index *ix, const float *d_q, int nq, int wmax, const int32_
     }
     MIVF_TRY(tm.mark());
     if (ngroups > 1)
-        MIVF_CHECK_HIP(launch_merge(ix->partial.as<uint64_t>(), nq, ngroups, kstride, d_kq, kuni, d_ids,
+        MIVF_CHECK_HIP(launch_merge(ws.partial.as<uint64_t>(), nq, ngroups, kstride, d_kq, kuni, d_ids,
                                     d_dis, st));
     MIVF_TRY(tm.mark());
     return kOk;
@@ -444,7 +360,12 @@ int finish_timer(mivf_index *ix, Timer &tm) {
     return kOk;
 }
 
-int search_entry(mivf_index *ix, const float *d_q, int nq, int wmax, const int32_t *d_wq,
+
+}  // namespace
+
+namespace mivf {
+
+int search_entry(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, const int32_t *d_wq,
                  const int32_t *d_kq, int kuni, int kstride, int32_t *d_ids, float *d_dis,
                  hipStream_t st) {
     MIVF_TRY(upload_lists(ix));
@@ -455,7 +376,7 @@ int search_entry(mivf_index *ix, const float *d_q, int nq, int wmax, const int32
     for (int64_t q0 = 0; q0 < nq; q0 += maxq) {
         const int nc = int(std::min<int64_t>(maxq, nq - q0));
         MIVF_TRY(tm.start(ix->profile, st, ix->ev));
-        MIVF_TRY(search_core(ix, d_q + q0 * ix->d, nc, wmax, d_wq ? d_wq + q0 : nullptr,
+        MIVF_TRY(search_core(ix, ws, d_q + q0 * ix->d, nc, wmax, d_wq ? d_wq + q0 : nullptr,
                              d_kq ? d_kq + q0 : nullptr, kuni, kstride, d_ids + q0 * kstride,
                              d_dis + q0 * kstride, st, nullptr, tm));
         MIVF_TRY(finish_timer(ix, tm));
@@ -463,7 +384,7 @@ int search_entry(mivf_index *ix, const float *d_q, int nq, int wmax, const int32
     return kOk;
 }
 
-}  // namespace
+}  // namespace mivf
 
 // ===========================================================================
 // C ABI
@@ -702,7 +623,7 @@ int mivf_search(mivf_index *ix, const float *d_q, int nq, int w, int k, int32_t 
     MIVF_REQUIRE(k >= 1 && k <= 1024, kInvalid, "search: k must be in 1..1024");
     if (nq == 0) return kOk;
     MIVF_REQUIRE(d_q && d_ids && d_dis, kInvalid, "search: null pointer");
-    return search_entry(ix, d_q, nq, w, nullptr, nullptr, k, k, d_ids, d_dis, static_cast<hipStream_t>(stream));
+    return search_entry(ix, ix->ws, d_q, nq, w, nullptr, nullptr, k, k, d_ids, d_dis, static_cast<hipStream_t>(stream));
 }
 
 int mivf_search_var(mivf_index *ix, const float *d_q, int nq, const int32_t *w, const int32_t *k,
@@ -717,11 +638,11 @@ int mivf_search_var(mivf_index *ix, const float *d_q, int nq, const int32_t *w, 
         MIVF_REQUIRE(k[i] >= 1 && k[i] <= kmax, kInvalid, "search_var: k[i] out of 1..kmax");
         wmax = std::max(wmax, int(w[i]));
     }
-    MIVF_TRY(ix->wq.ensure(sizeof(int32_t) * nq));
-    MIVF_TRY(ix->kq.ensure(sizeof(int32_t) * nq));
-    MIVF_CHECK_HIP(hipMemcpy(ix->wq.p, w, sizeof(int32_t) * nq, hipMemcpyHostToDevice));
-    MIVF_CHECK_HIP(hipMemcpy(ix->kq.p, k, sizeof(int32_t) * nq, hipMemcpyHostToDevice));
-    return search_entry(ix, d_q, nq, wmax, ix->wq.as<int32_t>(), ix->kq.as<int32_t>(), kmax, kmax, d_ids,
+    MIVF_TRY(ix->ws.wq.ensure(sizeof(int32_t) * nq));
+    MIVF_TRY(ix->ws.kq.ensure(sizeof(int32_t) * nq));
+    MIVF_CHECK_HIP(hipMemcpy(ix->ws.wq.p, w, sizeof(int32_t) * nq, hipMemcpyHostToDevice));
+    MIVF_CHECK_HIP(hipMemcpy(ix->ws.kq.p, k, sizeof(int32_t) * nq, hipMemcpyHostToDevice));
+    return search_entry(ix, ix->ws, d_q, nq, wmax, ix->ws.wq.as<int32_t>(), ix->ws.kq.as<int32_t>(), kmax, kmax, d_ids,
                         d_dis, static_cast<hipStream_t>(stream));
 }
 
@@ -734,9 +655,9 @@ int mivf_coarse(mivf_index *ix, const float *d_q, int nq, int w, int32_t *d_idx,
     const int64_t maxq = std::max<int64_t>(1, (int64_t(512) << 20) / (4 * int64_t(ix->nlist)));
     for (int64_t q0 = 0; q0 < nq; q0 += maxq) {
         const int nc = int(std::min<int64_t>(maxq, nq - q0));
-        MIVF_TRY(ix->qn.ensure(sizeof(double) * nc));
-        MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nc) * ix->nlist));
-        MIVF_TRY(coarse_stage(ix, d_q + q0 * ix->d, nc, w, d_idx + q0 * w, d_dis + q0 * w, st, nullptr));
+        MIVF_TRY(ix->ws.qn.ensure(sizeof(double) * nc));
+        MIVF_TRY(ix->ws.dmat.ensure(sizeof(float) * size_t(nc) * ix->nlist));
+        MIVF_TRY(coarse_stage(ix, ix->ws, d_q + q0 * ix->d, nc, w, d_idx + q0 * w, d_dis + q0 * w, st, nullptr));
     }
     return kOk;
 }
@@ -750,7 +671,7 @@ int mivf_search_residual(mivf_index *ix, const float *d_residual, const int32_t 
     MIVF_TRY(upload_lists(ix));
     Timer tm;
     MIVF_TRY(tm.start(false, static_cast<hipStream_t>(stream), nullptr));
-    return search_core(ix, d_residual, nq, w, nullptr, nullptr, k, k, d_ids, d_dis,
+    return search_core(ix, ix->ws, d_residual, nq, w, nullptr, nullptr, k, k, d_ids, d_dis,
                        static_cast<hipStream_t>(stream), d_list_of, tm);
 }
 
@@ -762,17 +683,17 @@ int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
     if (nq <= 0) return kOk;
     MIVF_REQUIRE(d_q && d_base && d_ids && d_dis && nbase >= 0, kInvalid, "rerank: null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    MIVF_TRY(ix->tmp_ids.ensure(sizeof(int32_t) * size_t(nq) * kl));
-    MIVF_TRY(ix->tmp_dis.ensure(sizeof(float) * size_t(nq) * kl));
-    MIVF_TRY(search_entry(ix, d_q, nq, w, nullptr, nullptr, kl, kl, ix->tmp_ids.as<int32_t>(),
-                          ix->tmp_dis.as<float>(), st));
+    MIVF_TRY(ix->ws.tmp_ids.ensure(sizeof(int32_t) * size_t(nq) * kl));
+    MIVF_TRY(ix->ws.tmp_dis.ensure(sizeof(float) * size_t(nq) * kl));
+    MIVF_TRY(search_entry(ix, ix->ws, d_q, nq, w, nullptr, nullptr, kl, kl, ix->ws.tmp_ids.as<int32_t>(),
+                          ix->ws.tmp_dis.as<float>(), st));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ix->profile) {
         MIVF_CHECK_HIP(hipEventCreate(&e0));
         MIVF_CHECK_HIP(hipEventCreate(&e1));
         MIVF_CHECK_HIP(hipEventRecord(e0, st));
     }
-    MIVF_CHECK_HIP(launch_rerank(d_base, nbase, d_q, nq, ix->d, ix->tmp_ids.as<int32_t>(), kl, k, d_ids, d_dis, st));
+    MIVF_CHECK_HIP(launch_rerank(d_base, nbase, d_q, nq, ix->d, ix->ws.tmp_ids.as<int32_t>(), kl, k, d_ids, d_dis, st));
     if (ix->profile) {
         MIVF_CHECK_HIP(hipEventRecord(e1, st));
         MIVF_CHECK_HIP(hipEventSynchronize(e1));
@@ -807,17 +728,17 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     std::fill(ix->timings, ix->timings + 8, 0.0f);
     MIVF_TRY(tm.start(ix->profile, st, ix->ev));
     // 1. coarse top-w
-    MIVF_TRY(ix->qn.ensure(sizeof(double) * nq));
-    MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nq) * nlist));
-    MIVF_TRY(ix->probes.ensure(sizeof(int32_t) * size_t(nq) * w));
-    MIVF_TRY(ix->coadis.ensure(sizeof(float) * size_t(nq) * w));
-    MIVF_TRY(coarse_stage(ix, d_q, nq, w, ix->probes.as<int32_t>(), ix->coadis.as<float>(), st, nullptr));
+    MIVF_TRY(ix->ws.qn.ensure(sizeof(double) * nq));
+    MIVF_TRY(ix->ws.dmat.ensure(sizeof(float) * size_t(nq) * nlist));
+    MIVF_TRY(ix->ws.probes.ensure(sizeof(int32_t) * size_t(nq) * w));
+    MIVF_TRY(ix->ws.coadis.ensure(sizeof(float) * size_t(nq) * w));
+    MIVF_TRY(coarse_stage(ix, ix->ws, d_q, nq, w, ix->ws.probes.as<int32_t>(), ix->ws.coadis.as<float>(), st, nullptr));
     MIVF_TRY(tm.mark());
     // 2. seed threshold from each query's ns[q] nearest lists (at least
     // fast_seed, then as many as it takes to hold kl vectors)
     MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
     MIVF_TRY(ix->f_tau.ensure(sizeof(float) * nq));
-    MIVF_CHECK_HIP(launch_seed_count(ix->probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), nlist, kl,
+    MIVF_CHECK_HIP(launch_seed_count(ix->ws.probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), nlist, kl,
                                      std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), st));
     {
         SeedArgs a{};
@@ -828,7 +749,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         a.pmax = ix->pmax.as<float>();
         a.list_off = ix->off.as<int64_t>();
         a.codes = ix->codes.as<uint8_t>();
-        a.probes = ix->probes.as<int32_t>();
+        a.probes = ix->ws.probes.as<int32_t>();
         a.nseed = ix->f_ns.as<int32_t>();
         a.nq = nq;
         a.w = w;
@@ -849,7 +770,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(ix->f_items.ensure(sizeof(FastItem) * std::max<int64_t>(np, 1)));
     MIVF_TRY(ix->f_ctrl.ensure(sizeof(int) * 4));
     int *ctrl = ix->f_ctrl.as<int>();  // [0] total items, [1] next item
-    MIVF_CHECK_HIP(launch_pairs(ix->probes.as<int32_t>(), nq, w, nullptr, ix->off.as<int64_t>(), ix->f_tau.as<float>(), nlist,
+    MIVF_CHECK_HIP(launch_pairs(ix->ws.probes.as<int32_t>(), nq, w, nullptr, ix->off.as<int64_t>(), ix->f_tau.as<float>(), nlist,
                                 ix->f_cnt.as<int>(), ix->f_poff.as<int>(), ix->f_ioff.as<int>(), ctrl,
                                 ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(), ix->f_items.as<FastItem>(), st));
     MIVF_TRY(ix->f_cand.ensure(sizeof(uint64_t) * size_t(nq) * capq));
@@ -951,12 +872,12 @@ int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
                  "fast_seed_tau: null or unaligned pointer");
     if (nq <= 0) return kOk;
     MIVF_TRY(upload_lists(ix));
-    MIVF_TRY(ix->dmat.ensure(sizeof(float) * size_t(nq) * ix->nlist));
-    MIVF_TRY(ix->probes.ensure(sizeof(int32_t) * size_t(nq) * w));
-    MIVF_TRY(ix->coadis.ensure(sizeof(float) * size_t(nq) * w));
+    MIVF_TRY(ix->ws.dmat.ensure(sizeof(float) * size_t(nq) * ix->nlist));
+    MIVF_TRY(ix->ws.probes.ensure(sizeof(int32_t) * size_t(nq) * w));
+    MIVF_TRY(ix->ws.coadis.ensure(sizeof(float) * size_t(nq) * w));
     MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
-    MIVF_TRY(coarse_stage(ix, d_q, nq, w, ix->probes.as<int32_t>(), ix->coadis.as<float>(), nullptr, nullptr));
-    MIVF_CHECK_HIP(launch_seed_count(ix->probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), ix->nlist, kl,
+    MIVF_TRY(coarse_stage(ix, ix->ws, d_q, nq, w, ix->ws.probes.as<int32_t>(), ix->ws.coadis.as<float>(), nullptr, nullptr));
+    MIVF_CHECK_HIP(launch_seed_count(ix->ws.probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), ix->nlist, kl,
                                      std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), nullptr));
     SeedArgs a{};
     a.queries = d_q;
@@ -966,7 +887,7 @@ int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
     a.pmax = ix->pmax.as<float>();
     a.list_off = ix->off.as<int64_t>();
     a.codes = ix->codes.as<uint8_t>();
-    a.probes = ix->probes.as<int32_t>();
+    a.probes = ix->ws.probes.as<int32_t>();
     a.nseed = ix->f_ns.as<int32_t>();
     a.nq = nq;
     a.w = w;

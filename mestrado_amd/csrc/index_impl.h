@@ -1,0 +1,122 @@
+// Host-side internals shared by the index layer (index.hip) and the serving
+// loop (serve.hip): device buffers, stage timers, search workspaces and the
+// index handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "mivf_internal.h"
+
+namespace mivf {
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want) {
+        if (want <= bytes) return kOk;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            set_error(std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
+            return kNoMem;
+        }
+        bytes = want;
+        return kOk;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+// Per-stage HIP event timing on the launch stream (events owned by the index).
+struct Timer {
+    bool on = false;
+    hipStream_t s = nullptr;
+    hipEvent_t *ev = nullptr;
+    int n = 0;
+    int start(bool enable, hipStream_t st, hipEvent_t *evs) {
+        on = enable && evs;
+        s = st;
+        n = 0;
+        ev = evs;
+        if (!on) return kOk;
+        for (int i = 0; i < 8; ++i)
+            if (!ev[i]) MIVF_CHECK_HIP(hipEventCreate(&ev[i]));
+        return mark();
+    }
+    int mark() {
+        if (!on) return kOk;
+        MIVF_CHECK_HIP(hipEventRecord(ev[n++], s));
+        return kOk;
+    }
+};
+
+// Scratch of one search in flight (grow-only).  The index owns one; each
+// serving slot owns another, so searches on different streams can overlap.
+struct Ws {
+    DevBuf qn, dmat, probes, coadis, partial, wq, kq, redo, tmp_ids, tmp_dis;
+};
+
+}  // namespace mivf
+
+using namespace mivf;
+
+struct mivf_index {
+    int d = 0, nlist = 0, m = 0, ks = 0, ds = 0;
+    // quantizers (device)
+    DevBuf coarse, coarse_n32, pq, pq_n64, pq_n32;
+    float maxcn = 0.0f;      // max_c |c|^2 (margin of the coarse recheck)
+    // lists: host mirror (list-major) + device arrays
+    std::vector<std::vector<int32_t>> h_ids;
+    std::vector<std::vector<uint8_t>> h_codes;
+    bool dirty = true;
+    int64_t ntotal = 0;
+    std::vector<int64_t> h_off;
+    DevBuf off, ids, codes;
+    // scratch
+    Ws ws;                   // scratch of the calls made on this handle
+    DevBuf keys, res, resn, assign;
+    DevBuf pmax;             // [m] max_c |p_sc| (reduced-precision table bound)
+    DevBuf fpb;              // fp16 B operands of the reduced-precision tables
+    DevBuf rows;             // raw vectors in list order (mivf_index_attach_base)
+    bool rows_ok = false;
+    // batched reduced-precision form scratch
+    DevBuf f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_items, f_ctrl, f_cand, f_ccnt,
+        f_short, f_ns;
+    int64_t f_last_cands = 0, f_last_overflow = 0;
+    DevBuf errflag;          // kernel-reported precondition failures (int)
+    bool scan_checked = false;
+    // options
+    int ppg_opt = 0;
+    int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
+    int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
+    bool profile = false;
+    float timings[8] = {};
+    hipEvent_t ev[8] = {};
+    ~mivf_index() {
+        for (auto &e : ev)
+            if (e) hipEventDestroy(e);
+    }
+};
+
+
+namespace mivf {
+
+// Search over nq device-resident queries with the handle's kernels and the
+// given workspace, on stream st.  d_wq/d_kq: per-query w and k (device) or
+// null for the uniform wmax / kuni.  Outputs [nq][kstride].
+int search_entry(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, const int32_t *d_wq,
+                 const int32_t *d_kq, int kuni, int kstride, int32_t *d_ids, float *d_dis, hipStream_t st);
+int upload_lists(mivf_index *ix);
+
+}  // namespace mivf
