@@ -38,6 +38,7 @@ def main():
     ap.add_argument("--fast-seed", type=int, default=1, help="lists scanned exactly to seed its filter")
     ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--serve-fracs", default="0.25,0.5,0.75", help="offered load as fractions of the batch QPS")
     ap.add_argument("--serve-reqs", type=int, default=12000)
     ap.add_argument("--profile-only", action="store_true",
@@ -126,6 +127,41 @@ def main():
                 "stages_ms": {n: round(float(v), 4) for n, v in
                               zip(("coarse", "seed", "pairs", "scan8", "select", "rerank"), fstage)}}
         log(f"fast form: {fast}")
+    uneven = None
+    if not args.no_uneven and not args.profile_only:
+        # batched uneven exact search: per-query w ~ U{1..64}, k ~ {1, 10, 100}
+        rng = np.random.default_rng(seed)
+        wq = rng.integers(1, 65, nq).astype(np.int32)
+        kq = rng.choice([1, 10, 100], nq).astype(np.int32)
+        v_ids = M.DeviceArray((nq, 100), np.int32)
+        v_dis = M.DeviceArray((nq, 100), np.float32)
+        b_ids = M.DeviceArray((nq, 100), np.int32)
+        b_dis = M.DeviceArray((nq, 100), np.float32)
+        for _ in range(args.warmup):
+            ix.search_var(query, wq, kq, 100, out=(v_ids, v_dis))
+        M.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ix.search_var(query, wq, kq, 100, out=(v_ids, v_dis))
+        M.synchronize()
+        t_var = (time.perf_counter() - t0) / args.steps
+        plan = M.Plan(ix, nq, 64, 100, autotune=True)
+        for _ in range(max(1, args.warmup)):
+            plan.search(query, wq, kq, out=(b_ids, b_dis))
+        M.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.search(query, wq, kq, out=(b_ids, b_dis))
+        M.synchronize()
+        t_b = (time.perf_counter() - t0) / args.steps
+        same = bool(np.array_equal(v_ids.numpy(), b_ids.numpy()) and
+                    np.array_equal(v_dis.numpy().view(np.uint32), b_dis.numpy().view(np.uint32)))
+        uneven = {"w": "U{1..64}", "k": "{1,10,100}", "plain_var_qps": round(nq / t_var, 1),
+                  "batched_graph_qps": round(nq / t_b, 1), "speedup": round(t_var / t_b, 3),
+                  "bit_identical_to_plain": same, **{f"tuned_{k2}": v for k2, v in plan.info().items()}}
+        plan.close()
+        log(f"batched uneven: {uneven}")
+
     serving = None
     if not args.no_serve and not args.profile_only:
         # open-loop Poisson requests (nq log-uniform 1..1024, mean ~148) against the
@@ -211,6 +247,7 @@ def main():
                      "kernel": "scan_kernel", "algorithmic_bytes_per_launch": scan_bytes},
         "cpu_baseline": cpu,
         "fast_form": fast,
+        "batched_uneven": uneven,
         "serving": serving,
     }
     print(json.dumps(res), flush=True)

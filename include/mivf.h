@@ -178,6 +178,38 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value);
  * [5] rerank (mivf_search_fast).                                            */
 int mivf_get_timings(mivf_index *ix, float *ms, int n);
 
+/* ---- batched exact form ----------------------------------------------------
+ * Per-query w and k (host arrays w[nq], k[nq], k[i] <= kmax), results
+ * identical to mivf_search_var (ids and FP32 distances, rows kmax wide),
+ * computed list-major: the (list, query) pairs of the batch are grouped by
+ * list into work items of G queries; one persistent kernel builds the G exact
+ * FP32 tables of an item in LDS and scans the list once for all of them,
+ * keeping the vectors within a per-query bound of the k-th distance (from the
+ * query's nearest lists); a select keeps the k best.  Replaces, for a batch
+ * with uneven w and k, the per-query loop of parallel_search
+ * (parallel/ivf_pq/ivf_search.cpp:97-137).  Options "batched_group" (G),
+ * "batched_threads", "batched_capacity".  d_q must be 16-byte aligned.      */
+int mivf_search_batched(mivf_index *ix, const float *d_q, int nq, const int32_t *w, const int32_t *k,
+                        int kmax, int32_t *d_ids, float *d_dis, void *stream);
+/* A reusable plan for batches of exactly nq queries with w <= wmax, k <= kmax:
+ * the first mivf_plan_search runs the autotuner (when asked: every compiled
+ * work-item shape timed on that batch), then captures the whole launch
+ * sequence in a hipGraph; later calls replay the graph.  Results are those of
+ * mivf_search_batched.                                                       */
+typedef struct mivf_plan mivf_plan;
+int mivf_plan_create(mivf_index *ix, int nq, int wmax, int kmax, int autotune, mivf_plan **out);
+int mivf_plan_search(mivf_plan *p, const float *d_q, const int32_t *w, const int32_t *k, int32_t *d_ids,
+                     float *d_dis, void *stream);
+/* Test hooks of the batched exact kernel: its FP32 tables of queries
+ * d_q[0..n-1] (n <= 8) against list `list`, d_out [m][256][8]; and the ADC
+ * distances its scan loop computes for every vector of the list,
+ * d_out [len][8] (m = 16).                                                  */
+int mivf_exact_tables(mivf_index *ix, const float *d_q, int n, int list, float *d_out);
+int mivf_exact_adc(mivf_index *ix, const float *d_q, int n, int list, float *d_out);
+/* Chosen work-item shape and the autotuner's best time (ms, 0 if not tuned). */
+int mivf_plan_info(const mivf_plan *p, int *g, int *threads, float *tuned_ms);
+int mivf_plan_free(mivf_plan *p);
+
 /* ---- serving loop (continuous batching) -----------------------------------
  * Replaces the search stage of the reference's MPI query pipeline (query
  * sender -> search ranks -> aggregator, parallel/ivf_test.cpp) on one GPU.

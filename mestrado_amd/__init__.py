@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-__all__ = ["lib", "MivfError", "DeviceArray", "Index", "Server", "serve_bench", "train", "kmeans",
+__all__ = ["lib", "MivfError", "DeviceArray", "Index", "Plan", "Server", "serve_bench", "train", "kmeans",
            "synth_centres", "zipf_thresholds", "synth_rows", "bruteforce_nn",
            "device_count", "synchronize", "LIB_PATH"]
 
@@ -60,6 +60,13 @@ _SIGS = {
     "mivf_index_attach_base": [_vp, _vp, _i64],
     "mivf_fast_tables": [_vp, _vp, _i, _i, _vp, _vp],
     "mivf_fast_seed_tau": [_vp, _vp, _i, _i, _i, _vp],
+    "mivf_search_batched": [_vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp],
+    "mivf_plan_create": [_vp, _i, _i, _i, _i, ctypes.POINTER(_vp)],
+    "mivf_plan_search": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "mivf_plan_info": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _f32p],
+    "mivf_plan_free": [_vp],
+    "mivf_exact_tables": [_vp, _vp, _i, _i, _vp],
+    "mivf_exact_adc": [_vp, _vp, _i, _i, _vp],
     "mivf_server_create": [_vp, _i, _i, _i64, ctypes.POINTER(_vp)],
     "mivf_server_free": [_vp],
     "mivf_server_submit": [_vp, _vp, _i, _i, _i, _vp, _vp, _i64p],
@@ -318,15 +325,16 @@ class Index:
             return ids, dis
         return ids.numpy(), dis.numpy()
 
-    def search_var(self, q, w, k, kmax=None):
+    def search_var(self, q, w, k, kmax=None, out=None):
         qd, _ = _dev(q, np.float32)
         nq = qd.shape[0]
         w = np.ascontiguousarray(w, dtype=np.int32)
         k = np.ascontiguousarray(k, dtype=np.int32)
         kmax = int(kmax or k.max())
-        ids = DeviceArray((nq, kmax), np.int32)
-        dis = DeviceArray((nq, kmax), np.float32)
+        ids, dis = out if out is not None else (DeviceArray((nq, kmax), np.int32), DeviceArray((nq, kmax), np.float32))
         _check(lib().mivf_search_var(self.h, qd.ptr, nq, _ptr(w), _ptr(k), ids.ptr, dis.ptr, kmax, None))
+        if out is not None:
+            return ids, dis
         return ids.numpy(), dis.numpy()
 
     def coarse(self, q, w):
@@ -350,6 +358,33 @@ class Index:
         """Keep a list-ordered copy of the raw vectors for search_fast(base=None)."""
         bd, _ = _dev(base, np.float32)
         _check(lib().mivf_index_attach_base(self.h, bd.ptr, bd.shape[0]))
+
+    def search_batched(self, q, w, k, kmax=None, out=None):
+        """Batched exact search with per-query w and k (same results as search_var)."""
+        qd, _ = _dev(q, np.float32)
+        nq = qd.shape[0]
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        k = np.ascontiguousarray(k, dtype=np.int32)
+        kmax = int(kmax or k.max())
+        ids, dis = out if out is not None else (DeviceArray((nq, kmax), np.int32), DeviceArray((nq, kmax), np.float32))
+        _check(lib().mivf_search_batched(self.h, qd.ptr, nq, _ptr(w), _ptr(k), kmax, ids.ptr, dis.ptr, None))
+        if out is not None:
+            return ids, dis
+        return ids.numpy(), dis.numpy()
+
+    def exact_tables(self, q, list_no):
+        """FP32 tables [m][256][8] of the batched exact kernel (queries q[:n], n <= 8)."""
+        qd, _ = _dev(q, np.float32)
+        out = DeviceArray((self.m, 256, 8), np.float32)
+        _check(lib().mivf_exact_tables(self.h, qd.ptr, qd.shape[0], int(list_no), out.ptr))
+        return out.numpy()
+
+    def exact_adc(self, q, list_no, list_len):
+        """ADC distances [len][8] of list `list_no` from the batched exact scan loop."""
+        qd, _ = _dev(q, np.float32)
+        out = DeviceArray((max(list_len, 1), 8), np.float32)
+        _check(lib().mivf_exact_adc(self.h, qd.ptr, qd.shape[0], int(list_no), out.ptr))
+        return out.numpy()[:list_len]
 
     def search_fast(self, q, w, kl, k, base=None, out=None):
         """Batched reduced-precision search with exact FP32 rerank (against
@@ -458,4 +493,40 @@ def serve_bench(index, pool, rate, nreq, seed=1, bmax=1024, w_choices=(32,), k_c
                                   seed, bmax, _ptr(wc), len(wc), _ptr(kc), len(kc), st))
     keys = ("qps", "p50_ms", "p99_ms", "mean_ms", "queries", "makespan_ms", "batches", "offered_qps")
     return {k: float(v) for k, v in zip(keys, st)}
+
+
+class Plan:
+    """Graph-captured, optionally autotuned batched exact search for nq queries."""
+
+    def __init__(self, index, nq, wmax, kmax, autotune=True):
+        h = ctypes.c_void_p()
+        _check(lib().mivf_plan_create(index.h, nq, wmax, kmax, 1 if autotune else 0, ctypes.byref(h)))
+        self.h, self.index, self.nq, self.kmax = h, index, nq, kmax
+
+    def search(self, q, w, k, out=None):
+        qd, _ = _dev(q, np.float32)
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        k = np.ascontiguousarray(k, dtype=np.int32)
+        ids, dis = out if out is not None else (DeviceArray((self.nq, self.kmax), np.int32),
+                                                DeviceArray((self.nq, self.kmax), np.float32))
+        _check(lib().mivf_plan_search(self.h, qd.ptr, _ptr(w), _ptr(k), ids.ptr, dis.ptr, None))
+        if out is not None:
+            return ids, dis
+        return ids.numpy(), dis.numpy()
+
+    def info(self):
+        g, t, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_float(0)
+        _check(lib().mivf_plan_info(self.h, ctypes.byref(g), ctypes.byref(t), ctypes.byref(ms)))
+        return {"group": g.value, "threads": t.value, "tuned_ms": ms.value}
+
+    def close(self):
+        if self.h is not None:
+            lib().mivf_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 

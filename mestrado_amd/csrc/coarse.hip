@@ -376,7 +376,7 @@ hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxc
                                 float *pdis, int32_t *redo, int *redo_cnt, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     if (nlist >= 64 && nlist <= 64 * NPL && w <= 64 && redo && redo_cnt) {
-        hipError_t e = hipMemsetAsync(redo_cnt, 0, sizeof(int), s);
+        hipError_t e = launch_fill_i32(redo_cnt, 1, 0, s);  // a kernel, not a memset: graph-capturable
         if (e != hipSuccess) return e;
         coarse_select_wave_kernel<<<unsigned((nq + NTC / 64 - 1) / (NTC / 64)), NTC, 0, s>>>(
             coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda, w, probes, pdis, redo, redo_cnt);

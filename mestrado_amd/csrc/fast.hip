@@ -468,10 +468,11 @@ __global__ __launch_bounds__(FNT) void fast_tables_kernel(FastArgs a, int l, int
 
 // ---- seed lists per query
 __global__ void seed_count_kernel(const int32_t *__restrict__ probes, int nq, int w,
-                                  const int64_t *__restrict__ list_off, int nlist, int kl, int min_seed,
-                                  int32_t *__restrict__ ns) {
+                                  const int64_t *__restrict__ list_off, int nlist, int kl0,
+                                  const int32_t *__restrict__ klq, int min_seed, int32_t *__restrict__ ns) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
+    const int kl = klq ? klq[q] : kl0;
     int64_t have = 0;
     int r = 0;
     for (; r < w; ++r) {
@@ -480,6 +481,11 @@ __global__ void seed_count_kernel(const int32_t *__restrict__ probes, int nq, in
         if (l >= 0 && l < nlist) have += list_off[l + 1] - list_off[l];
     }
     ns[q] = r;
+}
+
+__global__ void fill_i32_kernel(int *__restrict__ p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ---- list-major grouping of (query, probe) pairs with rank >= rank0[q].
@@ -515,7 +521,7 @@ __global__ __launch_bounds__(256) void pair_count_kernel(const int32_t *__restri
 }
 
 // exclusive scans of pair counts and of work items per list (one workgroup)
-__global__ void pair_scan_kernel(const int *__restrict__ cnt, int nlist, int *__restrict__ pair_off,
+__global__ void pair_scan_kernel(const int *__restrict__ cnt, int nlist, int G, int *__restrict__ pair_off,
                                  int *__restrict__ item_off, int *__restrict__ total_items,
                                  int *__restrict__ fill) {
     __shared__ int part_p[1024], part_i[1024];
@@ -524,7 +530,7 @@ __global__ void pair_scan_kernel(const int *__restrict__ cnt, int nlist, int *__
     int sp = 0, si = 0;
     for (int j = tid * per; j < min(nlist, (tid + 1) * per); ++j) {
         sp += cnt[j];
-        si += (cnt[j] + FG - 1) / FG;
+        si += (cnt[j] + G - 1) / G;
     }
     part_p[tid] = sp;
     part_i[tid] = si;
@@ -549,7 +555,7 @@ __global__ void pair_scan_kernel(const int *__restrict__ cnt, int nlist, int *__
         item_off[j] = ai;
         fill[j] = 0;
         ap += cnt[j];
-        ai += (cnt[j] + FG - 1) / FG;
+        ai += (cnt[j] + G - 1) / G;
     }
 }
 
@@ -584,11 +590,12 @@ __global__ __launch_bounds__(256) void pair_scatter_kernel(const int32_t *__rest
 // (item_off), so a binary search over item_off finds the item's list.
 __global__ void item_expand_kernel(const int *__restrict__ pair_off, const int *__restrict__ item_off,
                                    const int64_t *__restrict__ list_off, const int32_t *__restrict__ pair_q,
-                                   const float *__restrict__ tau, int nlist, const int *__restrict__ total_items,
-                                   int64_t max_slots, FastItem *__restrict__ items) {
+                                   const float *__restrict__ tau, int nlist, int G,
+                                   const int *__restrict__ total_items, int64_t max_slots,
+                                   FastItem *__restrict__ items) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= max_slots) return;
-    const int it = int(e / FG), slot = int(e % FG);
+    const int it = int(e / FG), slot = int(e % FG);  // FG threads per record, slots >= G empty
     if (it >= *total_items) return;
     int lo = 0, hi = nlist;  // last list with item_off[l] <= it (lists without items are skipped)
     while (hi - lo > 1) {
@@ -598,8 +605,8 @@ __global__ void item_expand_kernel(const int *__restrict__ pair_off, const int *
     }
     const int l = lo;
     const int j = it - item_off[l];
-    const int pb = pair_off[l] + j * FG;
-    const int gc = min(FG, pair_off[l + 1] - pb);
+    const int pb = pair_off[l] + j * G;
+    const int gc = min(G, pair_off[l + 1] - pb);
     FastItem &x = items[it];
     const int q = slot < gc ? pair_q[pb + slot] : -1;
     x.q[slot] = q;
@@ -721,8 +728,8 @@ __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__
                                                        const int64_t *__restrict__ list_off,
                                                        const uint8_t *__restrict__ codes,
                                                        const int32_t *__restrict__ probes, int w,
-                                                       const int32_t *__restrict__ nseed, int nlist, int ks, int kl,
-                                                       float *__restrict__ tau) {
+                                                       const int32_t *__restrict__ nseed, int nlist, int ks, int kl0,
+                                                       const int32_t *__restrict__ klq, float *__restrict__ tau) {
     __shared__ float tab[M * 256];
     __shared__ float r[D];
     __shared__ float As[M];
@@ -732,6 +739,7 @@ __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__
     const int64_t q = blockIdx.x;
     const int tid = threadIdx.x;
     const int ns = nseed[q];
+    const int kl = klq ? klq[q] : kl0;
     int64_t tot = 0;
     for (int rr = 0; rr < ns; ++rr) {
         const int l = probes[q * w + rr];
@@ -925,23 +933,29 @@ hipError_t launch_fast_prep(const float *pq, const float *pmax, int m, int ks, v
 }
 
 hipError_t launch_seed_count(const int32_t *probes, int nq, int w, const int64_t *list_off, int nlist, int kl,
-                             int min_seed, int32_t *ns, hipStream_t s) {
-    seed_count_kernel<<<nblk(nq, 256), 256, 0, s>>>(probes, nq, w, list_off, nlist, kl, min_seed, ns);
+                             const int32_t *klq, int min_seed, int32_t *ns, hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    seed_count_kernel<<<nblk(nq, 256), 256, 0, s>>>(probes, nq, w, list_off, nlist, kl, klq, min_seed, ns);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    fill_i32_kernel<<<nblk(n, 256), 256, 0, s>>>(p, n, v);
     return hipGetLastError();
 }
 
 hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *rank0, const int64_t *list_off,
-                        const float *tau, int nlist, int *cnt, int *pair_off, int *item_off, int *total_items,
+                        const float *tau, int nlist, int G, int *cnt, int *pair_off, int *item_off, int *total_items,
                         int *fill, int32_t *pair_q, FastItem *items, hipStream_t s) {
-    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(int) * nlist, s);
-    if (e != hipSuccess) return e;
+    if (nlist > LBINS || G < 1 || G > FG) return hipErrorInvalidValue;
     const int64_t np = int64_t(nq) * w;
-    if (nlist > LBINS) return hipErrorInvalidValue;
+    fill_i32_kernel<<<nblk(nlist, 256), 256, 0, s>>>(cnt, nlist, 0);
     pair_count_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rank0, nlist, cnt);
-    pair_scan_kernel<<<1, 1024, 0, s>>>(cnt, nlist, pair_off, item_off, total_items, fill);
+    pair_scan_kernel<<<1, 1024, 0, s>>>(cnt, nlist, G, pair_off, item_off, total_items, fill);
     pair_scatter_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rank0, nlist, pair_off, fill, pair_q);
-    const int64_t max_slots = (np / FG + nlist) * FG;  // >= FG * total items
-    item_expand_kernel<<<nblk(max_slots, 256), 256, 0, s>>>(pair_off, item_off, list_off, pair_q, tau, nlist,
+    const int64_t max_slots = (np / G + nlist) * FG;  // >= FG * total items
+    item_expand_kernel<<<nblk(max_slots, 256), 256, 0, s>>>(pair_off, item_off, list_off, pair_q, tau, nlist, G,
                                                           total_items, max_slots, items);
     return hipGetLastError();
 }
@@ -952,11 +966,11 @@ hipError_t launch_seed_tau(const SeedArgs &a, int m, hipStream_t s) {
     switch (m) {
         case 8:
             seed_tau_kernel<8><<<a.nq, 256, 0, s>>>(a.queries, a.coarse, a.pq, a.pnorm, a.pmax, a.list_off, a.codes,
-                                                     a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.tau);
+                                                     a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.klq, a.tau);
             break;
         case 16:
             seed_tau_kernel<16><<<a.nq, 256, 0, s>>>(a.queries, a.coarse, a.pq, a.pnorm, a.pmax, a.list_off, a.codes,
-                                                      a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.tau);
+                                                      a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.klq, a.tau);
             break;
         default: return hipErrorInvalidValue;
     }

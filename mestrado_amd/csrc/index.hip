@@ -243,30 +243,6 @@ int check_index(const mivf_index *ix) {
 }
 
 
-// Coarse top-w of nq queries into probes/dis [nq][w] (needs ix->ws.dmat sized
-// nq * nlist): query norms, distances (timer mark), selection.
-int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_t *probes, float *dis, hipStream_t st,
-                 Timer *tm) {
-    const int d = ix->d, nlist = ix->nlist;
-    MIVF_TRY(ws.qn.ensure(sizeof(double) * nq));
-    MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ws.qn.as<double>(), st));
-    if (coarse_mfma_ok(d, nlist, w) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0) {
-        MIVF_CHECK_HIP(launch_coarse_approx(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), nlist, d_q,
-                                            ws.qn.as<double>(), nq, ws.dmat.as<float>(), nlist, st));
-        if (tm) MIVF_TRY(tm->mark());
-        MIVF_TRY(ws.redo.ensure(sizeof(int32_t) * (size_t(nq) + 1)));
-        int32_t *redo = ws.redo.as<int32_t>();
-        MIVF_CHECK_HIP(launch_coarse_select(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), ix->maxcn, nlist,
-                                            d_q, ws.qn.as<double>(), nq, w, ws.dmat.as<float>(), nlist, probes,
-                                            dis, redo + 1, redo, st));
-    } else {
-        MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), nlist, d_q, d,
-                                         ws.qn.as<double>(), nq, d, ws.dmat.as<float>(), nlist, st));
-        if (tm) MIVF_TRY(tm->mark());
-        MIVF_CHECK_HIP(launch_row_topk(ws.dmat.as<float>(), nq, nlist, nlist, w, probes, dis, st));
-    }
-    return kOk;
-}
 
 int choose_groups(const mivf_index *ix, int nq, int wmax, int *ppg) {
     int p = ix->ppg_opt;
@@ -362,6 +338,35 @@ int finish_timer(mivf_index *ix, Timer &tm) {
 
 
 }  // namespace
+
+namespace mivf {
+
+// Coarse top-w of nq queries into probes/dis [nq][w] (needs ix->ws.dmat sized
+// nq * nlist): query norms, distances (timer mark), selection.
+int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_t *probes, float *dis, hipStream_t st,
+                 Timer *tm) {
+    const int d = ix->d, nlist = ix->nlist;
+    MIVF_TRY(ws.qn.ensure(sizeof(double) * nq));
+    MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ws.qn.as<double>(), st));
+    if (coarse_mfma_ok(d, nlist, w) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0) {
+        MIVF_CHECK_HIP(launch_coarse_approx(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), nlist, d_q,
+                                            ws.qn.as<double>(), nq, ws.dmat.as<float>(), nlist, st));
+        if (tm) MIVF_TRY(tm->mark());
+        MIVF_TRY(ws.redo.ensure(sizeof(int32_t) * (size_t(nq) + 1)));
+        int32_t *redo = ws.redo.as<int32_t>();
+        MIVF_CHECK_HIP(launch_coarse_select(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), ix->maxcn, nlist,
+                                            d_q, ws.qn.as<double>(), nq, w, ws.dmat.as<float>(), nlist, probes,
+                                            dis, redo + 1, redo, st));
+    } else {
+        MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), nlist, d_q, d,
+                                         ws.qn.as<double>(), nq, d, ws.dmat.as<float>(), nlist, st));
+        if (tm) MIVF_TRY(tm->mark());
+        MIVF_CHECK_HIP(launch_row_topk(ws.dmat.as<float>(), nq, nlist, nlist, w, probes, dis, st));
+    }
+    return kOk;
+}
+
+}  // namespace mivf
 
 namespace mivf {
 
@@ -531,6 +536,12 @@ int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, cons
             set_error("index_create: reduced-precision operand setup failed");
             return fail(kHip);
         }
+    }
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess)
+            hipDeviceGetAttribute(&ix->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (ix->ncu <= 0) ix->ncu = 256;
     }
     if ((rc = upload_lists(ix))) return fail(rc);
     *out = ix;
@@ -738,7 +749,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     // fast_seed, then as many as it takes to hold kl vectors)
     MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
     MIVF_TRY(ix->f_tau.ensure(sizeof(float) * nq));
-    MIVF_CHECK_HIP(launch_seed_count(ix->ws.probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), nlist, kl,
+    MIVF_CHECK_HIP(launch_seed_count(ix->ws.probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), nlist, kl, nullptr,
                                      std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), st));
     {
         SeedArgs a{};
@@ -770,13 +781,13 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(ix->f_items.ensure(sizeof(FastItem) * std::max<int64_t>(np, 1)));
     MIVF_TRY(ix->f_ctrl.ensure(sizeof(int) * 4));
     int *ctrl = ix->f_ctrl.as<int>();  // [0] total items, [1] next item
-    MIVF_CHECK_HIP(launch_pairs(ix->ws.probes.as<int32_t>(), nq, w, nullptr, ix->off.as<int64_t>(), ix->f_tau.as<float>(), nlist,
+    MIVF_CHECK_HIP(launch_pairs(ix->ws.probes.as<int32_t>(), nq, w, nullptr, ix->off.as<int64_t>(), ix->f_tau.as<float>(), nlist, fast_group_size(),
                                 ix->f_cnt.as<int>(), ix->f_poff.as<int>(), ix->f_ioff.as<int>(), ctrl,
                                 ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(), ix->f_items.as<FastItem>(), st));
     MIVF_TRY(ix->f_cand.ensure(sizeof(uint64_t) * size_t(nq) * capq));
     MIVF_TRY(ix->f_ccnt.ensure(sizeof(int) * nq));
-    MIVF_CHECK_HIP(hipMemsetAsync(ix->f_ccnt.p, 0, sizeof(int) * nq, st));
-    MIVF_CHECK_HIP(hipMemsetAsync(ctrl + 1, 0, sizeof(int), st));
+    MIVF_CHECK_HIP(launch_fill_i32(ix->f_ccnt.as<int>(), nq, 0, st));
+    MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, st));
     MIVF_TRY(tm.mark());
     // 4. persistent list-major scan
     {
@@ -878,7 +889,7 @@ int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
     MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
     MIVF_TRY(coarse_stage(ix, ix->ws, d_q, nq, w, ix->ws.probes.as<int32_t>(), ix->ws.coadis.as<float>(), nullptr, nullptr));
     MIVF_CHECK_HIP(launch_seed_count(ix->ws.probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), ix->nlist, kl,
-                                     std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), nullptr));
+                                     nullptr, std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), nullptr));
     SeedArgs a{};
     a.queries = d_q;
     a.coarse = ix->coarse.as<float>();
@@ -919,6 +930,15 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) {
     } else if (n == "fast_seed_lists") {
         MIVF_REQUIRE(value >= 1 && value <= 1024, kInvalid, "fast_seed_lists must be in 1..1024");
         ix->fast_seed = int(value);
+    } else if (n == "batched_capacity") {
+        MIVF_REQUIRE(value >= 64 && value <= (1 << 20), kInvalid, "batched_capacity must be in 64..2^20");
+        ix->batched_cap = int(value);
+    } else if (n == "batched_group") {
+        MIVF_REQUIRE(exact_lds_bytes(ix->m, int(value)) > 0, kInvalid, "batched_group: not a compiled shape");
+        ix->batched_g = int(value);
+    } else if (n == "batched_threads") {
+        MIVF_REQUIRE(value == 256 || value == 512, kInvalid, "batched_threads must be 256 or 512");
+        ix->batched_nt = int(value);
     } else if (n == "fast_capacity") {
         MIVF_REQUIRE(value >= 1024 && value <= (1 << 20), kInvalid, "fast_capacity must be in 1024..2^20");
         ix->fast_cap = int(value);

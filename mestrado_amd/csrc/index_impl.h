@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -100,6 +101,11 @@ struct mivf_index {
     int ppg_opt = 0;
     int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
     int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
+    int batched_cap = 4096;   // candidate capacity per query of the batched exact form (>= 32 kmax)
+    int batched_g = 8, batched_nt = 512;  // its work-item shape (queries per item, threads)
+    int batched_last_overflow = 0;
+    int ncu = 256;            // compute units of the device
+    std::shared_ptr<void> xws;  // workspace of the batched exact form (batched.hip)
     bool profile = false;
     float timings[8] = {};
     hipEvent_t ev[8] = {};
@@ -118,5 +124,8 @@ namespace mivf {
 int search_entry(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, const int32_t *d_wq,
                  const int32_t *d_kq, int kuni, int kstride, int32_t *d_ids, float *d_dis, hipStream_t st);
 int upload_lists(mivf_index *ix);
+// Coarse top-w of nq queries into probes/dis [nq][w] (ws.dmat sized nq * nlist).
+int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_t *probes, float *dis, hipStream_t st,
+                 Timer *tm);
 
 }  // namespace mivf

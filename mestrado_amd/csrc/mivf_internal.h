@@ -180,18 +180,50 @@ hipError_t launch_fast_prep(const float *pq, const float *pmax, int m, int ks, v
 // Seed lists per query: the smallest ns >= min_seed whose first ns probed
 // lists hold >= kl vectors (or ns = w).
 hipError_t launch_seed_count(const int32_t *probes, int nq, int w, const int64_t *list_off, int nlist, int kl,
-                             int min_seed, int32_t *ns, hipStream_t s);
+                             const int32_t *klq, int min_seed, int32_t *ns, hipStream_t s);
+hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t s);
 // (list, query) pairs of probe ranks >= rank0[q] (0 if rank0 is null), grouped by list, and the
 // work items (list, first pair) of at most fast_group_size() pairs.
+// G = queries per work item (1..16).  No memset: the counters are cleared by
+// kernels, so the sequence can be captured in a graph.
 hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *rank0, const int64_t *list_off,
-                        const float *tau, int nlist, int *cnt, int *pair_off, int *item_off, int *total_items,
+                        const float *tau, int nlist, int G, int *cnt, int *pair_off, int *item_off, int *total_items,
                         int *fill, int32_t *pair_q, FastItem *items, hipStream_t s);
+// Batched exact search (exact.hip).
+struct ExactArgs {
+    const float *queries;       // [nq][d]
+    const float *coarse;        // [nlist][d]
+    const float *pq;            // [m][ks][ds]
+    const double *pq_norm;      // [m][ks] f64 norms (table side of the reference formula)
+    const int32_t *list_ids;    // [N]
+    const uint8_t *codes;       // [N][m]
+    const FastItem *items;      // work items of G queries
+    const int *total_items;
+    int *next_item;             // work-queue counter (cleared before launch)
+    uint64_t *cand;             // [nq][capq] (distance, id) keys
+    int *cand_cnt;              // [nq]
+    int capq;
+    int ks;
+    int *err;
+};
+size_t exact_lds_bytes(int m, int g);
+// Compiled (m, g, threads): (16,4,256) (16,8,256) (16,8,512) (8,4,256) (8,8,256) (8,8,512) (8,16,512).
+hipError_t launch_exact_scan(const ExactArgs &a, int m, int g, int nt, int grid, hipStream_t s);
+hipError_t launch_mask_probes(int32_t *probes, int nq, int w, const int32_t *wq, hipStream_t s);
+// Test hook: exact tables [m][256][8] of queries 0..n-1 (n <= 8) against list l.
+hipError_t launch_exact_tables(const ExactArgs &a, int m, int l, int n, float *out, hipStream_t s);
+hipError_t launch_exact_adc(const ExactArgs &a, int m, int l, int n, int64_t beg, int len, float *out, hipStream_t s);
+// overflow [nq + 1]: per-query flags, then their count (cleared by the caller).
+hipError_t launch_exact_select(const uint64_t *cand, const int *cand_cnt, int capq, const int32_t *kq, int kuni,
+                               int kstride, int nq, int32_t *out_ids, float *out_dis, int *overflow, hipStream_t s);
+
 struct SeedArgs {
     const float *queries, *coarse, *pq, *pnorm, *pmax;
     const int64_t *list_off;
     const uint8_t *codes;
     const int32_t *probes;  // [nq][w]
     const int32_t *nseed;   // [nq] seed lists per query
+    const int32_t *klq;     // [nq] per-query kl or null (uniform kl)
     int nq, w, nlist, ks, kl;
     float *tau;             // [nq]
 };

@@ -1,0 +1,109 @@
+"""GPU tests of the batched exact form (mivf_search_batched, mivf_plan_*):
+per-query w and k, list-major persistent kernel, results bit-identical to the
+plain per-query search (mivf_search_var), for every compiled work-item shape,
+with the overflow fallback forced, and through a captured hipGraph replayed
+several times with eager work between the replays."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(ids, dis, rids, rdis, what):
+    bad = np.nonzero((ids != rids).any(1) | (dis.view(np.uint32) != rdis.view(np.uint32)).any(1))[0]
+    assert bad.size == 0, f"{what}: {bad.size} rows differ, first {bad[:5]}"
+
+
+def uneven(nq, seed, wmax=64):
+    rng = np.random.default_rng(seed)
+    return rng.integers(1, wmax + 1, nq).astype(np.int32), rng.choice([1, 10, 100], nq).astype(np.int32)
+
+
+@pytest.mark.parametrize("g,nt", [(8, 512), (8, 256), (4, 256)])
+def test_batched_equals_plain_m16(mivf, zipf16, g, nt):
+    ix, index, query, _ = zipf16
+    w, k = uneven(query.shape[0], seed=g + nt)
+    rids, rdis = ix.search_var(query, w, k, 100)
+    ix.set_option("batched_group", g)
+    ix.set_option("batched_threads", nt)
+    try:
+        ids, dis = ix.search_batched(query, w, k, 100)
+    finally:
+        ix.set_option("batched_group", 8)
+        ix.set_option("batched_threads", 512)
+    assert_same(ids, dis, rids, rdis, f"batched g={g} nt={nt}")
+
+
+def test_batched_overflow_fallback(mivf, zipf16):
+    """A tiny candidate buffer overflows for most queries; the plain rerun of
+    those queries keeps the results exact."""
+    ix, index, query, _ = zipf16
+    w, k = uneven(query.shape[0], seed=9)
+    rids, rdis = ix.search_var(query, w, k, 100)
+    ix.set_option("batched_capacity", 64)
+    try:
+        ids, dis = ix.search_batched(query, w, k, 100)
+    finally:
+        ix.set_option("batched_capacity", 4096)
+    assert_same(ids, dis, rids, rdis, "batched with overflow")
+
+
+def test_batched_m8_ties_empty_lists(mivf, orc, c1):
+    """m = 8 index with empty lists and few distinct codes (exact distance
+    ties -> lower id first; fewer candidates than k -> padding)."""
+    q = c1["gold_q"]
+    nlist = q["coarse"].shape[0]
+    rng = np.random.default_rng(11)
+    sizes = rng.integers(0, 40, nlist)
+    sizes[::3] = 0
+    off = np.zeros(nlist + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    n = int(off[-1])
+    ids = rng.permutation(n * 3)[:n].astype(np.int32)
+    codes = rng.integers(0, 4, (n, 8)).astype(np.uint8)
+    ix = mivf.Index(q["coarse"], q["pq"])
+    ix.set_lists(off, ids, codes)
+    query = c1["query"]
+    for g, nt in ((8, 512), (16, 512), (4, 256)):
+        ix.set_option("batched_group", g)
+        ix.set_option("batched_threads", nt)
+        w, k = uneven(query.shape[0], seed=g, wmax=64)
+        k[::5] = 1000
+        rids, rdis = ix.search_var(query, w, k, 1000)
+        bids, bdis = ix.search_batched(query, w, k, 1000)
+        assert_same(bids, bdis, rids, rdis, f"m8 ties g={g}")
+
+
+def test_plan_graph_replays(mivf, zipf16):
+    """Autotuned plan, captured graph, four replays with different queries, w
+    and k, eager searches in between; every replay equals the eager result."""
+    ix, index, query, _ = zipf16
+    nq = 200
+    plan = mivf.Plan(ix, nq, 64, 100, autotune=True)
+    rng = np.random.default_rng(4)
+    for rep in range(4):
+        rows = rng.choice(query.shape[0], nq, replace=False)
+        qs = np.ascontiguousarray(query[rows])
+        w, k = uneven(nq, seed=100 + rep)
+        ids, dis = plan.search(qs, w, k)
+        rids, rdis = ix.search_var(qs, w, k, 100)  # eager work between replays
+        assert_same(ids, dis, rids, rdis, f"graph replay {rep}")
+    info = plan.info()
+    assert info["group"] in (4, 8, 16) and info["threads"] in (256, 512) and info["tuned_ms"] > 0
+    plan.close()
+
+
+def test_exact_tables_and_scan_adc_bitexact(mivf, orc, zipf16):
+    """The batched kernel's FP32 tables and the ADC distances of its scan loop
+    equal the reference arithmetic (the oracle) bit for bit, per query slot."""
+    ix, index, query, _ = zipf16
+    coarse, pq, off, ids, codes = index
+    sizes = np.diff(off)
+    for l in (0, int(np.argmax(sizes)), int(np.nonzero(sizes)[0][-1])):
+        t = ix.exact_tables(query[:8], l)
+        d = ix.exact_adc(query[:8], l, int(sizes[l]))
+        for g in range(8):
+            ref = orc.tables((query[g] - coarse[l]).astype(np.float32), pq)
+            assert np.array_equal(t[:, :, g].view(np.uint32), ref.view(np.uint32)), f"table list {l} slot {g}"
+            adc = orc.adc(ref, codes[off[l]:off[l + 1]])
+            assert np.array_equal(d[:, g].view(np.uint32), adc.view(np.uint32)), f"ADC list {l} slot {g}"
