@@ -2,21 +2,26 @@
 // arithmetic, computed in two steps:
 //
 // 1. coarse_approx_kernel: D'(q, c) = fl(|c|^2 + |q|^2) - 2 q.c with the dot
-//    products on the matrix cores (v_mfma_f32_16x16x4_f32, FP32 products,
-//    FP32 accumulation in MFMA order).
-// 2. coarse_select_kernel (one workgroup per query): v' = the w-th smallest
-//    D', then every centroid with D' <= v' + 2 M_q is re-evaluated with the
-//    exact reference arithmetic (sequential FP32 dot, a = centroid with its
+//    products on the matrix cores in BF16 with FP32 accumulation
+//    (v_mfma_f32_16x16x32_bf16).  Each operand is split x = hi + lo (two
+//    BF16 terms, hi = bf16(x), lo = bf16(x - hi)), and q.c is accumulated as
+//    hi.hi + hi.lo + lo.hi: every BF16 x BF16 product is exact in FP32.
+// 2. coarse_select_*_kernel: v' = an upper bound of the w-th smallest D', then
+//    every centroid with D' <= v' + 2 M_q is re-evaluated with the exact
+//    reference arithmetic (sequential FP32 dot, a = centroid with its
 //    f32-accumulated norm, b = query with its f64-accumulated norm,
 //    D = fl(fl(-2 dot) + fl(|q|^2 + |c|^2)), parallel/ivf_pq/ivf_assign.cpp:28
 //    via knn_full) and the exact top-w by (distance, index) is kept.
 //
-// Why the recheck set holds the exact top-w: any two FP32 evaluations of a
-// 128-term dot product differ by at most ~2 * 128 * 2^-24 * sum|q_t c_t|
-// <= 1.6e-5 (|q|^2 + |c|^2) (sum|q_t c_t| <= (|q|^2 + |c|^2) / 2), and the
-// norm terms add a few ulps of |q|^2 + |c|^2, so |D' - D| <= M_q with
-// M_q = 1e-4 (|q|^2 + max_c |c|^2) (a factor ~5 of headroom).  w centroids
-// have D' <= v', hence D <= v' + M_q for them and the w-th exact distance is
+// Why the recheck set holds the exact top-w.  Two BF16 terms keep 16
+// significant bits, so x = hi + lo + r with |r| <= 2^-16 |x|; the dropped
+// lo.lo and r terms bound the split product's error by 3 * 2^-16 |q_t c_t|,
+// and FP32 accumulation of 3 * 128 products adds <= 384 * 2^-24 sum|q_t c_t|.
+// With sum|q_t c_t| <= (|q|^2 + |c|^2) / 2, the approximate and the exact
+// (sequential FP32) dot products differ by <= 3.3e-5 (|q|^2 + |c|^2); -2 dot
+// doubles it and the norm terms add a few ulps, so |D' - D| <= M_q with
+// M_q = 1.5e-4 (|q|^2 + max_c |c|^2) (over 2x headroom).  w centroids have
+// D' <= v', hence D <= v' + M_q for them and the w-th exact distance is
 // <= v' + M_q; a centroid of the exact top-w therefore has D' <= D + M_q <=
 // v' + 2 M_q.  A centroid outside the set has D > v' + M_q >= the w-th exact
 // distance, so it loses to all of the top-w, ties included.
@@ -33,16 +38,27 @@ constexpr int CP = CD + 4;    // padded LDS row (floats): conflict-free column r
 constexpr int NTC = 256;
 constexpr int RCAP = 1024;    // recheck capacity per query
 
-// One 64-query x 256-centroid tile of D'.  The query tile sits in LDS; wave
-// wv owns centroids 64 wv .. 64 wv + 63 of the tile (four 16-wide column
-// tiles, operands read straight from global memory one k block ahead) and
-// all four 16-query row tiles.  MFMA step j of a 16-wide k block uses, for
-// lane group h = l/16, element k = k0 + 4h + j of both operands (a
-// permutation of the k sum), so one 16-byte read feeds four steps.
+// One 64-query x 256-centroid tile of D'.  The query tile sits in LDS (FP32,
+// split to BF16 in registers); wave wv owns centroids 64 wv .. 64 wv + 63 of
+// the tile (four 16-wide column tiles whose BF16 hi/lo rows are read from
+// global memory one k block ahead) and all four 16-query row tiles.  In a
+// 32-wide k block, lane group h = l/16 supplies elements k0 + 8h .. k0 + 8h + 7
+// of both operands (the same k permutation on both sides of the sum).
 constexpr int CQ = 64;   // queries per tile
 constexpr int CC = 256;  // centroids per tile
 
-__global__ __launch_bounds__(NTC) void coarse_approx_kernel(const float *__restrict__ coarse,
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline void split_bf16(const float (&x)[8], bf16x8 &hi, bf16x8 &lo) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        hi[j] = __bf16(x[j]);
+        lo[j] = __bf16(x[j] - float(hi[j]));
+    }
+}
+
+__global__ __launch_bounds__(NTC) void coarse_approx_kernel(const __bf16 *__restrict__ chi,
+                                                            const __bf16 *__restrict__ clo,
                                                             const float *__restrict__ cn, int nlist,
                                                             const float *__restrict__ queries,
                                                             const double *__restrict__ qn, int nq,
@@ -60,12 +76,15 @@ __global__ __launch_bounds__(NTC) void coarse_approx_kernel(const float *__restr
     const int lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 4, i16 = lane & 15;
     const int c0 = blockIdx.x * CC + wave * 64;
-    const float *crow[4];
+    int64_t crow[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) crow[t] = coarse + int64_t(min(c0 + 16 * t + i16, nlist - 1)) * CD + 4 * h;
-    f32x4 bn[4];
+    for (int t = 0; t < 4; ++t) crow[t] = int64_t(min(c0 + 16 * t + i16, nlist - 1)) * CD + 8 * h;
+    bf16x8 bhn[4], bln[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) bn[t] = *reinterpret_cast<const f32x4 *>(crow[t]);
+    for (int t = 0; t < 4; ++t) {
+        bhn[t] = *reinterpret_cast<const bf16x8 *>(chi + crow[t]);
+        bln[t] = *reinterpret_cast<const bf16x8 *>(clo + crow[t]);
+    }
     f32x4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -73,24 +92,35 @@ __global__ __launch_bounds__(NTC) void coarse_approx_kernel(const float *__restr
         for (int t = 0; t < 4; ++t) acc[a][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     __syncthreads();
 #pragma unroll 1
-    for (int k0 = 0; k0 < CD; k0 += 16) {
-        f32x4 b[4];
+    for (int k0 = 0; k0 < CD; k0 += 32) {
+        bf16x8 bh[4], bl[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) b[t] = bn[t];
-        if (k0 + 16 < CD) {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) bn[t] = *reinterpret_cast<const f32x4 *>(crow[t] + k0 + 16);
+        for (int t = 0; t < 4; ++t) {
+            bh[t] = bhn[t];
+            bl[t] = bln[t];
         }
-        f32x4 a[4];
+        if (k0 + 32 < CD) {
 #pragma unroll
-        for (int qt = 0; qt < 4; ++qt) a[qt] = *reinterpret_cast<const f32x4 *>(Qs + (16 * qt + i16) * CP + k0 + 4 * h);
+            for (int t = 0; t < 4; ++t) {
+                bhn[t] = *reinterpret_cast<const bf16x8 *>(chi + crow[t] + k0 + 32);
+                bln[t] = *reinterpret_cast<const bf16x8 *>(clo + crow[t] + k0 + 32);
+            }
+        }
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int qt = 0; qt < 4; ++qt) {
+            const float *qr = Qs + (16 * qt + i16) * CP + k0 + 8 * h;
+            const float4 x0 = *reinterpret_cast<const float4 *>(qr);
+            const float4 x1 = *reinterpret_cast<const float4 *>(qr + 4);
+            const float xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            bf16x8 ah, al;
+            split_bf16(xv, ah, al);
 #pragma unroll
-            for (int qt = 0; qt < 4; ++qt)
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[qt][j], b[t][j], acc[qt][t], 0, 0, 0);
+            for (int t = 0; t < 4; ++t) {
+                acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[qt][t], 0, 0, 0);
+                acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[qt][t], 0, 0, 0);
+                acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[qt][t], 0, 0, 0);
+            }
+        }
     }
     // accumulator r of lane l: query row 16 qt + 4h + r, centroid column 16 t + l % 16
 #pragma unroll
@@ -106,6 +136,17 @@ __global__ __launch_bounds__(NTC) void coarse_approx_kernel(const float *__restr
                 if (q < nq) out[q * ldo + c] = float(qn[q] + cnc) - 2.0f * acc[qt][t][r];
             }
     }
+}
+
+// BF16 hi/lo split of the centroids (once per index).
+__global__ void coarse_split_kernel(const float *__restrict__ coarse, int64_t n, __bf16 *__restrict__ hi,
+                                    __bf16 *__restrict__ lo) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x = coarse[i];
+    const __bf16 h = __bf16(x);
+    hi[i] = h;
+    lo[i] = __bf16(x - float(h));
 }
 
 // Exact reference distance of centroid c to the query in LDS.
@@ -213,7 +254,7 @@ __global__ __launch_bounds__(NTC) void coarse_select_kernel(const float *__restr
     __syncthreads();
     const float vw = key_dist(uint64_t(vmax) << 32);
     auto akey = [&](int i) -> uint64_t { return ckey(row[i], i); };
-    const float lim = vw + 2.0e-4f * float(bn + double(maxcn));
+    const float lim = vw + 3.0e-4f * float(bn + double(maxcn));
     // recheck set
     for (int i = tid; i < nlist; i += NTC) {
         if (row[i] <= lim) {
@@ -326,7 +367,7 @@ __global__ __launch_bounds__(NTC) void coarse_select_wave_kernel(const float *__
         }
     }
     const float vw = __shfl(x, w - 1);
-    const float lim = vw + 2.0e-4f * float(bn + double(maxcn));
+    const float lim = vw + 3.0e-4f * float(bn + double(maxcn));
     int n = 0;
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
@@ -363,11 +404,20 @@ bool coarse_mfma_ok(int d, int nlist, int w) { return d == CD && nlist >= 1 && n
 
 size_t coarse_select_lds(int nlist) { return ((size_t(nlist) * 4 + 15) & ~size_t(15)) + RCAP * 8; }
 
-hipError_t launch_coarse_approx(const float *coarse, const float *cn, int nlist, const float *queries,
+hipError_t launch_coarse_split(const float *coarse, int64_t n, void *hi, void *lo, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    coarse_split_kernel<<<unsigned((n + 255) / 256), 256, 0, s>>>(coarse, n, static_cast<__bf16 *>(hi),
+                                                                  static_cast<__bf16 *>(lo));
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const float *queries,
                                 const double *qn, int nq, float *approx, int64_t lda, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     dim3 grid(unsigned((nlist + CC - 1) / CC), unsigned((nq + CQ - 1) / CQ));
-    coarse_approx_kernel<<<grid, NTC, CQ * CP * 4, s>>>(coarse, cn, nlist, queries, qn, nq, approx, lda);
+    coarse_approx_kernel<<<grid, NTC, CQ * CP * 4, s>>>(static_cast<const __bf16 *>(chi),
+                                                        static_cast<const __bf16 *>(clo), cn, nlist, queries, qn, nq,
+                                                        approx, lda);
     return hipGetLastError();
 }
 

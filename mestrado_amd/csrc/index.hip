@@ -349,7 +349,7 @@ int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_
     MIVF_TRY(ws.qn.ensure(sizeof(double) * nq));
     MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ws.qn.as<double>(), st));
     if (coarse_mfma_ok(d, nlist, w) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0) {
-        MIVF_CHECK_HIP(launch_coarse_approx(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), nlist, d_q,
+        MIVF_CHECK_HIP(launch_coarse_approx(ix->coarse_hi.p, ix->coarse_lo.p, ix->coarse_n32.as<float>(), nlist, d_q,
                                             ws.qn.as<double>(), nq, ws.dmat.as<float>(), nlist, st));
         if (tm) MIVF_TRY(tm->mark());
         MIVF_TRY(ws.redo.ensure(sizeof(int32_t) * (size_t(nq) + 1)));
@@ -528,6 +528,14 @@ int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, cons
             return fail(kHip);
         }
         for (float v : cn) ix->maxcn = std::max(ix->maxcn, v);
+    }
+    if ((rc = ix->coarse_hi.ensure(2 * size_t(nlist) * d)) || (rc = ix->coarse_lo.ensure(2 * size_t(nlist) * d)))
+        return fail(rc);
+    if (launch_coarse_split(ix->coarse.as<float>(), int64_t(nlist) * d, ix->coarse_hi.p, ix->coarse_lo.p, nullptr) !=
+            hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        set_error("index_create: coarse split failed");
+        return fail(kHip);
     }
     if (fast_pb_bytes(m) > 0) {  // operands of the reduced-precision tables
         if ((rc = ix->fpb.ensure(fast_pb_bytes(m)))) return fail(rc);

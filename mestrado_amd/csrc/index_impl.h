@@ -77,6 +77,7 @@ struct mivf_index {
     // quantizers (device)
     DevBuf coarse, coarse_n32, pq, pq_n64, pq_n32;
     float maxcn = 0.0f;      // max_c |c|^2 (margin of the coarse recheck)
+    DevBuf coarse_hi, coarse_lo;  // BF16 hi/lo split of the centroids (matrix-core coarse distances)
     // lists: host mirror (list-major) + device arrays
     std::vector<std::vector<int32_t>> h_ids;
     std::vector<std::vector<uint8_t>> h_codes;

@@ -100,13 +100,15 @@ hipError_t launch_keys_to_idx(const uint64_t *keys, int64_t n, int32_t *idx, hip
 hipError_t launch_row_topk(const float *D, int64_t rows, int n, int64_t ld, int k,
                            int32_t *idx, float *dis, hipStream_t s);
 
-// Coarse top-w with the exact reference arithmetic: MFMA approximate
+// Coarse top-w with the exact reference arithmetic: BF16-MFMA approximate
 // distances, then an exact recheck of every centroid within the proven error
 // margin of the w-th (d == 128, nlist <= 8192, w <= 1024, 16-byte aligned
 // queries; otherwise use launch_dist_store + launch_row_topk).
 bool coarse_mfma_ok(int d, int nlist, int w);
 size_t coarse_select_lds(int nlist);
-hipError_t launch_coarse_approx(const float *coarse, const float *cn, int nlist, const float *queries,
+// chi / clo: BF16 hi and lo terms of the centroids [nlist][d] (launch_coarse_split).
+hipError_t launch_coarse_split(const float *coarse, int64_t n, void *hi, void *lo, hipStream_t s);
+hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const float *queries,
                                 const double *qn, int nq, float *approx, int64_t lda, hipStream_t s);
 // redo [nq] and redo_cnt [1]: scratch for queries whose recheck set is large
 hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxcn, int nlist, const float *queries,
