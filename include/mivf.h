@@ -167,6 +167,13 @@ int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t 
  * kl vectors).  For tests.                                                  */
 int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, float *d_tau);
 
+/* The shortlist step of mivf_search_fast on its own: for each q < nq, the
+ * ids (low 32 bits) of the kl smallest of the keys d_cand[q*capq + i],
+ * i < min(d_cnt[q], capq), in any order, padded with -1 into d_short [nq][kl].
+ * Keys are (order-preserving distance bits << 32) | position.  For tests.   */
+int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int nq, int kl,
+                     int32_t *d_short);
+
 /* ---- options and timing ---------------------------------------------------
  * "probes_per_group" (0 = automatic), "profile" (1 = record per-stage HIP
  * event timings of the next searches), "fast_seed_lists" (lists per query

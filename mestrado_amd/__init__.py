@@ -60,6 +60,7 @@ _SIGS = {
     "mivf_index_attach_base": [_vp, _vp, _i64],
     "mivf_fast_tables": [_vp, _vp, _i, _i, _vp, _vp],
     "mivf_fast_seed_tau": [_vp, _vp, _i, _i, _i, _vp],
+    "mivf_fast_select": [_vp, _vp, _i, _i, _i, _vp],
     "mivf_search_batched": [_vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp],
     "mivf_plan_create": [_vp, _i, _i, _i, _i, ctypes.POINTER(_vp)],
     "mivf_plan_search": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -242,6 +243,18 @@ def bruteforce_nn(base, q):
     dis = DeviceArray((nq,), np.float64)
     _check(lib().mivf_bruteforce_nn(bd.ptr, nb, qd.ptr, nq, d, ids.ptr, dis.ptr))
     return ids.numpy(), dis.numpy()
+
+
+def fast_select(cand, cnt, kl):
+    """Shortlist step of the reduced-precision form: for each row of cand
+    [nq][capq] uint64 keys, the ids (low 32 bits) of the kl smallest of its
+    first cnt[q] keys, in any order, padded with -1."""
+    cd, _ = _dev(cand, np.uint64)
+    nd, _ = _dev(cnt, np.int32)
+    nq, capq = cd.shape
+    out = DeviceArray((nq, kl), np.int32)
+    _check(lib().mivf_fast_select(cd.ptr, nd.ptr, capq, nq, kl, out.ptr))
+    return out.numpy()
 
 
 class Index:

@@ -844,36 +844,114 @@ __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__
     }
 }
 
-// ---- per query: the kl best (approximate distance, list position) keys of the candidates
+// ---- per query: the kl best (approximate distance, list position) keys of
+// the candidates (in any order; the rerank sorts).  A 256-bin histogram of
+// the keys' distance bits over [min, max] finds the bin b holding the kl-th
+// key; keys of lower bins are taken whole, and the few keys of bin b go to
+// LDS, where an exact radix select picks the missing ones.  When bin b holds
+// more than SELB keys (heavy ties), a radix select over all keys is used.
+constexpr int SELB = 1024;
+
 __global__ __launch_bounds__(256) void fast_select_kernel(int kl, const uint64_t *__restrict__ cand,
                                                           const int *__restrict__ cand_cnt, int capq,
                                                           int32_t *__restrict__ short_ids) {
     __shared__ SelectShared sh;
-    __shared__ uint64_t buf[1024];
-    __shared__ int cnt, eqt;
+    __shared__ uint64_t buf[SELB];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t wmn[4], wmx[4];
+    __shared__ int sbin, sbelow, cnt, nb, eqt;
     const int q = blockIdx.x;
     const int tid = threadIdx.x;
     const int n = min(cand_cnt[q], capq);
     const uint64_t *C = cand + int64_t(q) * capq;
-    auto keyfn = [&](int i) -> uint64_t { return C[i]; };
-    SelectResult r = block_select<256>(keyfn, n, kl, sh);
+    int32_t *out = short_ids + int64_t(q) * kl;
+    if (n <= kl) {  // uniform: every candidate is in
+        for (int i = tid; i < kl; i += 256) out[i] = i < n ? key_id(C[i]) : -1;
+        return;
+    }
+    uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+    for (int i = tid; i < n; i += 256) {
+        const uint32_t h = uint32_t(C[i] >> 32);
+        mn = min(mn, h);
+        mx = max(mx, h);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, uint32_t(__shfl_xor(int(mn), off)));
+        mx = max(mx, uint32_t(__shfl_xor(int(mx), off)));
+    }
+    if ((tid & 63) == 0) {
+        wmn[tid >> 6] = mn;
+        wmx[tid >> 6] = mx;
+    }
+    hist[tid] = 0u;
     if (tid == 0) {
         cnt = 0;
+        nb = 0;
         eqt = 0;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += 256) {
-        const uint64_t key = C[i];
-        bool take = key < r.thr;
-        if (!take && r.eq > 0 && key == r.thr) take = atomicAdd(&eqt, 1) < r.eq;
-        if (take) {
-            const int pos = atomicAdd(&cnt, 1);
-            if (pos < 1024) buf[pos] = key;
+    mn = min(min(wmn[0], wmn[1]), min(wmn[2], wmn[3]));
+    mx = max(max(wmx[0], wmx[1]), max(wmx[2], wmx[3]));
+    // any monotone map of the distance bits to bins keeps the order
+    const float scale = 256.0f / (float(mx - mn) + 1.0f);
+    auto bin = [&](uint64_t k) { return min(255, int(float(uint32_t(k >> 32) - mn) * scale)); };
+    for (int i = tid; i < n; i += 256) atomicAdd(&hist[bin(C[i])], 1u);
+    __syncthreads();
+    if (tid < 64) {
+        const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
+        const uint32_t tot = h0 + h1 + h2 + h3;
+        uint32_t incl = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, 64);
+            if (tid >= off) incl += o;
+        }
+        const uint32_t excl = incl - tot;
+        const uint32_t need = uint32_t(kl);
+        if (excl < need && incl >= need) {
+            uint32_t c = excl;
+            int dg;
+            if (c + h0 >= need) dg = 0;
+            else if (c + h0 + h1 >= need) { c += h0; dg = 1; }
+            else if (c + h0 + h1 + h2 >= need) { c += h0 + h1; dg = 2; }
+            else { c += h0 + h1 + h2; dg = 3; }
+            sbin = 4 * tid + dg;
+            sbelow = int(c);
         }
     }
     __syncthreads();
-    const int c = min(cnt, kl);
-    for (int i = tid; i < kl; i += 256) short_ids[int64_t(q) * kl + i] = i < c ? key_id(buf[i]) : -1;
+    const int b = sbin, below = sbelow;
+    if (int(hist[b]) > SELB) {  // uniform: heavy ties, radix select over all keys
+        auto keyfn = [&](int i) -> uint64_t { return C[i]; };
+        const SelectResult r = block_select<256>(keyfn, n, kl, sh);
+        for (int i = tid; i < n; i += 256) {
+            const uint64_t key = C[i];
+            bool take = key < r.thr;
+            if (!take && r.eq > 0 && key == r.thr) take = atomicAdd(&eqt, 1) < r.eq;
+            if (take) {
+                const int pos = atomicAdd(&cnt, 1);
+                if (pos < kl) out[pos] = key_id(key);
+            }
+        }
+        return;
+    }
+    for (int i = tid; i < n; i += 256) {
+        const uint64_t key = C[i];
+        const int bk = bin(key);
+        if (bk < b) out[atomicAdd(&cnt, 1)] = key_id(key);
+        else if (bk == b) buf[atomicAdd(&nb, 1)] = key;
+    }
+    __syncthreads();
+    const int m = nb, need = kl - below;
+    auto bkey = [&](int i) -> uint64_t { return buf[i]; };
+    const SelectResult r = block_select<256>(bkey, m, need, sh);
+    for (int i = tid; i < m; i += 256) {
+        const uint64_t key = buf[i];
+        bool take = key < r.thr;
+        if (!take && r.eq > 0 && key == r.thr) take = atomicAdd(&eqt, 1) < r.eq;
+        if (take) out[atomicAdd(&cnt, 1)] = key_id(key);  // cnt starts at `below`
+    }
 }
 
 inline unsigned nblk(int64_t n, int t) { return unsigned((n + t - 1) / t); }

@@ -926,6 +926,15 @@ int mivf_fast_stats(mivf_index *ix, int64_t *candidates, int64_t *overflowed_que
     return kOk;
 }
 
+int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int nq, int kl, int32_t *d_short) {
+    MIVF_REQUIRE(capq >= 1 && nq >= 0 && kl >= 1, kInvalid, "fast_select: bad sizes");
+    MIVF_REQUIRE(d_cand && d_cnt && d_short, kInvalid, "fast_select: null pointer");
+    if (nq == 0) return kOk;
+    MIVF_CHECK_HIP(launch_fast_select(kl, d_cand, d_cnt, capq, nq, d_short, nullptr));
+    MIVF_CHECK_HIP(hipDeviceSynchronize());
+    return kOk;
+}
+
 int mivf_set_option(mivf_index *ix, const char *name, int64_t value) {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(name, kInvalid, "set_option: null name");

@@ -232,3 +232,41 @@ def test_fast_seed_threshold(mivf, orc, zipf16, c1, which, kl, seed_lists):
         kth = float(vals[kl - 1])
         assert tau[qi] >= kth, f"q={qi}: tau {tau[qi]} below the kl-th seed distance {kth}"
         assert tau[qi] <= kth * 1.001 + 1.0, f"q={qi}: tau {tau[qi]} loose against {kth}"
+
+
+@pytest.mark.parametrize("case", ["distinct", "ties", "one_value", "few_values", "negative"])
+def test_fast_select_picks_the_kl_smallest_keys(mivf, orc, case):
+    """The shortlist step alone: the ids of the kl smallest (distance, position)
+    keys of each query's candidates, whatever their spread.  Rows cover empty,
+    shorter than kl, exactly kl, and more than the buffer (count clipped);
+    heavy ties push the distance histogram past its LDS bin and into the
+    all-keys radix select."""
+    rng = np.random.default_rng(7)
+    capq, kl = 6000, 100
+    counts = [0, 1, kl - 1, kl, kl + 1, 700, 3000, capq, capq + 50, 4096]
+    nq = len(counts)
+    cand = np.zeros((nq, capq), np.uint64)
+    for q, c in enumerate(counts):
+        n = min(c, capq)
+        if case == "distinct":
+            d = rng.uniform(0.0, 1e5, n).astype(np.float32)
+        elif case == "ties":
+            d = rng.integers(0, 40, n).astype(np.float32) * np.float32(1.5)
+        elif case == "one_value":
+            d = np.full(n, 123.25, np.float32)
+        elif case == "few_values":
+            d = np.where(rng.random(n) < 0.02, rng.uniform(0, 10, n), 50.0).astype(np.float32)
+        else:
+            d = rng.normal(0.0, 1.0, n).astype(np.float32) * np.float32(1e3)
+        pos = rng.permutation(1 << 20)[:n]
+        cand[q, :n] = orc.ckey(d, pos)
+        cand[q, n:] = rng.integers(0, 1 << 62, capq - n, dtype=np.uint64)  # beyond the count: ignored
+    short = mivf.fast_select(cand, np.array(counts, np.int32), kl)
+    for q, c in enumerate(counts):
+        n = min(c, capq)
+        want = np.sort(cand[q, :n])[:kl] & np.uint64(0xFFFFFFFF)
+        got = short[q]
+        m = min(n, kl)
+        assert (got[m:] == -1).all(), f"{case} q={q}: padding"
+        assert np.array_equal(np.sort(got[:m].astype(np.int64)), np.sort(want.astype(np.int64))), \
+            f"{case} q={q} n={n}: not the kl smallest keys"
