@@ -42,7 +42,8 @@ def main():
     ap.add_argument("--serve-fracs", default="0.25,0.5,0.75", help="offered load as fractions of the batch QPS")
     ap.add_argument("--serve-reqs", type=int, default=12000)
     ap.add_argument("--profile-only", action="store_true",
-                    help="build, run the timed steps and a 1 GiB device copy (counter calibration), then exit")
+                    help="run the timed steps of the search forms and a 1 GiB device copy "
+                         "(counter calibration), then exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -128,7 +129,7 @@ def main():
                               zip(("coarse", "seed", "pairs", "scan8", "select", "rerank"), fstage)}}
         log(f"fast form: {fast}")
     uneven = None
-    if not args.no_uneven and not args.profile_only:
+    if not args.no_uneven:
         # batched uneven exact search: per-query w ~ U{1..64}, k ~ {1, 10, 100}
         rng = np.random.default_rng(seed)
         wq = rng.integers(1, 65, nq).astype(np.int32)
@@ -198,14 +199,30 @@ def main():
     ids_h = out_ids.numpy()
     ng = min(args.gt_queries, nq)
     qsub = M.DeviceArray.from_numpy(query.numpy()[:ng])
-    gt, _ = M.bruteforce_nn(base, qsub)
-    def recall(ids_h):
-        return {f"recall@{R}": round(100.0 * float(np.mean([gt[i] in ids_h[i, :R] for i in range(ng)])), 3)
-                for R in (1, 10, 100) if R <= k}
+    gt, gt_d = M.bruteforce_nn(base, qsub)
+    base_h = base.numpy()  # host copy for the FP64 distances of returned ids (untimed)
+    q64 = qsub.numpy().astype(np.float64)
 
-    rec = recall(ids_h)
+    def recall(ids_h, dis_h):
+        r = {f"recall@{R}": round(100.0 * float(np.mean([gt[i] in ids_h[i, :R] for i in range(ng)])), 3)
+             for R in (1, 10, 100) if R <= k}
+        # distance error against FP64: (1) the returned distance of each
+        # returned id vs its FP64 squared distance, (2) the first result's
+        # FP64 distance vs the exact nearest neighbour's
+        ids_s, dis_s = ids_h[:ng], dis_h[:ng].astype(np.float64)
+        ok = ids_s >= 0
+        rows = base_h[np.where(ok, ids_s, 0)].astype(np.float64)
+        d64 = ((rows - q64[:, None, :]) ** 2).sum(axis=2)
+        rel = np.abs(dis_s - d64)[ok] / np.maximum(d64[ok], 1e-30)
+        first = d64[:, 0][ok[:, 0]]
+        gap = (first - gt_d[:ng][ok[:, 0]]) / np.maximum(gt_d[:ng][ok[:, 0]], 1e-30)
+        r["dist_rel_err_vs_fp64"] = {"mean": float(f"{rel.mean():.3g}"), "max": float(f"{rel.max():.3g}")}
+        r["top1_dist_gap_vs_fp64_nn"] = {"mean": float(f"{gap.mean():.3g}"), "max": float(f"{gap.max():.3g}")}
+        return r
+
+    rec = recall(ids_h, out_dis.numpy())
     if fast is not None:
-        fast.update(recall(f_ids.numpy()))
+        fast.update(recall(f_ids.numpy(), f_dis.numpy()))
         fast["speedup_vs_plain"] = round(fast["qps"] / qps, 3)
 
     cpu = None

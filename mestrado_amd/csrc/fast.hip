@@ -18,6 +18,10 @@
 // quanta of invd * T and a vector's 8-bit sum within 0.75 m quanta of
 // invd * ADC.  B operands are prepared once per index (fast_prep_kernel).
 //
+// Sums.  The entries are stored as signed bytes t - 128 and summed over
+// subquantizers on the matrix cores (v_mfma_i32_16x16x64_i8 with a one-hot B
+// operand), exact in int32; sum = D + 128 m.
+//
 // Candidate filter: a per-query threshold tau_q >= the kl-th smallest ADC
 // distance over the query's seed lists (seed_tau_kernel).  A vector passes
 // if its sum <= tau_q * invd + 3m/4 + 2, which keeps every vector whose ADC
@@ -33,13 +37,15 @@ namespace {
 
 constexpr int D = 128;
 constexpr int FG = 16;    // queries per work item
-constexpr int FNT = 256;  // threads per workgroup
+constexpr int FNT = 512;  // threads per workgroup
 constexpr int POOL = 1536;
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const u32x4 lds_u32x4;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const i32x4 lds_i32x4;
 
 // K of the table dot products, padded to whole 16-wide MFMA steps
 template <int M>
@@ -60,8 +66,7 @@ struct FLay {
     static_assert(size_t(M) * FG * KP * 2 <= POOL * 8, "A operands must fit in the pool");
     static constexpr size_t delt = pool + POOL * 8;            // float [FG]
     static constexpr size_t pmx = delt + FG * 4;               // float [M]
-    static constexpr size_t sthr = pmx + M * 4;                // uint32 [FG/2] (u16 pairs of limits)
-    static constexpr size_t stmp = sthr + FG / 2 * 4;          // uint32 [FG] unpacked limits
+    static constexpr size_t stmp = pmx + M * 4;                // uint32 [FG] per-slot limits
     static constexpr size_t qid = stmp + FG * 4;               // int [FG] query of each slot (-1 empty)
     static constexpr size_t gcnt = qid + FG * 4;               // int [FG]
     static constexpr size_t gbase = gcnt + FG * 4;             // int [FG]
@@ -72,6 +77,17 @@ struct FLay {
     static constexpr size_t nidx = item + 4;                   // int next item index
     static constexpr size_t total = nidx + 8;
 };
+
+// Workgroup barrier that orders LDS only.  __syncthreads() also fences
+// global memory, which makes every wave wait for its outstanding global
+// loads (here: the code prefetch of later steps) at each barrier.  Global
+// data is never exchanged between the threads of a workgroup at these
+// barriers, so the LDS-scope fence is all they need.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // Move the staged candidates to the per-query global buffers (whole
 // workgroup, uniform call).  Entry: x = list position, y = g | sum << 4,
@@ -88,15 +104,15 @@ __device__ void flush_pool(const FastArgs &a, unsigned char *smem, int64_t beg) 
     const int tid = threadIdx.x;
     const int n = min(*pcnt, POOL);
     if (tid < FG) gcnt[tid] = 0;
-    __syncthreads();
+    lds_barrier();
     for (int i = tid; i < n; i += FNT) {
         const uint32_t y = pool[i].y;
         const int r = atomicAdd(gcnt + (y & 15u), 1);
         pool[i].y = y | (uint32_t(r) << 16);
     }
-    __syncthreads();
+    lds_barrier();
     if (tid < FG && gcnt[tid] > 0) gbase[tid] = atomicAdd(a.cand_cnt + qid[tid], gcnt[tid]);
-    __syncthreads();
+    lds_barrier();
     for (int i = tid; i < n; i += FNT) {
         const uint2 e = pool[i];
         const int g = int(e.y & 15u);
@@ -105,9 +121,9 @@ __device__ void flush_pool(const FastArgs &a, unsigned char *smem, int64_t beg) 
         if (pos < a.capq)
             a.cand[int64_t(qid[g]) * a.capq + pos] = ckey(delt[g] * float(sum), int32_t(beg + e.x));
     }
-    __syncthreads();
+    lds_barrier();
     if (tid == 0) *pcnt = 0;
-    __syncthreads();
+    lds_barrier();
 }
 
 template <int M>
@@ -119,6 +135,14 @@ __device__ inline void load_codes(const uint8_t *lc, int v, uint32_t (&cw)[M / 4
         const uint2 c2 = *reinterpret_cast<const uint2 *>(lc + uint32_t(v) * 8u);
         cw[0] = c2.x; cw[1] = c2.y;
     }
+}
+
+// The M/4 code bytes of subquantizers kb*M/4 .. of vector v (one dword for
+// m = 16, a u16 for m = 8), low byte first.
+template <int M>
+__device__ inline uint32_t load_tcodes(const uint8_t *lc, int v, int kb) {
+    if constexpr (M == 16) return *reinterpret_cast<const uint32_t *>(lc + uint32_t(v) * 16u + uint32_t(kb) * 4u);
+    else return *reinterpret_cast<const uint16_t *>(lc + uint32_t(v) * 8u + uint32_t(kb) * 2u);
 }
 
 // Inputs of one work item's A side: thread (g, s) = (tid / M, tid % M),
@@ -152,7 +176,6 @@ __device__ void build_item_tables(const FastArgs &a, unsigned char *smem, const 
     _Float16 *ao = reinterpret_cast<_Float16 *>(smem + L::ao);
     float *delt = reinterpret_cast<float *>(smem + L::delt);
     const float *pmx = reinterpret_cast<const float *>(smem + L::pmx);
-    uint32_t *sthr = reinterpret_cast<uint32_t *>(smem + L::sthr);
     uint32_t *stmp = reinterpret_cast<uint32_t *>(smem + L::stmp);
     const int *qid = reinterpret_cast<const int *>(smem + L::qid);
     const int tid = threadIdx.x;
@@ -194,13 +217,6 @@ __device__ void build_item_tables(const FastArgs &a, unsigned char *smem, const 
         }
     }
     __syncthreads();
-    if (tid < FG / 2) {
-        // sthr[k] holds the limits of the two queries of accumulator k:
-        // k = 2i + odd pairs queries 4i + odd (low half) and 4i + odd + 2 (high half)
-        const int i = tid >> 1, odd = tid & 1;
-        const int qa = 4 * i + odd, qb = qa + 2;
-        sthr[tid] = stmp[qa] | (stmp[qb] << 16);
-    }
 
     // v_mfma_f32_16x16x16_f16: lane l supplies A[l%16][4(l/16) .. +3] and
     // B[4(l/16) .. +3][l%16] and receives D[4(l/16) + r][l%16] in
@@ -245,7 +261,8 @@ __device__ void build_item_tables(const FastArgs &a, unsigned char *smem, const 
             for (int r = 0; r < 4; ++r)
                 w = __builtin_amdgcn_cvt_pk_u8_f32(__builtin_amdgcn_fmed3f(acc[r], 0.0f, 255.0f), uint32_t(r), w);
             const int c = (wave * TPW + t) * 16 + c16;
-            *reinterpret_cast<uint32_t *>(smem + L::t8 + (size_t(s) * 256 + c) * FG + 4 * h) = w;
+            // stored as signed bytes t - 128, the operand form of the i8 MFMA sums
+            *reinterpret_cast<uint32_t *>(smem + L::t8 + (size_t(s) * 256 + c) * FG + 4 * h) = w ^ 0x80808080u;
         }
     }
     __syncthreads();  // tables ready; the pool may now overwrite the A operands
@@ -280,10 +297,6 @@ __global__ void fast_prep_kernel(const float *__restrict__ pq, const float *__re
     for (int t = DS + 2; t < KP; ++t) row[t] = _Float16(0.0f);
 }
 
-// Accumulators: ae[i] holds the sums of queries 4i (low u16) and 4i+2 (high),
-// ao[i] those of 4i+1 and 4i+3; per query the sum is <= m * 255 < 2^16, so
-// the low half never carries into the high one.
-//
 // Persistent: workgroups take items from a global counter.  While item i is
 // scanned, the next item's index, its record and then its query and centroid
 // values are fetched one stage per scan step, so the table phase of item
@@ -296,8 +309,11 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     float *pmx = reinterpret_cast<float *>(smem + L::pmx);
     const float *delt = reinterpret_cast<const float *>(smem + L::delt);
-    const uint32_t *sthr = reinterpret_cast<const uint32_t *>(smem + L::sthr);
+    const uint32_t *stmp_sh = reinterpret_cast<const uint32_t *>(smem + L::stmp);  // per-slot limits
     int *qid = reinterpret_cast<int *>(smem + L::qid);
+    constexpr int NJ = M / 4;               // subquantizers per lane per tile = MFMAs per tile
+    constexpr int UT = 4;                   // tiles of 16 vectors per wave per step
+    static_assert(UT * 16 * (FNT / 64) == FNT, "a step covers FNT vectors");
     int *pcnt = reinterpret_cast<int *>(smem + L::pcnt);
     int *item_sh = reinterpret_cast<int *>(smem + L::item);
     int *nidx_sh = reinterpret_cast<int *>(smem + L::nidx);
@@ -309,6 +325,13 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
     if (uint32_t(uintptr_t((__attribute__((address_space(3))) unsigned char *)(smem))) != 0) {
         if (tid == 0) atomicOr(a.err, 2);
         return;
+    }
+    // one-hot B operand of the sums: byte (lane & 15) of lane's 16 bytes is 1
+    i32x4 bsel;
+    {
+        const int r = lane & 15;
+        const int w1 = 1 << (8 * (r & 3));
+        bsel = i32x4{(r >> 2) == 0 ? w1 : 0, (r >> 2) == 1 ? w1 : 0, (r >> 2) == 2 ? w1 : 0, (r >> 2) == 3 ? w1 : 0};
     }
     const int total = *a.total_items;
     if (tid == 0) {
@@ -353,42 +376,65 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
             ++stage;
         };
 
-        // ---- scan: each lane one vector per step, codes loaded one step ahead
+        // ---- scan on the int8 matrix cores.  A wave takes tiles of 16
+        //      vectors; lane l = 16 kb + r gathers, for vector r of the tile,
+        //      the 16 query bytes of subquantizers kb*NJ .. kb*NJ + NJ - 1
+        //      (one ds_read_b128 each) and feeds them as the A operand of
+        //      v_mfma_i32_16x16x64_i8 against a one-hot B (byte r of lane
+        //      r's B is 1), so D[vector][query] sums the entries of four
+        //      subquantizers per instruction.  Lane l then holds the sums of
+        //      vectors 4 kb .. 4 kb + 3 of the tile for query slot r.
         const uint8_t *lc = a.codes + beg * M;
-        uint32_t th[FG / 2];
+        const int r16 = lane & 15, kb = lane >> 4;
+        const uint32_t lim = stmp_sh[r16];
+        const uint32_t tb = uint32_t(L::t8) + uint32_t(kb * NJ) * 4096u;
+        const int wv0 = wave * (UT * 16);  // first vector of this wave's tiles in a step
+        // code loads run one step ahead (addresses clamped to the list, so
+        // loads past its end stay in bounds)
+        uint32_t cn[UT];
 #pragma unroll
-        for (int k = 0; k < FG / 2; ++k) th[k] = sthr[k];
-        uint32_t cn[M / 4];
-        load_codes<M>(lc, min(tid, max(len - 1, 0)), cn);
+        for (int u = 0; u < UT; ++u) cn[u] = load_tcodes<M>(lc, min(wv0 + u * 16 + r16, max(len - 1, 0)), kb);
         for (int base = 0; base < len; base += FNT) {
-            const int v = base + tid;
-            uint32_t cw[M / 4];
+            uint32_t cw[UT];
 #pragma unroll
-            for (int i = 0; i < M / 4; ++i) cw[i] = cn[i];
-            if (base + FNT < len) load_codes<M>(lc, min(v + FNT, len - 1), cn);
-            uint32_t ae[4] = {0u, 0u, 0u, 0u}, ao[4] = {0u, 0u, 0u, 0u};
+            for (int u = 0; u < UT; ++u) cw[u] = cn[u];
+            if (base + FNT < len) {
 #pragma unroll
-            for (int s = 0; s < M; ++s) {
-                const uint32_t code = (cw[s >> 2] >> (8 * (s & 3))) & 255u;
-                const u32x4 x = *reinterpret_cast<lds_u32x4 *>(uintptr_t(L::t8 + (uint32_t(s) * 256u + code) * FG));
-                const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
+                for (int u = 0; u < UT; ++u)
+                    cn[u] = load_tcodes<M>(lc, min(base + FNT + wv0 + u * 16 + r16, len - 1), kb);
+            }
+            // gathers in batches of GB subquantizer steps (GB * UT reads in
+            // flight), each followed by its sums
+            constexpr int GB = NJ >= 2 ? 2 : 1;
+            i32x4 acc[UT];
+#pragma unroll
+            for (int u = 0; u < UT; ++u) acc[u] = i32x4{0, 0, 0, 0};
+#pragma unroll
+            for (int j0 = 0; j0 < NJ; j0 += GB) {
+                i32x4 xg[GB][UT];
+#pragma unroll
+                for (int j = 0; j < GB; ++j)
+#pragma unroll
+                    for (int u = 0; u < UT; ++u) {
+                        const uint32_t code = (cw[u] >> (8 * (j0 + j))) & 255u;
+                        xg[j][u] = *reinterpret_cast<lds_i32x4 *>(uintptr_t(tb + uint32_t(j0 + j) * 4096u + code * 16u));
+                    }
+#pragma unroll
+                for (int j = 0; j < GB; ++j)
+#pragma unroll
+                    for (int u = 0; u < UT; ++u)
+                        acc[u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(xg[j][u], bsel, acc[u], 0, 0, 0);
+            }
+            // pass bits: bit 4u + i = vector base + wv0 + 16u + 4kb + i
+            uint32_t pm = 0u;
+            const int vrow = base + wv0 + 4 * kb;
+#pragma unroll
+            for (int u = 0; u < UT; ++u)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    ae[i] += xs[i] & 0x00FF00FFu;
-                    ao[i] += __builtin_amdgcn_perm(0u, xs[i], 0x0C030C01u);
+                    const uint32_t sum = uint32_t(acc[u][i] + 128 * M);
+                    pm |= uint32_t(vrow + 16 * u + i < len && sum < lim) << (4 * u + i);
                 }
-            }
-            // pass mask over the 16 query slots: slot g = 4i + 2hi + odd
-            uint32_t pm = 0u;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t te = th[2 * i], to = th[2 * i + 1];
-                pm |= uint32_t((ae[i] & 0xFFFFu) < (te & 0xFFFFu)) << (4 * i);
-                pm |= uint32_t((ao[i] & 0xFFFFu) < (to & 0xFFFFu)) << (4 * i + 1);
-                pm |= uint32_t((ae[i] >> 16) < (te >> 16)) << (4 * i + 2);
-                pm |= uint32_t((ao[i] >> 16) < (to >> 16)) << (4 * i + 3);
-            }
-            if (v >= len) pm = 0u;
             if (__ballot(pm != 0u) != 0) {  // wave-uniform
                 // wave prefix sum of the per-lane pass counts (<= 16: five bit planes)
                 const uint32_t n = uint32_t(__popc(pm));
@@ -403,14 +449,16 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
                 int slot = 0;
                 if (lane == 0) slot = atomicAdd(pcnt, int(tot));
                 slot = __shfl(slot, 0) + int(pre);
+                const int g = r16;
                 while (pm) {
-                    const int g = __ffs(pm) - 1;
+                    const int bit = __ffs(pm) - 1;
                     pm &= pm - 1u;
-                    // select without dynamic register indexing (g varies per lane)
-                    uint32_t accw = (g & 1) ? ao[0] : ae[0];
+                    // select without dynamic register indexing (bit varies per lane)
+                    int sv = acc[0][0];
 #pragma unroll
-                    for (int i = 1; i < 4; ++i) accw = (g >> 2) == i ? ((g & 1) ? ao[i] : ae[i]) : accw;
-                    const uint32_t sum = (g & 2) ? (accw >> 16) : (accw & 0xFFFFu);
+                    for (int k = 1; k < 4 * UT; ++k) sv = bit == k ? acc[k >> 2][k & 3] : sv;
+                    const uint32_t sum = uint32_t(sv + 128 * M);
+                    const int v = vrow + 16 * (bit >> 2) + (bit & 3);
                     if (slot < POOL) {
                         pool[slot] = make_uint2(uint32_t(v), uint32_t(g) | (sum << 4));
                     } else {  // pool full: straight to the global buffer
@@ -422,9 +470,9 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
                     ++slot;
                 }
             }
-            __syncthreads();  // this step's appends are done
+            lds_barrier();  // this step's appends are done
             const int filled = *pcnt;
-            __syncthreads();  // every thread has read the count before the next appends
+            lds_barrier();  // every thread has read the count before the next appends
             if (filled > POOL - FNT) flush_pool<M>(a, smem, beg);  // uniform
             if (stage < 3) advance();
         }
@@ -462,7 +510,11 @@ __global__ __launch_bounds__(FNT) void fast_tables_kernel(FastArgs a, int l, int
     if (tid < FG * M) in.load(a, l, rec->q[tid / M], tid % M);
     build_item_tables<M>(a, smem, in, rec->tau);
     const uint4 *src = reinterpret_cast<const uint4 *>(smem + L::t8);
-    for (int i = tid; i < M * 256; i += FNT) reinterpret_cast<uint4 *>(t8)[i] = src[i];
+    for (int i = tid; i < M * 256; i += FNT) {
+        uint4 x = src[i];
+        x.x ^= 0x80808080u; x.y ^= 0x80808080u; x.z ^= 0x80808080u; x.w ^= 0x80808080u;
+        reinterpret_cast<uint4 *>(t8)[i] = x;
+    }
     if (tid < FG) delta[tid] = reinterpret_cast<const float *>(smem + L::delt)[tid];
 }
 

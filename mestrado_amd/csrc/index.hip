@@ -302,6 +302,11 @@ int search_core(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, cons
     a.ngroups = ngroups;
     a.resid_mode = d_probes_in ? 1 : 0;
     a.err = ix->errflag.as<int>();
+    if (!d_probes_in && ix->query_order && nq >= 2048 && query_order_ok(nlist)) {
+        MIVF_TRY(ws.qorder.ensure(sizeof(int32_t) * size_t(nq)));
+        MIVF_CHECK_HIP(launch_query_order(probes, wmax, nq, nlist, ws.qorder.as<int32_t>(), st));
+        a.qorder = ws.qorder.as<int32_t>();
+    }
     if (ngroups > 1) {
         MIVF_TRY(ws.partial.ensure(sizeof(uint64_t) * size_t(nq) * ngroups * kstride));
         a.partial = ws.partial.as<uint64_t>();
@@ -942,6 +947,8 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) {
     if (n == "probes_per_group") {
         MIVF_REQUIRE(value >= 0 && value <= 1024, kInvalid, "probes_per_group must be in 0..1024");
         ix->ppg_opt = int(value);
+    } else if (n == "query_order") {
+        ix->query_order = value != 0;
     } else if (n == "profile") {
         ix->profile = value != 0;
     } else if (n == "fast_seed_lists") {

@@ -65,7 +65,7 @@ struct Timer {
 // Scratch of one search in flight (grow-only).  The index owns one; each
 // serving slot owns another, so searches on different streams can overlap.
 struct Ws {
-    DevBuf qn, dmat, probes, coadis, partial, wq, kq, redo, tmp_ids, tmp_dis;
+    DevBuf qn, dmat, probes, coadis, partial, wq, kq, redo, tmp_ids, tmp_dis, qorder;
 };
 
 }  // namespace mivf
@@ -100,6 +100,7 @@ struct mivf_index {
     bool scan_checked = false;
     // options
     int ppg_opt = 0;
+    int query_order = 1;      // scan queries grouped by nearest list, per XCD (scheduling only)
     int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
     int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
     int batched_cap = 4096;   // candidate capacity per query of the batched exact form (>= 32 kmax)

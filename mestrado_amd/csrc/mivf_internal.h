@@ -141,11 +141,17 @@ struct ScanArgs {
     int ngroups;
     int resid_mode;           // 1: queries holds residuals, row = q*wstride + p
     int *err;                 // set non-zero by the kernel on a violated precondition
+    const int32_t *qorder;    // [nq] scan order of the queries (XCD-blocked), or nullptr
     uint64_t *partial;        // [nq][ngroups][kstride] when ngroups > 1
     int32_t *out_ids;         // [nq][kstride] when ngroups == 1
     float *out_dis;
 };
 hipError_t launch_scan(const ScanArgs &a, int m, hipStream_t s);
+// Queries ordered by their first probed list (counting sort; the order
+// inside a list is arbitrary).  Only the schedule of the scan depends on it.
+bool query_order_ok(int nlist);
+hipError_t launch_query_order(const int32_t *probes, int wstride, int nq, int nlist, int32_t *order,
+                              hipStream_t s);
 // Merge per-group partial keys to the final sorted top-k.
 hipError_t launch_merge(const uint64_t *partial, int nq, int ngroups, int kstride,
                         const int32_t *kq, int k, int32_t *out_ids, float *out_dis,

@@ -214,8 +214,18 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
         if (tid == 0) atomicOr(a.err, 1);
         return;
     }
-    const int q = blockIdx.x / a.ngroups;
-    const int g = blockIdx.x - q * a.ngroups;
+    // Workgroups go to the XCDs round robin (block b to XCD b mod 8).  With
+    // a query order, XCD x takes a contiguous run of it, so the workgroups
+    // resident on one XCD scan neighbouring queries, whose probed lists
+    // overlap and stay in that XCD's L2.
+    int b = blockIdx.x;
+    if (a.qorder) {
+        const int per = int(gridDim.x) >> 3;
+        if (b < per * 8) b = (b & 7) * per + (b >> 3);
+    }
+    const int qs = b / a.ngroups;
+    const int g = b - qs * a.ngroups;
+    const int q = a.qorder ? a.qorder[qs] : qs;
     const int wq = a.wq ? a.wq[q] : a.w;
     const int kq = a.kq ? a.kq[q] : a.k;
     const int p0 = g * a.ppg;
@@ -416,6 +426,44 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
     }
 }
 
+constexpr int QO_NT = 1024;
+constexpr int QO_MAXL = 16384;  // LDS histogram bins
+
+__global__ __launch_bounds__(QO_NT) void query_order_kernel(const int32_t *__restrict__ probes, int wstride,
+                                                            int nq, int nlist, int32_t *__restrict__ order) {
+    extern __shared__ int hist[];  // [nlist]
+    __shared__ int wsum[QO_NT / 64];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < nlist; i += QO_NT) hist[i] = 0;
+    __syncthreads();
+    auto bucket = [&](int q) { return min(max(probes[int64_t(q) * wstride], 0), nlist - 1); };
+    for (int q = tid; q < nq; q += QO_NT) atomicAdd(&hist[bucket(q)], 1);
+    __syncthreads();
+    // exclusive scan: thread t owns bins [t*per, (t+1)*per)
+    const int per = (nlist + QO_NT - 1) / QO_NT;
+    const int b0 = min(tid * per, nlist), b1 = min(b0 + per, nlist);
+    int loc = 0;
+    for (int i = b0; i < b1; ++i) loc += hist[i];
+    int incl = loc;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if ((tid & 63) >= off) incl += o;
+    }
+    if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < (tid >> 6); ++w) base += wsum[w];
+    int run = base + incl - loc;
+    for (int i = b0; i < b1; ++i) {
+        const int c = hist[i];
+        hist[i] = run;
+        run += c;
+    }
+    __syncthreads();
+    for (int q = tid; q < nq; q += QO_NT) order[atomicAdd(&hist[bucket(q)], 1)] = q;
+}
+
 template <int M, int KM>
 hipError_t launch_mk(const ScanArgs &a, hipStream_t s) {
     constexpr int CAP = KM <= 128 ? 2048 : 4096;
@@ -438,6 +486,23 @@ hipError_t launch_m(const ScanArgs &a, hipStream_t s) {
 }
 
 }  // namespace
+
+bool query_order_ok(int nlist) { return nlist >= 1 && nlist <= QO_MAXL; }
+
+hipError_t launch_query_order(const int32_t *probes, int wstride, int nq, int nlist, int32_t *order,
+                              hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    if (!query_order_ok(nlist)) return hipErrorInvalidValue;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&query_order_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(QO_MAXL * sizeof(int)));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    query_order_kernel<<<1, QO_NT, size_t(nlist) * sizeof(int), s>>>(probes, wstride, nq, nlist, order);
+    return hipGetLastError();
+}
 
 hipError_t launch_scan(const ScanArgs &a, int m, hipStream_t s) {
     if (a.nq <= 0) return hipSuccess;

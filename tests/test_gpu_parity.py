@@ -225,6 +225,30 @@ def test_search_m16_zipf(orc, zipf16, w, k, ppg):
     assert_same_results(ids, dis, rids, rdis, f"m16 zipf w={w} k={k} ppg={ppg}")
 
 
+@pytest.mark.parametrize("w,k,ppg", [(32, 100, 0), (16, 10, 4)])
+def test_search_query_order(orc, zipf16, w, k, ppg):
+    """Batches of >= 2048 queries are scanned grouped by nearest list and
+    blocked per XCD; only the schedule changes, so the results equal the
+    oracle's and the unordered scan's bit for bit."""
+    ix, index, _, _ = zipf16
+    from mestrado_amd.workloads import NCOMP
+    seed = 103
+    cent = orc.gen_centres(seed, NCOMP, 128)
+    thr = orc.zipf_thresholds(seed, NCOMP, 1.1)
+    query = orc.gen_rows(seed, STREAM_QUERY, 1000, 2500, cent, thr, SIGMA)  # 2500 queries: ordering on
+    ix.set_option("probes_per_group", ppg)
+    try:
+        ids, dis = ix.search(query, w, k)
+        ix.set_option("query_order", 0)
+        uids, udis = ix.search(query, w, k)
+    finally:
+        ix.set_option("query_order", 1)
+        ix.set_option("probes_per_group", 0)
+    assert np.array_equal(ids, uids) and np.array_equal(dis.view(np.uint32), udis.view(np.uint32))
+    rids, rdis = orc.search(index, query, w, k)
+    assert_same_results(ids, dis, rids, rdis, f"ordered scan w={w} k={k} ppg={ppg}")
+
+
 def test_search_m16_rerank_and_var(orc, zipf16, mivf):
     ix, index, query, _ = zipf16
     rng = np.random.default_rng(21)
