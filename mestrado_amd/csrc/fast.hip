@@ -39,6 +39,7 @@ constexpr int D = 128;
 constexpr int FG = 16;    // queries per work item
 constexpr int FNT = 512;  // threads per workgroup
 constexpr int POOL = 1536;
+constexpr int NBH = 32;   // bins of the running-threshold histograms
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const u32x4 lds_u32x4;
@@ -70,10 +71,12 @@ struct FLay {
     static constexpr size_t qid = stmp + FG * 4;               // int [FG] query of each slot (-1 empty)
     static constexpr size_t gcnt = qid + FG * 4;               // int [FG]
     static constexpr size_t gbase = gcnt + FG * 4;             // int [FG]
-    static constexpr size_t rec = gbase + FG * 4;              // FastItem: the current item
+    static constexpr size_t hsc = gbase + FG * 4;              // float [FG] histogram scale NBH / tau (0: none)
+    static constexpr size_t lhist = hsc + FG * 4;              // int [FG][NBH] flush histogram
+    static constexpr size_t rec = lhist + FG * NBH * 4;        // FastItem: the current item
     static constexpr size_t nrec = rec + sizeof(FastItem);     // FastItem: the next item
-    static constexpr size_t pcnt = nrec + sizeof(FastItem);    // int
-    static constexpr size_t item = pcnt + 4;                   // int current item index
+    static constexpr size_t pcnt = nrec + sizeof(FastItem);    // int [4] rotating append counters
+    static constexpr size_t item = pcnt + 16;                  // int current item index
     static constexpr size_t nidx = item + 4;                   // int next item index
     static constexpr size_t total = nidx + 8;
 };
@@ -89,21 +92,23 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Move the staged candidates to the per-query global buffers (whole
+// Move the filled staged candidates to the per-query global buffers (whole
 // workgroup, uniform call).  Entry: x = list position, y = g | sum << 4,
 // and the rank within its query slot goes into bits 16..27 of y.
 template <int M>
-__device__ void flush_pool(const FastArgs &a, unsigned char *smem, int64_t beg) {
+__device__ void flush_pool(const FastArgs &a, unsigned char *smem, int64_t beg, int filled) {
     using L = FLay<M>;
     uint2 *pool = reinterpret_cast<uint2 *>(smem + L::pool);
     int *gcnt = reinterpret_cast<int *>(smem + L::gcnt);
     int *gbase = reinterpret_cast<int *>(smem + L::gbase);
-    int *pcnt = reinterpret_cast<int *>(smem + L::pcnt);
     const int *qid = reinterpret_cast<const int *>(smem + L::qid);
     const float *delt = reinterpret_cast<const float *>(smem + L::delt);
+    const float *hsc = reinterpret_cast<const float *>(smem + L::hsc);
+    int *lhist = reinterpret_cast<int *>(smem + L::lhist);
     const int tid = threadIdx.x;
-    const int n = min(*pcnt, POOL);
+    const int n = min(filled, POOL);
     if (tid < FG) gcnt[tid] = 0;
+    for (int i = tid; i < FG * NBH; i += FNT) lhist[i] = 0;
     lds_barrier();
     for (int i = tid; i < n; i += FNT) {
         const uint32_t y = pool[i].y;
@@ -118,12 +123,43 @@ __device__ void flush_pool(const FastArgs &a, unsigned char *smem, int64_t beg) 
         const int g = int(e.y & 15u);
         const uint32_t sum = (e.y >> 4) & 0xFFFu;
         const int pos = gbase[g] + int(e.y >> 16);
-        if (pos < a.capq)
-            a.cand[int64_t(qid[g]) * a.capq + pos] = ckey(delt[g] * float(sum), int32_t(beg + e.x));
+        const float ad = delt[g] * float(sum);
+        if (pos < a.capq) {
+            a.cand[int64_t(qid[g]) * a.capq + pos] = ckey(ad, int32_t(beg + e.x));
+            if (hsc[g] > 0.0f) atomicAdd(lhist + g * NBH + min(NBH - 1, int(ad * hsc[g])), 1);
+        }
     }
-    lds_barrier();
-    if (tid == 0) *pcnt = 0;
-    lds_barrier();
+    lds_barrier();  // the pool is free again
+    if (a.runthr) {
+        // Running bound per query: add this flush's counts to the query's
+        // global histogram of candidate distances (bin b holds distances in
+        // [b, b + 1) tau / NBH), then the first bin whose cumulative count
+        // reaches kl bounds the kl-th smallest distance emitted so far.
+        // Counts read back may miss concurrent additions; a smaller count only
+        // gives a larger (still valid) bound.
+        for (int i = tid; i < FG * NBH; i += FNT) {
+            const int c = lhist[i];
+            if (c > 0) atomicAdd(a.rhist + int64_t(qid[i / NBH]) * NBH + (i % NBH), c);
+        }
+        const int lane = tid & 63;
+        for (int g = tid >> 6; g < FG; g += FNT / 64) {  // one wavefront per slot
+            if (!(hsc[g] > 0.0f) || gcnt[g] == 0) continue;  // wave-uniform
+            const int q = qid[g];
+            int c = lane < NBH - 1 ? __hip_atomic_load(a.rhist + int64_t(q) * NBH + lane, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT) : 0;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int o = __shfl_up(c, off, 64);
+                if (lane >= off) c += o;
+            }
+            const uint64_t reach = __ballot(lane < NBH - 1 && c >= a.kl);
+            if (reach != 0 && lane == 0) {
+                const int b = __ffsll((unsigned long long)reach) - 1;
+                const float R = (float(b + 1) + 0.01f) / hsc[g];
+                atomicMin(reinterpret_cast<int *>(a.runthr) + q, __float_as_int(R));
+            }
+        }
+    }
 }
 
 template <int M>
@@ -177,6 +213,7 @@ __device__ void build_item_tables(const FastArgs &a, unsigned char *smem, const 
     float *delt = reinterpret_cast<float *>(smem + L::delt);
     const float *pmx = reinterpret_cast<const float *>(smem + L::pmx);
     uint32_t *stmp = reinterpret_cast<uint32_t *>(smem + L::stmp);
+    float *hsc = reinterpret_cast<float *>(smem + L::hsc);
     const int *qid = reinterpret_cast<const int *>(smem + L::qid);
     const int tid = threadIdx.x;
 
@@ -208,12 +245,21 @@ __device__ void build_item_tables(const FastArgs &a, unsigned char *smem, const 
             const float dlt = bound / 255.0f;
             delt[g] = dlt;
             const float tau = taus[g];
+            const bool seeded = tau < 3.0e38f;
             // stored limit = thr + 1 (a sum passes iff sum < limit)
             uint32_t lim;
             if (qid[g] < 0) lim = 0u;                     // empty slot: nothing passes
-            else if (!(tau < 3.0e38f)) lim = 0xFFFFu;     // no seed bound: keep everything
+            else if (!seeded) lim = 0xFFFFu;              // no seed bound: keep everything
             else lim = uint32_t(fminf(tau * invd + float(3 * M / 4 + 2), 65534.0f)) + 1u;
+            if (qid[g] >= 0 && a.runthr) {
+                // running bound R: >= kl candidates of the query already have an
+                // approximate distance <= R, so a vector above R cannot enter its
+                // shortlist.  A sum passes if dlt * sum <= R (with rounding room).
+                const float R = a.runthr[qid[g]];
+                if (R < 3.0e38f) lim = min(lim, uint32_t(fminf(R * invd * 1.0002f + 1.0f, 65534.0f)) + 1u);
+            }
             stmp[g] = lim;
+            hsc[g] = (qid[g] >= 0 && seeded && tau > 0.0f && a.runthr) ? float(NBH) / tau : 0.0f;
         }
     }
     __syncthreads();
@@ -302,7 +348,7 @@ __global__ void fast_prep_kernel(const float *__restrict__ pq, const float *__re
 // values are fetched one stage per scan step, so the table phase of item
 // i + 1 starts with its inputs in registers.
 template <int M>
-__global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
+__global__ __launch_bounds__(FNT, 4) void fast_scan_kernel(FastArgs a) {  // 4 waves per SIMD: two workgroups per CU
     using L = FLay<M>;
     constexpr int RW = int(sizeof(FastItem) / 4);  // record words
     static_assert(RW <= 64, "a record is loaded by one wavefront");
@@ -334,10 +380,8 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
         bsel = i32x4{(r >> 2) == 0 ? w1 : 0, (r >> 2) == 1 ? w1 : 0, (r >> 2) == 2 ? w1 : 0, (r >> 2) == 3 ? w1 : 0};
     }
     const int total = *a.total_items;
-    if (tid == 0) {
-        *pcnt = 0;
-        *item_sh = atomicAdd(a.next_item, 1);
-    }
+    if (tid < 4) pcnt[tid] = 0;
+    if (tid == 0) *item_sh = atomicAdd(a.next_item, 1);
     if (tid < M) pmx[tid] = a.pmax[tid];
     __syncthreads();
     if (*item_sh < total && tid < RW)
@@ -352,6 +396,7 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
         const int64_t beg = rec->beg;
         const int len = rec->len;
         if (tid < FG) qid[tid] = rec->q[tid];
+        if (tid < 3) pcnt[tid] = 0;
         __syncthreads();
         build_item_tables<M>(a, smem, in, rec->tau);
 
@@ -394,7 +439,10 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
         uint32_t cn[UT];
 #pragma unroll
         for (int u = 0; u < UT; ++u) cn[u] = load_tcodes<M>(lc, min(wv0 + u * 16 + r16, max(len - 1, 0)), kb);
+        int fill = 0;  // staged candidates before this step (uniform)
+        int rot = 0;   // append counter of this step
         for (int base = 0; base < len; base += FNT) {
+            if (tid == 0) pcnt[rot == 2 ? 0 : rot + 1] = 0;
             uint32_t cw[UT];
 #pragma unroll
             for (int u = 0; u < UT; ++u) cw[u] = cn[u];
@@ -447,8 +495,8 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
                     tot += uint32_t(__popcll(bm)) << b;
                 }
                 int slot = 0;
-                if (lane == 0) slot = atomicAdd(pcnt, int(tot));
-                slot = __shfl(slot, 0) + int(pre);
+                if (lane == 0) slot = atomicAdd(pcnt + rot, int(tot));
+                slot = fill + __shfl(slot, 0) + int(pre);
                 const int g = r16;
                 while (pm) {
                     const int bit = __ffs(pm) - 1;
@@ -470,18 +518,27 @@ __global__ __launch_bounds__(FNT) void fast_scan_kernel(FastArgs a) {
                     ++slot;
                 }
             }
+            // Three append counters rotate over the steps: counter rot takes
+            // this step's appends, counter rot + 1 (cleared at this step's
+            // start) the next step's, and counter rot + 2, read by every
+            // thread before this step's barrier, is free.  One barrier per
+            // step then orders the appends before the reads.
             lds_barrier();  // this step's appends are done
-            const int filled = *pcnt;
-            lds_barrier();  // every thread has read the count before the next appends
-            if (filled > POOL - FNT) flush_pool<M>(a, smem, beg);  // uniform
+            fill += pcnt[rot];
+            rot = rot == 2 ? 0 : rot + 1;
+            if (fill > POOL - FNT) {  // uniform
+                flush_pool<M>(a, smem, beg, fill);
+                fill = 0;
+            }
             if (stage < 3) advance();
         }
-        // the loop's last barrier pair ordered every append before this read,
-        // and no thread appends again before the next item's barriers
-        if (*pcnt > 0) flush_pool<M>(a, smem, beg);
+        if (fill > 0) flush_pool<M>(a, smem, beg, fill);
         while (stage < 3) {
-            advance();
+            // the previous stage (possibly the last step's, which no barrier
+            // follows when nothing is left to flush) wrote nrec / nidx, which
+            // this stage reads
             __syncthreads();
+            advance();
         }
         __syncthreads();  // rec, qid and the pool are free; nrec is complete
         if (tid < RW) reinterpret_cast<uint32_t *>(rec)[tid] = reinterpret_cast<const uint32_t *>(nrec)[tid];

@@ -801,6 +801,11 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(ix->f_ccnt.ensure(sizeof(int) * nq));
     MIVF_CHECK_HIP(launch_fill_i32(ix->f_ccnt.as<int>(), nq, 0, st));
     MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, st));
+    // running bounds (+inf) and their histograms (0)
+    MIVF_TRY(ix->f_rthr.ensure(sizeof(float) * nq));
+    MIVF_TRY(ix->f_rhist.ensure(sizeof(int) * size_t(nq) * 32));
+    MIVF_CHECK_HIP(launch_fill_i32(ix->f_rthr.as<int>(), nq, 0x7f800000, st));
+    MIVF_CHECK_HIP(launch_fill_i32(ix->f_rhist.as<int>(), nq * 32, 0, st));
     MIVF_TRY(tm.mark());
     // 4. persistent list-major scan
     {
@@ -818,6 +823,9 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         a.capq = capq;
         a.ks = ix->ks;
         a.err = ix->errflag.as<int>();
+        a.runthr = ix->fast_running ? ix->f_rthr.as<float>() : nullptr;
+        a.rhist = ix->fast_running ? ix->f_rhist.as<int>() : nullptr;
+        a.kl = kl;
         int dev = 0, ncu = 256;
         MIVF_CHECK_HIP(hipGetDevice(&dev));
         MIVF_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -947,6 +955,8 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) {
     if (n == "probes_per_group") {
         MIVF_REQUIRE(value >= 0 && value <= 1024, kInvalid, "probes_per_group must be in 0..1024");
         ix->ppg_opt = int(value);
+    } else if (n == "fast_running_bound") {
+        ix->fast_running = value != 0;
     } else if (n == "query_order") {
         ix->query_order = value != 0;
     } else if (n == "profile") {

@@ -94,13 +94,14 @@ struct mivf_index {
     bool rows_ok = false;
     // batched reduced-precision form scratch
     DevBuf f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_items, f_ctrl, f_cand, f_ccnt,
-        f_short, f_ns;
+        f_short, f_ns, f_rthr, f_rhist;
     int64_t f_last_cands = 0, f_last_overflow = 0;
     DevBuf errflag;          // kernel-reported precondition failures (int)
     bool scan_checked = false;
     // options
     int ppg_opt = 0;
     int query_order = 1;      // scan queries grouped by nearest list, per XCD (scheduling only)
+    bool fast_running = true;  // running per-query bound in the reduced-precision scan
     int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
     int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
     int batched_cap = 4096;   // candidate capacity per query of the batched exact form (>= 32 kmax)

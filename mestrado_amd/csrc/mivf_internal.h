@@ -181,6 +181,9 @@ struct FastArgs {
     int capq;
     int ks;
     int *err;
+    float *runthr;              // [nq] running bound on the kl-th candidate distance (+inf), or nullptr
+    int *rhist;                 // [nq][32] candidate-distance histograms (zeroed), with runthr
+    int kl;
 };
 int fast_group_size();
 size_t fast_pb_bytes(int m);

@@ -270,3 +270,28 @@ def test_fast_select_picks_the_kl_smallest_keys(mivf, orc, case):
         assert (got[m:] == -1).all(), f"{case} q={q}: padding"
         assert np.array_equal(np.sort(got[:m].astype(np.int64)), np.sort(want.astype(np.int64))), \
             f"{case} q={q} n={n}: not the kl smallest keys"
+
+
+def test_fast_running_bound_changes_nothing(mivf, zipf16):
+    """The running per-query bound (shared by the workgroups through
+    candidate-distance histograms) only drops vectors that cannot enter the
+    shortlist: results equal those without it, run after run, although the
+    number of candidates varies with the schedule."""
+    ix, index, query, base = zipf16
+    ix.set_option("fast_capacity", 1 << 16)
+    ix.set_option("profile", 1)
+    try:
+        ix.set_option("fast_running_bound", 0)
+        ref_ids, ref_dis = ix.search_fast(query, 32, 256, 100, base)
+        ref_c = ix.fast_stats()[0]
+        ix.set_option("fast_running_bound", 1)
+        cands = []
+        for _ in range(4):
+            ids, dis = ix.search_fast(query, 32, 256, 100, base)
+            cands.append(ix.fast_stats()[0])
+            assert np.array_equal(ids, ref_ids) and np.array_equal(dis.view(np.uint32), ref_dis.view(np.uint32))
+    finally:
+        ix.set_option("fast_running_bound", 1)
+        ix.set_option("fast_capacity", 16384)
+        ix.set_option("profile", 0)
+    assert max(cands) <= ref_c
