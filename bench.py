@@ -194,6 +194,14 @@ def main():
     qps = nq * args.steps / elapsed
     scan_ms = float(stage[2])
     achieved = scan_bytes / (scan_ms * 1e-3) / 1e9 if scan_ms > 0 else None
+    # HBM bytes per launch of the scan kernel from the FETCH_SIZE / WRITE_SIZE
+    # counter runs recorded for this workload (null when none is recorded)
+    traffic = None
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"counters_{args.workload}.md")
+    if os.path.exists(prof) and not args.nbase:
+        for line in open(prof):
+            if line.startswith("traffic_bytes_per_launch scan_kernel:"):
+                traffic = int(line.split(":")[1])
 
     # recall against the exact FP64 nearest neighbour
     ids_h = out_ids.numpy()
@@ -260,7 +268,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None,
+                     "traffic": traffic,
                      "kernel": "scan_kernel", "algorithmic_bytes_per_launch": scan_bytes},
         "cpu_baseline": cpu,
         "fast_form": fast,
