@@ -176,8 +176,13 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
 
 /* ---- options and timing ---------------------------------------------------
  * "probes_per_group" (0 = automatic), "profile" (1 = record per-stage HIP
- * event timings of the next searches), "fast_seed_lists" (lists per query
- * scanned exactly to seed mivf_search_fast, default 1), "fast_capacity".  */
+ * event timings of the next searches), "query_order" (1 = scan batches of
+ * >= 2048 queries grouped by nearest list; schedule only, default 1),
+ * "fast_seed_lists" (lists per query scanned exactly to seed
+ * mivf_search_fast, default 1), "fast_capacity", "fast_running_bound" (1 =
+ * the 8-bit scan tightens each query's filter as its candidates accumulate;
+ * results unchanged, default 1), "batched_group", "batched_threads",
+ * "batched_capacity" (mivf_search_batched).                                 */
 int mivf_set_option(mivf_index *ix, const char *name, int64_t value);
 /* Per-stage milliseconds of the last profiled search:
  * [0] coarse distances, [1] probe select, [2] scan, [3] merge (mivf_search);
