@@ -165,9 +165,43 @@ hipError_t launch_norm_f32(const float *x, int64_t n, int d, int ldx, float *out
     return hipGetLastError();
 }
 
+// Same sums, rows staged through LDS: 64 rows per workgroup, 32 dimensions
+// per chunk loaded coalesced by all 256 threads, then each of 64 threads
+// continues its row's sequential f64 sum over the chunk (t ascending).
+constexpr int NR = 64, NCH = 32;
+__global__ __launch_bounds__(256) void norm_f64_rows_kernel(const float *__restrict__ x, int64_t n, int d, int ldx,
+                                                            double *__restrict__ out) {
+    __shared__ float tile[NR][NCH + 1];
+    const int tid = threadIdx.x;
+    const int64_t r0 = int64_t(blockIdx.x) * NR;
+    double s = 0.0;
+    for (int t0 = 0; t0 < d; t0 += NCH) {
+        for (int e = tid; e < NR * NCH; e += 256) {
+            const int rr = e / NCH, tt = e % NCH;
+            const int64_t row = r0 + rr;
+            tile[rr][tt] = (row < n && t0 + tt < d) ? x[row * ldx + t0 + tt] : 0.0f;
+        }
+        __syncthreads();
+        if (tid < NR) {
+            const int nt = min(NCH, d - t0);
+            for (int tt = 0; tt < nt; ++tt) {
+                const float v = tile[tid][tt];
+                const float p = v * v;
+                s = s + double(p);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < NR && r0 + tid < n) out[r0 + tid] = s;
+}
+
 hipError_t launch_norm_f64(const float *x, int64_t n, int d, int ldx, double *out, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    norm_f64_kernel<<<nblk(n, 256), 256, 0, s>>>(x, n, d, ldx, out);
+    if (n < (int64_t(1) << 20)) {  // small batches (queries): coalesced through LDS
+        norm_f64_rows_kernel<<<nblk(n, NR), 256, 0, s>>>(x, n, d, ldx, out);
+    } else {
+        norm_f64_kernel<<<nblk(n, 256), 256, 0, s>>>(x, n, d, ldx, out);
+    }
     return hipGetLastError();
 }
 

@@ -38,8 +38,8 @@ namespace {
 constexpr int D = 128;
 constexpr int FG = 16;    // queries per work item
 constexpr int FNT = 512;  // threads per workgroup
-constexpr int POOL = 1536;
-constexpr int NBH = 32;   // bins of the running-threshold histograms
+constexpr int POOL = 1280;
+constexpr int NBH = 64;   // bins of the running-threshold histograms
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const u32x4 lds_u32x4;
@@ -244,7 +244,7 @@ __device__ void build_item_tables(const FastArgs &a, unsigned char *smem, const 
         if (s == 0) {
             const float dlt = bound / 255.0f;
             delt[g] = dlt;
-            const float tau = taus[g];
+            const float tau = taus ? taus[g] : (qid[g] >= 0 ? a.tau[qid[g]] : -1.0f);
             const bool seeded = tau < 3.0e38f;
             // stored limit = thr + 1 (a sum passes iff sum < limit)
             uint32_t lim;
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(FNT, 4) void fast_scan_kernel(FastArgs a) {  // 4 w
         if (tid < FG) qid[tid] = rec->q[tid];
         if (tid < 3) pcnt[tid] = 0;
         __syncthreads();
-        build_item_tables<M>(a, smem, in, rec->tau);
+        build_item_tables<M>(a, smem, in, a.tau ? nullptr : rec->tau);
 
         // prefetch of the next item: 0 index wanted, 1 record in registers,
         // 2 record in LDS, 3 inputs in registers (stage is uniform)
@@ -719,7 +719,7 @@ __global__ void item_expand_kernel(const int *__restrict__ pair_off, const int *
     FastItem &x = items[it];
     const int q = slot < gc ? pair_q[pb + slot] : -1;
     x.q[slot] = q;
-    x.tau[slot] = q >= 0 ? tau[q] : -1.0f;
+    x.tau[slot] = q >= 0 ? (tau ? tau[q] : __builtin_huge_valf()) : -1.0f;
     if (slot == 0) {
         x.l = l;
         x.gc = gc;
@@ -838,7 +838,8 @@ __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__
                                                        const uint8_t *__restrict__ codes,
                                                        const int32_t *__restrict__ probes, int w,
                                                        const int32_t *__restrict__ nseed, int nlist, int ks, int kl0,
-                                                       const int32_t *__restrict__ klq, float *__restrict__ tau) {
+                                                       const int32_t *__restrict__ klq, int cap,
+                                                       float *__restrict__ tau) {
     __shared__ float tab[M * 256];
     __shared__ float r[D];
     __shared__ float As[M];
@@ -849,11 +850,16 @@ __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__
     const int tid = threadIdx.x;
     const int ns = nseed[q];
     const int kl = klq ? klq[q] : kl0;
+    // with a cap, only the first cap vectors (list prefixes in probe order, a
+    // sample of the lists) enter: the kl-th smallest of a subset of the
+    // probed vectors still bounds the kl-th smallest of all of them
+    const int64_t lim = cap > 0 ? int64_t(cap) : (int64_t(1) << 62);
     int64_t tot = 0;
     for (int rr = 0; rr < ns; ++rr) {
         const int l = probes[q * w + rr];
         if (l >= 0 && l < nlist) tot += list_off[l + 1] - list_off[l];
     }
+    tot = min(tot, lim);
     if (tot < kl) {  // uniform
         if (tid == 0) tau[q] = __builtin_huge_valf();
         return;
@@ -865,9 +871,10 @@ __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__
         for (int rr = 0; rr < ns; ++rr) {
             const int l = probes[q * w + rr];
             if (l < 0 || l >= nlist) continue;  // uniform
+            if (off >= tot) break;  // uniform: the cap is reached
             marg = fmaxf(marg, seed_table<M>(queries, coarse, pq, pnorm, pmax, q, l, ks, tab, r, As));
             const int64_t beg = list_off[l];
-            const int len = int(list_off[l + 1] - beg);
+            const int len = int(min(list_off[l + 1] - beg, tot - off));
             const uint8_t *lc = codes + beg * M;
             // four vectors per thread per round: their code loads are issued together
             for (int v0 = tid; v0 < len; v0 += 1024) {
@@ -1096,6 +1103,7 @@ hipError_t launch_tables_m(const FastArgs &a, int l, int n, uint8_t *t8, float *
 
 }  // namespace
 
+int fast_hist_bins() { return NBH; }
 int fast_group_size() { return FG; }
 
 size_t fast_pb_bytes(int m) {
@@ -1150,14 +1158,15 @@ hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *ran
 hipError_t launch_seed_tau(const SeedArgs &a, int m, hipStream_t s) {
     if (a.nq <= 0) return hipSuccess;
     if (a.kl < 1) return hipErrorInvalidValue;
+    if (a.cap > 0 && (a.klq || a.cap < a.kl || a.cap > SCAP)) return hipErrorInvalidValue;
     switch (m) {
         case 8:
             seed_tau_kernel<8><<<a.nq, 256, 0, s>>>(a.queries, a.coarse, a.pq, a.pnorm, a.pmax, a.list_off, a.codes,
-                                                     a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.klq, a.tau);
+                                                     a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.klq, a.cap, a.tau);
             break;
         case 16:
             seed_tau_kernel<16><<<a.nq, 256, 0, s>>>(a.queries, a.coarse, a.pq, a.pnorm, a.pmax, a.list_off, a.codes,
-                                                      a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.klq, a.tau);
+                                                      a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.klq, a.cap, a.tau);
             break;
         default: return hipErrorInvalidValue;
     }

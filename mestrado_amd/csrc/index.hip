@@ -758,10 +758,42 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(ix->ws.coadis.ensure(sizeof(float) * size_t(nq) * w));
     MIVF_TRY(coarse_stage(ix, ix->ws, d_q, nq, w, ix->ws.probes.as<int32_t>(), ix->ws.coadis.as<float>(), st, nullptr));
     MIVF_TRY(tm.mark());
-    // 2. seed threshold from each query's ns[q] nearest lists (at least
-    // fast_seed, then as many as it takes to hold kl vectors)
+    // 2. (list, query) grouping and the buffer clears on the second stream,
+    //    overlapped with 3. the seed thresholds on the caller's stream
+    if (!ix->aux) {
+        MIVF_CHECK_HIP(hipStreamCreateWithFlags(&ix->aux, hipStreamNonBlocking));
+        for (auto &e : ix->aux_ev) MIVF_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    const int64_t np = int64_t(nq) * w;
+    MIVF_TRY(ix->f_cnt.ensure(sizeof(int) * (nlist + 1)));
+    MIVF_TRY(ix->f_poff.ensure(sizeof(int) * (nlist + 1)));
+    MIVF_TRY(ix->f_ioff.ensure(sizeof(int) * (nlist + 1)));
+    MIVF_TRY(ix->f_fill.ensure(sizeof(int) * (nlist + 1)));
+    MIVF_TRY(ix->f_pairq.ensure(sizeof(int32_t) * std::max<int64_t>(np, 1)));
+    MIVF_TRY(ix->f_items.ensure(sizeof(FastItem) * std::max<int64_t>(np, 1)));
+    MIVF_TRY(ix->f_ctrl.ensure(sizeof(int) * 4));
+    MIVF_TRY(ix->f_cand.ensure(sizeof(uint64_t) * size_t(nq) * capq));
+    MIVF_TRY(ix->f_ccnt.ensure(sizeof(int) * nq));
+    MIVF_TRY(ix->f_rthr.ensure(sizeof(float) * nq));
+    MIVF_TRY(ix->f_rhist.ensure(sizeof(int) * size_t(nq) * fast_hist_bins()));
     MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
     MIVF_TRY(ix->f_tau.ensure(sizeof(float) * nq));
+    int *ctrl = ix->f_ctrl.as<int>();  // [0] total items, [1] next item
+    hipStream_t sa = ix->aux;
+    MIVF_CHECK_HIP(hipEventRecord(ix->aux_ev[0], st));
+    MIVF_CHECK_HIP(hipStreamWaitEvent(sa, ix->aux_ev[0], 0));
+    MIVF_CHECK_HIP(launch_pairs(ix->ws.probes.as<int32_t>(), nq, w, nullptr, ix->off.as<int64_t>(), nullptr, nlist,
+                                fast_group_size(), ix->f_cnt.as<int>(), ix->f_poff.as<int>(), ix->f_ioff.as<int>(),
+                                ctrl, ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(), ix->f_items.as<FastItem>(),
+                                sa));
+    MIVF_CHECK_HIP(launch_fill_i32(ix->f_ccnt.as<int>(), nq, 0, sa));
+    MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, sa));
+    // running bounds (+inf) and their histograms (0)
+    MIVF_CHECK_HIP(launch_fill_i32(ix->f_rthr.as<int>(), nq, 0x7f800000, sa));
+    MIVF_CHECK_HIP(launch_fill_i32(ix->f_rhist.as<int>(), nq * fast_hist_bins(), 0, sa));
+    MIVF_CHECK_HIP(hipEventRecord(ix->aux_ev[1], sa));
+    // seed threshold from each query's ns[q] nearest lists (at least
+    // fast_seed, then as many as it takes to hold kl vectors)
     MIVF_CHECK_HIP(launch_seed_count(ix->ws.probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), nlist, kl, nullptr,
                                      std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), st));
     {
@@ -780,32 +812,12 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         a.nlist = nlist;
         a.ks = ix->ks;
         a.kl = kl;
+        a.cap = ix->fast_seed_cap > 0 ? std::max(ix->fast_seed_cap, kl) : 0;
         a.tau = ix->f_tau.as<float>();
         MIVF_CHECK_HIP(launch_seed_tau(a, ix->m, st));
     }
     MIVF_TRY(tm.mark());
-    // 3. all (list, query) pairs, grouped by list, and work items
-    const int64_t np = int64_t(nq) * w;
-    MIVF_TRY(ix->f_cnt.ensure(sizeof(int) * (nlist + 1)));
-    MIVF_TRY(ix->f_poff.ensure(sizeof(int) * (nlist + 1)));
-    MIVF_TRY(ix->f_ioff.ensure(sizeof(int) * (nlist + 1)));
-    MIVF_TRY(ix->f_fill.ensure(sizeof(int) * (nlist + 1)));
-    MIVF_TRY(ix->f_pairq.ensure(sizeof(int32_t) * std::max<int64_t>(np, 1)));
-    MIVF_TRY(ix->f_items.ensure(sizeof(FastItem) * std::max<int64_t>(np, 1)));
-    MIVF_TRY(ix->f_ctrl.ensure(sizeof(int) * 4));
-    int *ctrl = ix->f_ctrl.as<int>();  // [0] total items, [1] next item
-    MIVF_CHECK_HIP(launch_pairs(ix->ws.probes.as<int32_t>(), nq, w, nullptr, ix->off.as<int64_t>(), ix->f_tau.as<float>(), nlist, fast_group_size(),
-                                ix->f_cnt.as<int>(), ix->f_poff.as<int>(), ix->f_ioff.as<int>(), ctrl,
-                                ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(), ix->f_items.as<FastItem>(), st));
-    MIVF_TRY(ix->f_cand.ensure(sizeof(uint64_t) * size_t(nq) * capq));
-    MIVF_TRY(ix->f_ccnt.ensure(sizeof(int) * nq));
-    MIVF_CHECK_HIP(launch_fill_i32(ix->f_ccnt.as<int>(), nq, 0, st));
-    MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, st));
-    // running bounds (+inf) and their histograms (0)
-    MIVF_TRY(ix->f_rthr.ensure(sizeof(float) * nq));
-    MIVF_TRY(ix->f_rhist.ensure(sizeof(int) * size_t(nq) * 32));
-    MIVF_CHECK_HIP(launch_fill_i32(ix->f_rthr.as<int>(), nq, 0x7f800000, st));
-    MIVF_CHECK_HIP(launch_fill_i32(ix->f_rhist.as<int>(), nq * 32, 0, st));
+    MIVF_CHECK_HIP(hipStreamWaitEvent(st, ix->aux_ev[1], 0));
     MIVF_TRY(tm.mark());
     // 4. persistent list-major scan
     {
@@ -823,6 +835,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         a.capq = capq;
         a.ks = ix->ks;
         a.err = ix->errflag.as<int>();
+        a.tau = ix->f_tau.as<float>();
         a.runthr = ix->fast_running ? ix->f_rthr.as<float>() : nullptr;
         a.rhist = ix->fast_running ? ix->f_rhist.as<int>() : nullptr;
         a.kl = kl;
@@ -926,6 +939,7 @@ int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
     a.nlist = ix->nlist;
     a.ks = ix->ks;
     a.kl = kl;
+    a.cap = ix->fast_seed_cap > 0 ? std::max(ix->fast_seed_cap, kl) : 0;
     a.tau = d_tau;
     MIVF_CHECK_HIP(launch_seed_tau(a, ix->m, nullptr));
     MIVF_CHECK_HIP(hipDeviceSynchronize());
@@ -955,6 +969,9 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) {
     if (n == "probes_per_group") {
         MIVF_REQUIRE(value >= 0 && value <= 1024, kInvalid, "probes_per_group must be in 0..1024");
         ix->ppg_opt = int(value);
+    } else if (n == "fast_seed_cap") {
+        MIVF_REQUIRE(value == 0 || (value >= 1 && value <= 8192), kInvalid, "fast_seed_cap must be 0 or in 1..8192");
+        ix->fast_seed_cap = int(value);
     } else if (n == "fast_running_bound") {
         ix->fast_running = value != 0;
     } else if (n == "query_order") {

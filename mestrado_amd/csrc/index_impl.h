@@ -102,7 +102,8 @@ struct mivf_index {
     int ppg_opt = 0;
     int query_order = 1;      // scan queries grouped by nearest list, per XCD (scheduling only)
     bool fast_running = true;  // running per-query bound in the reduced-precision scan
-    int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
+    int fast_seed = 1;
+    int fast_seed_cap = 1024; // vectors entering the seed bound (0 = all of the seed lists)        // lists scanned exactly to seed the reduced-precision filter
     int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
     int batched_cap = 4096;   // candidate capacity per query of the batched exact form (>= 32 kmax)
     int batched_g = 8, batched_nt = 512;  // its work-item shape (queries per item, threads)
@@ -112,9 +113,16 @@ struct mivf_index {
     bool profile = false;
     float timings[8] = {};
     hipEvent_t ev[8] = {};
+    // second stream of the reduced-precision form: the (list, query) grouping
+    // runs there while the seed kernel runs on the caller's stream
+    hipStream_t aux = nullptr;
+    hipEvent_t aux_ev[2] = {};
     ~mivf_index() {
         for (auto &e : ev)
             if (e) hipEventDestroy(e);
+        for (auto &e : aux_ev)
+            if (e) hipEventDestroy(e);
+        if (aux) hipStreamDestroy(aux);
     }
 };
 

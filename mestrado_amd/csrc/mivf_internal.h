@@ -181,11 +181,13 @@ struct FastArgs {
     int capq;
     int ks;
     int *err;
+    const float *tau;           // [nq] seed bounds, or nullptr to take them from the item records
     float *runthr;              // [nq] running bound on the kl-th candidate distance (+inf), or nullptr
-    int *rhist;                 // [nq][32] candidate-distance histograms (zeroed), with runthr
+    int *rhist;                 // [nq][fast_hist_bins()] candidate-distance histograms (zeroed), with runthr
     int kl;
 };
 int fast_group_size();
+int fast_hist_bins();
 size_t fast_pb_bytes(int m);
 hipError_t launch_fast_prep(const float *pq, const float *pmax, int m, int ks, void *pb, hipStream_t s);
 // Seed lists per query: the smallest ns >= min_seed whose first ns probed
@@ -236,6 +238,7 @@ struct SeedArgs {
     const int32_t *nseed;   // [nq] seed lists per query
     const int32_t *klq;     // [nq] per-query kl or null (uniform kl)
     int nq, w, nlist, ks, kl;
+    int cap;                // > 0: at most cap vectors (list prefixes, in probe order) enter the seed
     float *tau;             // [nq]
 };
 // tau[q] >= the kl-th smallest ADC distance over q's seed lists (+inf when

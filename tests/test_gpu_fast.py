@@ -190,11 +190,15 @@ def test_fast_attached_base(mivf, orc, c1):
         ix.search_fast(query, 8, 128, 100)          # stale copy dropped
 
 
-@pytest.mark.parametrize("which,kl,seed_lists", [("m16", 256, 1), ("m16", 64, 2), ("m8", 128, 1), ("m8", 1000, 1)])
-def test_fast_seed_threshold(mivf, orc, zipf16, c1, which, kl, seed_lists):
+@pytest.mark.parametrize("which,kl,seed_lists,cap", [("m16", 256, 1, 0), ("m16", 64, 2, 0), ("m8", 128, 1, 0),
+                                                      ("m8", 1000, 1, 0), ("m16", 128, 1, 300), ("m16", 64, 2, 1024),
+                                                      ("m8", 128, 1, 129)])
+def test_fast_seed_threshold(mivf, orc, zipf16, c1, which, kl, seed_lists, cap):
     """tau_q bounds the kl-th smallest reference ADC distance over the query's
     seed lists from above (the filter keeps everything up to it) and is
-    tight (within 0.1%)."""
+    tight (within 0.1%).  With a cap, the seed sees only the first cap
+    vectors of its lists: tau then bounds the kl-th smallest of that sample
+    (>= the kl-th of the whole seed lists) as tightly."""
     if which == "m16":
         ix, index, query, _ = zipf16
     else:
@@ -207,10 +211,12 @@ def test_fast_seed_threshold(mivf, orc, zipf16, c1, which, kl, seed_lists):
     w = 16
     query = query[:40]
     ix.set_option("fast_seed_lists", seed_lists)
+    ix.set_option("fast_seed_cap", cap)
     try:
         tau = ix.fast_seed_tau(query, w, kl)
     finally:
         ix.set_option("fast_seed_lists", 1)
+        ix.set_option("fast_seed_cap", 1024)  # the library default
     probes, _ = ix.coarse(query, w)
     lens = np.diff(off)
     for qi in range(query.shape[0]):
@@ -225,11 +231,14 @@ def test_fast_seed_threshold(mivf, orc, zipf16, c1, which, kl, seed_lists):
                 continue
             tab = orc.tables((query[qi] - coarse[l]).astype(np.float32), pq)
             vals.append(orc.adc(tab, codes[off[l]:off[l + 1]]))
-        vals = np.sort(np.concatenate(vals)) if vals else np.zeros(0, np.float32)
+        allv = np.concatenate(vals) if vals else np.zeros(0, np.float32)
+        sample = allv[:max(cap, kl)] if cap else allv
+        vals = np.sort(sample)
         if len(vals) < kl:
             assert np.isinf(tau[qi]), f"q={qi}: fewer than kl seed vectors but finite tau"
             continue
         kth = float(vals[kl - 1])
+        assert tau[qi] >= float(np.sort(allv)[kl - 1]), f"q={qi}: tau below the kl-th seed-list distance"
         assert tau[qi] >= kth, f"q={qi}: tau {tau[qi]} below the kl-th seed distance {kth}"
         assert tau[qi] <= kth * 1.001 + 1.0, f"q={qi}: tau {tau[qi]} loose against {kth}"
 
