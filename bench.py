@@ -45,6 +45,8 @@ def main():
     ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--batched-cap", type=int, default=0, help="candidate buffer per query of the batched form "
+                                                                   "(0 = library default)")
     ap.add_argument("--serve-fracs", default="0.25,0.5,0.75", help="offered load as fractions of the batch QPS")
     ap.add_argument("--serve-reqs", type=int, default=12000)
     ap.add_argument("--profile-only", action="store_true",
@@ -192,6 +194,8 @@ def main():
         log(f"fast form: {fast}")
     uneven = None
     if not args.no_uneven:
+        if args.batched_cap:
+            ix.set_option("batched_capacity", args.batched_cap)
         # batched uneven exact search: per-query w ~ U{1..64}, k ~ {1, 10, 100}
         rng = np.random.default_rng(seed)
         wq = rng.integers(1, 65, nq).astype(np.int32)
@@ -221,9 +225,54 @@ def main():
                     np.array_equal(v_dis.numpy().view(np.uint32), b_dis.numpy().view(np.uint32)))
         uneven = {"w": "U{1..64}", "k": "{1,10,100}", "plain_var_qps": round(nq / t_var, 1),
                   "batched_graph_qps": round(nq / t_b, 1), "speedup": round(t_var / t_b, 3),
-                  "bit_identical_to_plain": same, **{f"tuned_{k2}": v for k2, v in plan.info().items()}}
+                  "bit_identical_to_plain": same, "overflowed_queries": ix.batched_stats(),
+                  **{f"tuned_{k2}": v for k2, v in plan.info().items()}}
         plan.close()
         log(f"batched uneven: {uneven}")
+        # the same batched exact form on the headline batch (uniform w and k)
+        wu = np.full(nq, w, np.int32)
+        ku = np.full(nq, k, np.int32)
+        u_ids = M.DeviceArray((nq, k), np.int32)
+        u_dis = M.DeviceArray((nq, k), np.float32)
+        plan = M.Plan(ix, nq, w, k, autotune=True)
+        for _ in range(max(1, args.warmup)):
+            plan.search(query, wu, ku, out=(u_ids, u_dis))
+        M.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.search(query, wu, ku, out=(u_ids, u_dis))
+        M.synchronize()
+        t_u = (time.perf_counter() - t0) / args.steps
+        same_u = bool(np.array_equal(out_ids.numpy(), u_ids.numpy()) and
+                      np.array_equal(out_dis.numpy().view(np.uint32), u_dis.numpy().view(np.uint32)))
+        uneven["uniform_headline"] = {"w": w, "k": k, "batched_graph_qps": round(nq / t_u, 1),
+                                      "ms_per_step": round(t_u * 1000, 4),
+                                      "algorithmic_scan_GBps_whole_search": round(scan_bytes / t_u / 1e9, 1),
+                                      "bit_identical_to_plain": same_u, "overflowed_queries": ix.batched_stats(),
+                                      **{f"tuned_{k2}": v for k2, v in plan.info().items()}}
+        info = plan.info()
+        plan.close()
+        # its scan kernel's time (HIP events, same work-item shape as the plan)
+        ix.set_option("batched_group", info["group"])
+        ix.set_option("batched_threads", info["threads"])
+        ix.set_option("profile", 1)
+        bst = np.zeros(8)
+        for _ in range(args.steps):
+            ix.search_batched(query, wu, ku, k, out=(u_ids, u_dis))
+            bst += ix.timings()
+        ix.set_option("profile", 0)
+        bst /= args.steps
+        b_scan = float(bst[2])
+        uneven["uniform_headline"]["stages_ms"] = {n: round(float(v), 4) for n, v in
+                                                   zip(("coarse", "seed_grouping", "scan", "select"), bst)}
+        uneven["uniform_headline"]["scan_kernel_roofline"] = {
+            "kernel": "exact_scan_kernel", "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "achieved": round(scan_bytes / (b_scan * 1e-3) / 1e9, 1) if b_scan > 0 else None,
+            "frac": round(scan_bytes / (b_scan * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if b_scan > 0 else None,
+            "algorithmic_bytes_per_launch": scan_bytes,
+            "note": "codes of every (query, probed list) pair, as for the plain scan; each list is read once "
+                    "per group of queries"}
+        log(f"batched uniform: {uneven['uniform_headline']}")
 
     serving = None
     if not args.no_serve and not args.profile_only:
@@ -258,12 +307,18 @@ def main():
     achieved = scan_bytes / (scan_ms * 1e-3) / 1e9 if scan_ms > 0 else None
     # HBM bytes per launch of the scan kernel from the FETCH_SIZE / WRITE_SIZE
     # counter runs recorded for this workload (null when none is recorded)
-    traffic = None
-    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"counters_{args.workload}.md")
-    if os.path.exists(prof) and not args.nbase:
-        for line in open(prof):
-            if line.startswith("traffic_bytes_per_launch scan_kernel:"):
-                traffic = int(line.split(":")[1])
+    def recorded_traffic(fname, kernel):
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{fname}_{args.workload}.md")
+        if os.path.exists(path) and not args.nbase:
+            for line in open(path):
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return int(line.split(":")[1])
+        return None
+
+    traffic = recorded_traffic("counters", "scan_kernel")
+    rb = (uneven or {}).get("uniform_headline", {}).get("scan_kernel_roofline")
+    if rb is not None:
+        rb["traffic"] = recorded_traffic("batched_exact_scan", "exact_scan_kernel")
 
     if fast is not None:
         fast["speedup_vs_plain"] = round(fast["qps"] / qps, 3)
@@ -311,6 +366,8 @@ def main():
                      "traffic": traffic,
                      "kernel": "scan_kernel", "algorithmic_bytes_per_launch": scan_bytes},
         "cpu_baseline": cpu,
+        # the same exact arithmetic, list-major: the batched form's scan kernel on the headline batch
+        "roofline_batched_exact": (uneven or {}).get("uniform_headline", {}).get("scan_kernel_roofline"),
         "fast_form": fast,
         "batched_uneven": uneven,
         "serving": serving,

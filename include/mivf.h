@@ -186,8 +186,10 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
 int mivf_set_option(mivf_index *ix, const char *name, int64_t value);
 /* Per-stage milliseconds of the last profiled search:
  * [0] coarse distances, [1] probe select, [2] scan, [3] merge (mivf_search);
- * [0] coarse, [1] seed scan, [2] pair grouping, [3] 8-bit scan, [4] shortlist,
- * [5] rerank (mivf_search_fast).                                            */
+ * [0] coarse, [1] seed scan, [2] wait for the pair grouping (which runs on a
+ * second stream beside the seed), [3] 8-bit scan, [4] shortlist, [5] rerank
+ * (mivf_search_fast); [0] coarse, [1] probe masking + seed + grouping,
+ * [2] exact scan, [3] select (mivf_search_batched).                         */
 int mivf_get_timings(mivf_index *ix, float *ms, int n);
 
 /* ---- batched exact form ----------------------------------------------------
@@ -203,6 +205,10 @@ int mivf_get_timings(mivf_index *ix, float *ms, int n);
  * "batched_threads", "batched_capacity".  d_q must be 16-byte aligned.      */
 int mivf_search_batched(mivf_index *ix, const float *d_q, int nq, const int32_t *w, const int32_t *k,
                         int kmax, int32_t *d_ids, float *d_dis, void *stream);
+/* Queries of the last mivf_search_batched / mivf_plan_search whose
+ * candidates exceeded the buffer ("batched_capacity") and were recomputed
+ * by the plain kernels.                                                     */
+int mivf_batched_stats(mivf_index *ix, int64_t *overflowed_queries);
 /* A reusable plan for batches of exactly nq queries with w <= wmax, k <= kmax:
  * the first mivf_plan_search runs the autotuner (when asked: every compiled
  * work-item shape timed on that batch), then captures the whole launch

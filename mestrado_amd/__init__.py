@@ -62,6 +62,7 @@ _SIGS = {
     "mivf_fast_seed_tau": [_vp, _vp, _i, _i, _i, _vp],
     "mivf_fast_select": [_vp, _vp, _i, _i, _i, _vp],
     "mivf_search_batched": [_vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp],
+    "mivf_batched_stats": [_vp, _i64p],
     "mivf_plan_create": [_vp, _i, _i, _i, _i, ctypes.POINTER(_vp)],
     "mivf_plan_search": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "mivf_plan_info": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _f32p],
@@ -371,6 +372,12 @@ class Index:
         """Keep a list-ordered copy of the raw vectors for search_fast(base=None)."""
         bd, _ = _dev(base, np.float32)
         _check(lib().mivf_index_attach_base(self.h, bd.ptr, bd.shape[0]))
+
+    def batched_stats(self):
+        """Overflowed (recomputed) queries of the last batched search."""
+        o = ctypes.c_int64(0)
+        _check(lib().mivf_batched_stats(self.h, ctypes.byref(o)))
+        return o.value
 
     def search_batched(self, q, w, k, kmax=None, out=None):
         """Batched exact search with per-query w and k (same results as search_var)."""

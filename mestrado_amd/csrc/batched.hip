@@ -35,12 +35,12 @@ bool cfg_ok(int m, const XCfg &c) { return exact_lds_bytes(m, c.g) > 0 && (c.nt 
 
 std::vector<XCfg> candidates(int m) {
     std::vector<XCfg> v;
-    const XCfg all[] = {{4, 256}, {8, 256}, {8, 512}, {16, 512}};
+    const XCfg all[] = {{4, 256}, {4, 512}, {8, 256}, {8, 512}, {16, 512}};
     for (const auto &c : all) {
         // only the compiled shapes
         const int key = (m * 100 + c.g) * 10000 + c.nt;
-        if (key == 16040256 || key == 16080256 || key == 16080512 || key == 8040256 || key == 8080256 ||
-            key == 8080512 || key == 8160512)
+        if (key == 16040256 || key == 16040512 || key == 16080256 || key == 16080512 || key == 8040256 ||
+            key == 8040512 || key == 8080256 || key == 8080512 || key == 8160512)
             v.push_back(c);
     }
     return v;
@@ -49,7 +49,7 @@ std::vector<XCfg> candidates(int m) {
 // Enqueue the batched exact search (graph-capturable once the workspace has
 // been sized by a first run).
 int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, int nq, int wmax, int kmax,
-                    int32_t *d_ids, float *d_dis, hipStream_t st) {
+                    int32_t *d_ids, float *d_dis, hipStream_t st, Timer *tm = nullptr) {
     const int nlist = ix->nlist;
     const int capq = capacity(ix, kmax);
     const int64_t np = int64_t(nq) * wmax;
@@ -73,6 +73,7 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     int *ctrl = x.ctrl.as<int>();  // [0] total items, [1] next item
 
     MIVF_TRY(coarse_stage(ix, x.ws, d_q, nq, wmax, probes, x.ws.coadis.as<float>(), st, nullptr));
+    if (tm) MIVF_TRY(tm->mark());
     MIVF_CHECK_HIP(launch_mask_probes(probes, nq, wmax, x.wq.as<int32_t>(), st));
     MIVF_CHECK_HIP(launch_seed_count(probes, nq, wmax, ix->off.as<int64_t>(), nlist, kmax, x.kq.as<int32_t>(), 1,
                                      x.ns.as<int32_t>(), st));
@@ -116,9 +117,12 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     a.ks = ix->ks;
     a.err = ix->errflag.as<int>();
     const int per_cu = std::max(1, int((160 * 1024) / exact_lds_bytes(ix->m, cfg.g)));
+    if (tm) MIVF_TRY(tm->mark());
     MIVF_CHECK_HIP(launch_exact_scan(a, ix->m, cfg.g, cfg.nt, ix->ncu * per_cu, st));
+    if (tm) MIVF_TRY(tm->mark());
     MIVF_CHECK_HIP(launch_exact_select(x.cand.as<uint64_t>(), x.ccnt.as<int>(), capq, x.kq.as<int32_t>(), kmax, kmax,
                                        nq, d_ids, d_dis, x.ovf.as<int>(), st));
+    if (tm) MIVF_TRY(tm->mark());
     return kOk;
 }
 
@@ -274,10 +278,30 @@ int mivf_search_batched(mivf_index *ix, const float *d_q, int nq, const int32_t 
     cfg.nt = ix->batched_nt;
     MIVF_REQUIRE(cfg_ok(ix->m, cfg), kInvalid, "search_batched: work-item shape not compiled");
     MIVF_TRY(upload_wk(x, nq, w, k, st));
-    MIVF_TRY(batched_enqueue(ix, x, cfg, d_q, nq, wmax, kmax, d_ids, d_dis, st));
+    Timer tm;
+    std::fill(ix->timings, ix->timings + 8, 0.0f);
+    MIVF_TRY(tm.start(ix->profile, st, ix->ev));
+    MIVF_TRY(batched_enqueue(ix, x, cfg, d_q, nq, wmax, kmax, d_ids, d_dis, st, tm.on ? &tm : nullptr));
+    if (tm.on) {
+        MIVF_CHECK_HIP(hipEventSynchronize(tm.ev[tm.n - 1]));
+        for (int i = 0; i + 1 < tm.n && i < 8; ++i) {
+            float ms = 0;
+            MIVF_CHECK_HIP(hipEventElapsedTime(&ms, tm.ev[i], tm.ev[i + 1]));
+            ix->timings[i] = ms;
+        }
+    }
     int nov = 0;
+    float kept[8];
+    std::copy(ix->timings, ix->timings + 8, kept);  // the rerun of overflowed queries records its own
     MIVF_TRY(batched_fixup(ix, x, d_q, nq, w, k, kmax, d_ids, d_dis, st, &nov));
+    std::copy(kept, kept + 8, ix->timings);
     ix->batched_last_overflow = nov;
+    return kOk;
+}
+
+int mivf_batched_stats(mivf_index *ix, int64_t *overflowed_queries) {
+    MIVF_REQUIRE(ix, kInvalid, "null index");
+    if (overflowed_queries) *overflowed_queries = ix->batched_last_overflow;
     return kOk;
 }
 

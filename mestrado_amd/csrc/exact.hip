@@ -39,8 +39,8 @@ struct XLay {
     static constexpr size_t gbase = gcnt + 64;                      // int [16]
     static constexpr size_t rec = gbase + 64;                       // FastItem
     static constexpr size_t nrec = rec + sizeof(FastItem);          // FastItem
-    static constexpr size_t pcnt = nrec + sizeof(FastItem);         // int
-    static constexpr size_t item = pcnt + 4;                        // int
+    static constexpr size_t pcnt = nrec + sizeof(FastItem);         // int [4] rotating append counters
+    static constexpr size_t item = pcnt + 16;                       // int
     static constexpr size_t nidx = item + 4;                        // int
     static constexpr size_t total = nidx + 8;
 };
@@ -59,22 +59,21 @@ __device__ inline void xload_codes(const uint8_t *lc, int v, uint32_t (&cw)[M / 
 // Staged candidates -> per-query global buffers as (distance, id) keys.
 // Entry: x = list position | g << 28, y = distance bits.
 template <int M, int G, int NT>
-__device__ void xflush(const ExactArgs &a, unsigned char *smem, int64_t beg) {
+__device__ void xflush(const ExactArgs &a, unsigned char *smem, int64_t beg, int filled) {
     using L = XLay<M, G>;
     const uint2 *pool = reinterpret_cast<const uint2 *>(smem + L::pool);
     uint16_t *prank = reinterpret_cast<uint16_t *>(smem + L::prank);
     int *gcnt = reinterpret_cast<int *>(smem + L::gcnt);
     int *gbase = reinterpret_cast<int *>(smem + L::gbase);
-    int *pcnt = reinterpret_cast<int *>(smem + L::pcnt);
     const int *qid = reinterpret_cast<const int *>(smem + L::qid);
     const int tid = threadIdx.x;
-    const int n = min(*pcnt, POOLX);
+    const int n = min(filled, POOLX);
     if (tid < 16) gcnt[tid] = 0;
-    __syncthreads();
+    lds_barrier();
     for (int i = tid; i < n; i += NT) prank[i] = uint16_t(atomicAdd(gcnt + (pool[i].x >> 28), 1));
-    __syncthreads();
+    lds_barrier();
     if (tid < G && gcnt[tid] > 0) gbase[tid] = atomicAdd(a.cand_cnt + qid[tid], gcnt[tid]);
-    __syncthreads();
+    lds_barrier();
     for (int i = tid; i < n; i += NT) {
         const uint2 e = pool[i];
         const int g = int(e.x >> 28);
@@ -83,9 +82,7 @@ __device__ void xflush(const ExactArgs &a, unsigned char *smem, int64_t beg) {
             a.cand[int64_t(qid[g]) * a.capq + pos] =
                 ckey(__builtin_bit_cast(float, e.y), a.list_ids[beg + (e.x & 0x0FFFFFFFu)]);
     }
-    __syncthreads();
-    if (tid == 0) *pcnt = 0;
-    __syncthreads();
+    lds_barrier();  // the pool is free again
 }
 
 // The G exact tables of one item (reference arithmetic, as the plain scan's
@@ -150,7 +147,7 @@ __device__ void xtables(const ExactArgs &a, unsigned char *smem, int l) {
 }
 
 template <int M, int G, int NT>
-__global__ __launch_bounds__(NT) void exact_scan_kernel(ExactArgs a) {
+__global__ __launch_bounds__(NT, NT / 128) void exact_scan_kernel(ExactArgs a) {  // two workgroups per CU
     using L = XLay<M, G>;
     constexpr int RW = int(sizeof(FastItem) / 4);
     static_assert(G % 4 == 0 && G <= 16, "G must be 4, 8, 12 or 16");
@@ -170,10 +167,8 @@ __global__ __launch_bounds__(NT) void exact_scan_kernel(ExactArgs a) {
         return;
     }
     const int total = *a.total_items;
-    if (tid == 0) {
-        *pcnt = 0;
-        *item_sh = atomicAdd(a.next_item, 1);
-    }
+    if (tid < 4) pcnt[tid] = 0;
+    if (tid == 0) *item_sh = atomicAdd(a.next_item, 1);
     __syncthreads();
     if (*item_sh < total && tid < RW)
         reinterpret_cast<uint32_t *>(rec)[tid] = reinterpret_cast<const uint32_t *>(a.items + *item_sh)[tid];
@@ -190,6 +185,7 @@ __global__ __launch_bounds__(NT) void exact_scan_kernel(ExactArgs a) {
             qid[tid] = q;
             lim[tid] = q >= 0 ? rec->tau[tid] : -__builtin_huge_valf();
         }
+        if (tid < 3) pcnt[tid] = 0;
         __syncthreads();
         xtables<M, G, NT>(a, smem, l);
 
@@ -217,7 +213,10 @@ __global__ __launch_bounds__(NT) void exact_scan_kernel(ExactArgs a) {
         const uint8_t *lc = a.codes + beg * M;
         uint32_t cn[M / 4];
         xload_codes<M>(lc, min(tid, max(len - 1, 0)), cn);
+        int fill = 0;  // staged candidates before this step (uniform)
+        int rot = 0;   // append counter of this step
         for (int base = 0; base < len; base += NT) {
+            if (tid == 0) pcnt[rot == 2 ? 0 : rot + 1] = 0;
             const int v = base + tid;
             uint32_t cw[M / 4];
 #pragma unroll
@@ -252,8 +251,8 @@ __global__ __launch_bounds__(NT) void exact_scan_kernel(ExactArgs a) {
                     tot += uint32_t(__popcll(bm)) << b;
                 }
                 int slot = 0;
-                if (lane == 0) slot = atomicAdd(pcnt, int(tot));
-                slot = __shfl(slot, 0) + int(pre);
+                if (lane == 0) slot = atomicAdd(pcnt + rot, int(tot));
+                slot = fill + __builtin_amdgcn_readfirstlane(slot) + int(pre);
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     if (!((pm >> g) & 1u)) continue;
@@ -267,16 +266,24 @@ __global__ __launch_bounds__(NT) void exact_scan_kernel(ExactArgs a) {
                     ++slot;
                 }
             }
-            __syncthreads();  // this step's appends are done
-            const int filled = *pcnt;
-            __syncthreads();  // every thread has read the count before the next appends
-            if (filled > POOLX / 2) xflush<M, G, NT>(a, smem, beg);  // uniform
+            // Three append counters rotate over the steps: counter rot takes
+            // this step's appends, counter rot + 1 (cleared at this step's
+            // start) the next step's, and counter rot + 2, read by every
+            // thread before this step's barrier, is free.  One LDS-only
+            // barrier per step then orders the appends before the reads.
+            lds_barrier();  // this step's appends are done
+            fill += pcnt[rot];
+            rot = rot == 2 ? 0 : rot + 1;
+            if (fill > POOLX / 2) {  // uniform
+                xflush<M, G, NT>(a, smem, beg, fill);
+                fill = 0;
+            }
             if (stage < 2) advance();
         }
-        if (*pcnt > 0) xflush<M, G, NT>(a, smem, beg);
+        if (fill > 0) xflush<M, G, NT>(a, smem, beg, fill);
         while (stage < 2) {
+            __syncthreads();  // orders the previous stage's LDS writes before this stage
             advance();
-            __syncthreads();
         }
         __syncthreads();  // rec, qid and the pool are free; nrec is complete
         if (tid < RW) reinterpret_cast<uint32_t *>(rec)[tid] = reinterpret_cast<const uint32_t *>(nrec)[tid];
@@ -422,6 +429,8 @@ size_t exact_lds_bytes(int m, int g) {
 hipError_t launch_exact_scan(const ExactArgs &a, int m, int g, int nt, int grid, hipStream_t s) {
     switch ((m * 100 + g) * 10000 + nt) {
         case 16040256: return launch_x<16, 4, 256>(a, grid, s);
+        case 16040512: return launch_x<16, 4, 512>(a, grid, s);
+        case 8040512: return launch_x<8, 4, 512>(a, grid, s);
         case 16080256: return launch_x<16, 8, 256>(a, grid, s);
         case 16080512: return launch_x<16, 8, 512>(a, grid, s);
         case 8040256: return launch_x<8, 4, 256>(a, grid, s);
@@ -434,12 +443,14 @@ hipError_t launch_exact_scan(const ExactArgs &a, int m, int g, int nt, int grid,
 
 hipError_t launch_exact_tables(const ExactArgs &a, int m, int l, int n, float *out, hipStream_t s) {
     if (m == 16) {
-        hipFuncSetAttribute(reinterpret_cast<const void *>(&exact_tables_kernel<16, 8, 512>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(XLay<16, 8>::total));
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&exact_tables_kernel<16, 8, 512>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(XLay<16, 8>::total));
+        if (e != hipSuccess) return e;
         exact_tables_kernel<16, 8, 512><<<1, 512, XLay<16, 8>::total, s>>>(a, l, n, out);
     } else if (m == 8) {
-        hipFuncSetAttribute(reinterpret_cast<const void *>(&exact_tables_kernel<8, 8, 512>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(XLay<8, 8>::total));
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&exact_tables_kernel<8, 8, 512>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(XLay<8, 8>::total));
+        if (e != hipSuccess) return e;
         exact_tables_kernel<8, 8, 512><<<1, 512, XLay<8, 8>::total, s>>>(a, l, n, out);
     } else {
         return hipErrorInvalidValue;
@@ -449,8 +460,9 @@ hipError_t launch_exact_tables(const ExactArgs &a, int m, int l, int n, float *o
 
 hipError_t launch_exact_adc(const ExactArgs &a, int m, int l, int n, int64_t beg, int len, float *out, hipStream_t s) {
     if (m != 16) return hipErrorInvalidValue;
-    hipFuncSetAttribute(reinterpret_cast<const void *>(&exact_adc_kernel<16, 8, 512>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, int(XLay<16, 8>::total));
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&exact_adc_kernel<16, 8, 512>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(XLay<16, 8>::total));
+    if (e != hipSuccess) return e;
     exact_adc_kernel<16, 8, 512><<<1, 512, XLay<16, 8>::total, s>>>(a, l, n, beg, len, out);
     return hipGetLastError();
 }

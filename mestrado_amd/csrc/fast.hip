@@ -81,17 +81,6 @@ struct FLay {
     static constexpr size_t total = nidx + 8;
 };
 
-// Workgroup barrier that orders LDS only.  __syncthreads() also fences
-// global memory, which makes every wave wait for its outstanding global
-// loads (here: the code prefetch of later steps) at each barrier.  Global
-// data is never exchanged between the threads of a workgroup at these
-// barriers, so the LDS-scope fence is all they need.
-__device__ inline void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Move the filled staged candidates to the per-query global buffers (whole
 // workgroup, uniform call).  Entry: x = list position, y = g | sum << 4,
 // and the rank within its query slot goes into bits 16..27 of y.

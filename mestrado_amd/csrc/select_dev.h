@@ -11,6 +11,17 @@
 
 namespace mivf {
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also fences
+// global memory, which makes every wave wait for its outstanding global
+// loads (here: the code prefetch of later steps) at each barrier.  Global
+// data is never exchanged between the threads of a workgroup at these
+// barriers, so the LDS-scope fence is all they need.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 struct SelectResult {
     uint64_t thr;
     int eq;
