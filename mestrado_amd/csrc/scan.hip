@@ -383,12 +383,13 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
                         const int before = __popcll(mask & ((uint64_t(1) << lane) - 1));
                         int pos = 0;
                         if (lane == 0) pos = atomicAdd(&C.inc[rot], __popcll(mask));
-                        pos = fill + __shfl(pos, 0) + before;
+                        pos = fill + __builtin_amdgcn_readfirstlane(pos) + before;  // lane 0's count
                         if (pass && pos < CAP)
                             cand[pos] = (uint64_t(fkey(accs[u])) << 32) | uint32_t(bg + v);
                     }
                 }
-                __syncthreads();
+                // LDS-only: the code loads of the next steps stay in flight
+                lds_barrier();
                 fill += C.inc[rot];
                 ++step;
                 if (fill > CAP - STEP) {
