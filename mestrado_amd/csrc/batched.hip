@@ -29,7 +29,7 @@ struct XWs {
     DevBuf wq, kq, ns, tau, cnt, poff, ioff, fill, pairq, items, ctrl, cand, ccnt, ovf;
 };
 
-int capacity(const mivf_index *ix, int kmax) { return std::max(ix->batched_cap, 32 * kmax); }
+int capacity(const mivf_index *ix, int kmax) { return std::max(ix->batched_cap, 2 * kmax); }
 
 bool cfg_ok(int m, const XCfg &c) { return exact_lds_bytes(m, c.g) > 0 && (c.nt == 256 || c.nt == 512); }
 

@@ -105,7 +105,7 @@ struct mivf_index {
     int fast_seed = 1;
     int fast_seed_cap = 1024; // vectors entering the seed bound (0 = all of the seed lists)        // lists scanned exactly to seed the reduced-precision filter
     int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
-    int batched_cap = 8192;   // candidate capacity per query of the batched exact form (>= 32 kmax)
+    int batched_cap = 8192;   // candidate capacity per query of the batched exact form (>= 2 kmax)
     int batched_g = 8, batched_nt = 512;  // its work-item shape (queries per item, threads)
     int batched_last_overflow = 0;
     int ncu = 256;            // compute units of the device

@@ -19,7 +19,7 @@ def uneven(nq, seed, wmax=64):
     return rng.integers(1, wmax + 1, nq).astype(np.int32), rng.choice([1, 10, 100], nq).astype(np.int32)
 
 
-@pytest.mark.parametrize("g,nt", [(8, 512), (8, 256), (4, 256)])
+@pytest.mark.parametrize("g,nt", [(8, 512), (8, 256), (4, 256), (4, 512)])
 def test_batched_equals_plain_m16(mivf, zipf16, g, nt):
     ix, index, query, _ = zipf16
     w, k = uneven(query.shape[0], seed=g + nt)
@@ -43,9 +43,14 @@ def test_batched_overflow_fallback(mivf, zipf16):
     ix.set_option("batched_capacity", 64)
     try:
         ids, dis = ix.search_batched(query, w, k, 100)
+        over = ix.batched_stats()
     finally:
-        ix.set_option("batched_capacity", 4096)
+        ix.set_option("batched_capacity", 8192)  # the library default
+    assert over > 0, "the tiny buffer should overflow"
     assert_same(ids, dis, rids, rdis, "batched with overflow")
+    ids, dis = ix.search_batched(query, w, k, 100)
+    assert ix.batched_stats() == 0
+    assert_same(ids, dis, rids, rdis, "batched without overflow")
 
 
 def test_batched_m8_ties_empty_lists(mivf, orc, c1):
