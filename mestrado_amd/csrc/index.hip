@@ -307,6 +307,12 @@ int search_core(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, cons
         MIVF_CHECK_HIP(launch_query_order(probes, wmax, nq, nlist, ws.qorder.as<int32_t>(), st));
         a.qorder = ws.qorder.as<int32_t>();
     }
+    if (!d_probes_in) {
+        MIVF_TRY(ws.rres.ensure(sizeof(float) * size_t(nq) * wmax * d));
+        a.rres = ws.rres.as<float>();
+    }
+    MIVF_TRY(ws.rnd.ensure(sizeof(double) * size_t(nq) * wmax * ix->m));
+    a.rnd = ws.rnd.as<double>();
     if (ngroups > 1) {
         MIVF_TRY(ws.partial.ensure(sizeof(uint64_t) * size_t(nq) * ngroups * kstride));
         a.partial = ws.partial.as<uint64_t>();
@@ -381,8 +387,10 @@ int search_entry(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, con
     MIVF_TRY(upload_lists(ix));
     Timer tm;
     std::fill(ix->timings, ix->timings + 8, 0.0f);
-    // bound the coarse distance matrix to ~512 MiB per chunk
-    const int64_t maxq = std::max<int64_t>(1, (int64_t(512) << 20) / (4 * int64_t(ix->nlist)));
+    // bound the coarse distance matrix and the residual rows of the probes
+    // to ~512 MiB each per chunk
+    const int64_t per_q = std::max<int64_t>(4 * int64_t(ix->nlist), int64_t(wmax) * (4 * ix->d + 8 * ix->m));
+    const int64_t maxq = std::max<int64_t>(1, (int64_t(512) << 20) / per_q);
     for (int64_t q0 = 0; q0 < nq; q0 += maxq) {
         const int nc = int(std::min<int64_t>(maxq, nq - q0));
         MIVF_TRY(tm.start(ix->profile, st, ix->ev));

@@ -65,7 +65,7 @@ struct Timer {
 // Scratch of one search in flight (grow-only).  The index owns one; each
 // serving slot owns another, so searches on different streams can overlap.
 struct Ws {
-    DevBuf qn, dmat, probes, coadis, partial, wq, kq, redo, tmp_ids, tmp_dis, qorder;
+    DevBuf qn, dmat, probes, coadis, partial, wq, kq, redo, tmp_ids, tmp_dis, qorder, rres, rnd;
 };
 
 }  // namespace mivf

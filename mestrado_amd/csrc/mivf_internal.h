@@ -145,7 +145,11 @@ struct ScanArgs {
     uint64_t *partial;        // [nq][ngroups][kstride] when ngroups > 1
     int32_t *out_ids;         // [nq][kstride] when ngroups == 1
     float *out_dis;
+    float *rres;              // [nq*wstride][d] residual rows (written by the scan prologue; unused in resid_mode)
+    double *rnd;              // [nq*wstride][m] f64 sub-vector norms of the residual rows
 };
+// Prologue (residual rows and their sub-vector norms of every valid
+// (query, probe) pair) followed by the fused table + ADC + top-k scan.
 hipError_t launch_scan(const ScanArgs &a, int m, hipStream_t s);
 // Queries ordered by their first probed list (counting sort; the order
 // inside a list is arbitrary).  Only the schedule of the scan depends on it.

@@ -4,14 +4,17 @@
 //   ADC  d = 0f; d = fl32(d + T[s][code_s]), s ascending   (ivf_search.cpp:451-468)
 //   running exact top-k by (distance, id)          (parallel2/ivf_pq/topk.cu:26-32)
 //
-// Workgroup = 512 threads (8 wavefronts).  Thread (j, h) keeps the PQ
-// centroid j of subquantizers [h*M/2, (h+1)*M/2) and their f64 norms in
-// registers for the whole launch, so a table costs no memory traffic beyond
-// the coarse centroid row.  The m x 256 FP32 table lives in LDS; every
-// lane scans one vector at a time with one 8/16-byte coalesced code load and
-// M LDS gathers.  Candidates that beat the current k-th key are appended to an
-// LDS buffer with one LDS atomic per wavefront; when the buffer fills, an exact
-// block radix select shrinks it back to k and tightens the threshold.
+// Workgroup = 512 threads (8 wavefronts).  A prologue kernel writes the
+// residual row and the f64 sub-vector norms of every (query, probe) pair;
+// the scan reads them with scalar loads, so a table entry costs one vector
+// load of P[s][j] (shared by the TB tables of a phase), DS multiplies and
+// adds with a scalar operand, one f64 add and one LDS store.  The m x 256
+// FP32 tables live in LDS; every lane scans U vectors per step with one
+// 8/16-byte coalesced code load each and M LDS gathers.  Candidates that
+// beat the current threshold are appended to an LDS buffer with one LDS
+// atomic per wavefront; when the buffer fills, a radix select on the distance
+// keys shrinks it to the candidates within the k-th distance (positions are
+// resolved to ids only for the final exact selection by (distance, id)).
 #include "select_dev.h"
 
 namespace mivf {
@@ -61,9 +64,11 @@ struct ScanCtl {
     int inc[3];  // per-step append counters, rotated so one barrier per step suffices
     int newcnt;
     int eqt;
+    uint32_t maxhi;
 };
 
 constexpr int TB = 2;  // tables built per phase (probes scanned per phase)
+constexpr int PD = 1;  // subquantizers of P loaded ahead during a table build
 
 // Dynamic LDS layout.  The tables sit at LDS address 0 so that a gather is
 // one ds_read_b32 at code*4 with the table row as immediate offset.
@@ -71,18 +76,60 @@ template <int M, int CAP>
 struct Lay {
     static constexpr size_t tab = 0;  // float [TB][M][256]
     static constexpr size_t cand = tab + size_t(TB) * M * 256 * 4;
-    static constexpr size_t qv = cand + size_t(CAP) * 8;
-    static constexpr size_t resid = qv + D * 4;          // float [TB][D]
-    static constexpr size_t rnorm = resid + TB * D * 4;  // float [TB][M]
-    static constexpr size_t ctl = (rnorm + TB * M * 4 + 15) & ~size_t(15);
+    static constexpr size_t ctl = cand + size_t(CAP) * 8;
     static constexpr size_t total = ctl + sizeof(ScanCtl);
 };
 
+// Float whose order key is `kh` (+inf for keys at or past +inf).
+__device__ inline float key_hi_dist(uint32_t kh) {
+    return kh >= 0xFF800000u ? __builtin_huge_valf() : key_dist(uint64_t(kh) << 32);
+}
+
 // Candidates carry (distance key, list position) while scanning, so the hot
-// loop issues no id loads; cand[conv..c) become (distance, id) keys here,
-// right before a selection.  Then shrink cand[0..c) to the kk smallest keys.
-// Returns the new count; *thr receives the new filter threshold.  All threads
-// call with the same c and conv.
+// loop issues no id loads.  Two ways to shrink the buffer (all threads call
+// with the same arguments; only LDS is exchanged, so the barriers are
+// LDS-only and code loads of later steps stay in flight):
+//
+// compact_dist: keep every candidate whose distance is <= the kk-th smallest
+// distance.  Positions stay unresolved: ids only order keys of equal
+// distance, and all of those are kept.  Returns the new count and sets
+// *thr_f to the largest kept distance.
+template <int CAP>
+__device__ int compact_dist(ScanCtl &C, uint64_t *cand, int c, int kk, float *thr_f) {
+    constexpr int PER = CAP / NT;
+    const int tid = threadIdx.x;
+    if (c <= kk) return c;
+    const SelectResultT<uint32_t> r =
+        block_select_lds<NT, uint32_t>([&](int i) { return uint32_t(cand[i] >> 32); }, c, kk, C.sel);
+    uint64_t mine[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int i = tid + e * NT;
+        mine[e] = i < c ? cand[i] : kNoKey;
+    }
+    if (tid == 0) {
+        C.newcnt = 0;
+        C.maxhi = 0;
+    }
+    lds_barrier();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const uint64_t key = mine[e];
+        if (key == kNoKey) continue;
+        const uint32_t h = uint32_t(key >> 32);
+        if (r.eq > 0 ? h <= r.thr : h < r.thr) {  // eq > 0: thr is the kk-th distance itself
+            cand[atomicAdd(&C.newcnt, 1)] = key;
+            atomicMax(&C.maxhi, h);
+        }
+    }
+    lds_barrier();
+    *thr_f = key_hi_dist(C.maxhi);
+    return C.newcnt;
+}
+
+// compact: cand[conv..c) become (distance, id) keys, then cand[0..c) shrinks
+// to exactly the kk smallest keys.  Returns the new count; *thr receives the
+// selection threshold.
 template <int CAP>
 __device__ int compact(ScanCtl &C, uint64_t *cand, int c, int conv, int kk,
                        const int32_t *__restrict__ list_ids, uint64_t *thr) {
@@ -92,23 +139,23 @@ __device__ int compact(ScanCtl &C, uint64_t *cand, int c, int conv, int kk,
         const uint64_t k = cand[i];
         cand[i] = (k & 0xFFFFFFFF00000000ull) | uint32_t(list_ids[uint32_t(k)]);
     }
-    __syncthreads();
+    lds_barrier();
     if (c <= kk) return c;
-    SelectResult r = block_select<NT>([&](int i) { return cand[i]; }, c, kk, C.sel);
+    const SelectResultT<uint64_t> r = block_select_lds<NT, uint64_t>([&](int i) { return cand[i]; }, c, kk, C.sel);
     uint64_t mine[PER];
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-        int i = tid + e * NT;
+        const int i = tid + e * NT;
         mine[e] = i < c ? cand[i] : kNoKey;
     }
     if (tid == 0) {
         C.newcnt = 0;
         C.eqt = 0;
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-        uint64_t key = mine[e];
+        const uint64_t key = mine[e];
         if (key == kNoKey) continue;
         bool take = key < r.thr;
         if (!take && r.eq > 0 && key == r.thr) take = atomicAdd(&C.eqt, 1) < r.eq;
@@ -117,11 +164,9 @@ __device__ int compact(ScanCtl &C, uint64_t *cand, int c, int conv, int kk,
             if (pos < CAP) cand[pos] = key;
         }
     }
-    __syncthreads();
-    const int nc = min(C.newcnt, CAP);
-    __syncthreads();
+    lds_barrier();
     *thr = r.thr;
-    return nc;
+    return min(C.newcnt, CAP);
 }
 
 // Branch-free code loads: lanes past the end re-read the last vector (the
@@ -146,8 +191,11 @@ __device__ inline float lds_at(uint32_t addr) {
 }
 
 // ADC of U vectors against the LDS table at byte address tbase:
-// acc = 0f; acc = fl32(acc + T[s][c_s]), s ascending.  Wavefronts whose
-// vectors all lie past the list end (wave0 + u*NT >= len) skip the gathers.
+// acc = 0f; acc = fl32(acc + T[s][c_s]), s ascending.  No table entry is -0
+// (an entry is fl32(-2 dot + c) with c >= +0, and an exact zero sum rounds to
+// +0), so 0f + T[0][c_0] = T[0][c_0] and the sum starts from the first entry.
+// Wavefronts whose vectors all lie past the list end (wave0 + u*NT >= len)
+// skip the gathers.
 template <int M, int U>
 __device__ inline void adc_step(const uint32_t (&cw)[U][CodeVec<M>::W], uint32_t tbase, int wave_v0,
                                 int len, float (&acc)[U]) {
@@ -161,16 +209,12 @@ __device__ inline void adc_step(const uint32_t (&cw)[U][CodeVec<M>::W], uint32_t
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const uint32_t off4 = ((cw[u][w] >> (8 * b)) & 255u) << 2;
-                s = s + lds_at(tbase + off4 + uint32_t((w * 4 + b) * 1024));
+                const float t = lds_at(tbase + off4 + uint32_t((w * 4 + b) * 1024));
+                s = (w == 0 && b == 0) ? t : s + t;
             }
         }
         acc[u] = s;
     }
-}
-
-// Float whose order key is `kh` (+inf for keys at or past +inf).
-__device__ inline float key_hi_dist(uint32_t kh) {
-    return kh >= 0xFF800000u ? __builtin_huge_valf() : key_dist(uint64_t(kh) << 32);
 }
 
 // The probed lists of one workgroup, in coarse order, skipping invalid ids.
@@ -187,12 +231,16 @@ struct ProbeSeq {
     }
 };
 
+// rres: residual rows, row (q * wstride + p) * D; rnd: f64 sub-vector norms
+// of the same rows, row (q * wstride + p) * M.  Both are read at uniform
+// addresses (scalar loads).
 template <int M, int KM>
-__global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
+__global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a, const float *__restrict__ rres,
+                                                     const double *__restrict__ rnd) {
     constexpr int DS = D / M;
     constexpr int HM = M / 2;
     constexpr int CW = CodeVec<M>::W;
-    constexpr int U = 2;                 // vectors per thread per step
+    constexpr int U = 3;                 // vectors per thread per step
     constexpr int STEP = NT * U;
     constexpr int CAP = KM <= 128 ? 2048 : 4096;
     static_assert(CAP - STEP >= KM, "candidate buffer too small");
@@ -200,9 +248,6 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     float *tab = reinterpret_cast<float *>(smem + L::tab);  // written here, read through lds_at()
     uint64_t *cand = reinterpret_cast<uint64_t *>(smem + L::cand);
-    float *qv = reinterpret_cast<float *>(smem + L::qv);
-    float *resid = reinterpret_cast<float *>(smem + L::resid);
-    float *rnorm = reinterpret_cast<float *>(smem + L::rnorm);
     ScanCtl &C = *reinterpret_cast<ScanCtl *>(smem + L::ctl);
 
     const int tid = threadIdx.x;
@@ -234,7 +279,6 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
     const int h = __builtin_amdgcn_readfirstlane(tid >> 8);  // wave-uniform half index
     const bool jok = j < a.ks;  // table rows past ks are never indexed by valid codes
 
-    if (!a.resid_mode && tid < D) qv[tid] = a.queries[int64_t(q) * D + tid];
     if (tid < 3) C.inc[tid] = 0;
     int fill = 0;      // candidates held at the start of a step (uniform)
     int conv = 0;      // cand[0..conv) already carry ids
@@ -244,7 +288,7 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
     __syncthreads();
 
     ProbeSeq seq{a, q, p0, p1};
-    constexpr int RD = 3;  // register ring depth: loads run RD-1 steps ahead
+    constexpr int RD = 2;  // register ring depth: loads run RD-1 steps ahead
     uint32_t ring[RD][U][CW];
     for (;;) {
         // ---- pick up to TB probes
@@ -267,14 +311,10 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
         }
         if (nb == 0) break;
 
-        // ---- residuals of the batch, then the first code step of list 0
-        float cval[TB];
+        // residual rows of the batch (a missing second probe repeats the first)
+        int64_t row[TB];
 #pragma unroll
-        for (int t = 0; t < TB; ++t) {
-            const int64_t rrow = a.resid_mode ? (int64_t(q) * a.wstride + pp[t]) * D : int64_t(ll[t]) * D;
-            const float *rbase = a.resid_mode ? a.queries : a.coarse;
-            cval[t] = t < nb ? rbase[rrow + (tid & (D - 1))] : 0.0f;
-        }
+        for (int t = 0; t < TB; ++t) row[t] = int64_t(q) * a.wstride + pp[t < nb ? t : 0];
         // flat step sequence over the batch's lists: step k covers STEP
         // vectors of list 0 (k < ns0) or list 1; loads run RD-1 steps ahead
         const int ns0 = (len[0] + STEP - 1) / STEP;
@@ -285,69 +325,56 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
             const int t1 = kc >= ns0;
             load_step<M, U>(a.codes + beg[t1] * M, (kc - (t1 ? ns0 : 0)) * STEP, len[t1], dst);
         };
-#pragma unroll
-        for (int r = 0; r + 1 < RD; ++r) issue(r, ring[r]);
-        if (tid < D) {  // waves 0 and 1, whole wavefronts
-#pragma unroll
-            for (int t = 0; t < TB; ++t) {
-                const float r = a.resid_mode ? cval[t] : qv[tid] - cval[t];
-                resid[t * D + tid] = r;
-                // f32 norm of each sub-vector, summed in dimension order
-                const float sq = r * r;
-                const int g0 = (tid & 63) & ~(DS - 1);
-                float acc = 0.0f;
-#pragma unroll
-                for (int i = 0; i < DS; ++i) {
-                    const float v = __shfl(sq, g0 + i, 64);
-                    acc = acc + v;
-                }
-                if ((tid & (DS - 1)) == 0) rnorm[t * M + tid / DS] = acc;
-            }
-        }
-        __syncthreads();
 
         // ---- TB tables: thread (j, h) streams P[s][j] of its subquantizers
-        //      from L2 once for the whole batch
+        //      from L2 once for the whole batch, PD subquantizers ahead
         // lane offset kept opaque so the per-subquantizer addresses are not
         // hoisted out of the batch loop into long-lived 64-bit registers
         unsigned jj = jok ? unsigned(j) : 0u;
         asm volatile("" : "+v"(jj));
-#pragma unroll
-        for (int s2 = 0; s2 < HM; ++s2) {
+        float pr[PD + 1][DS];
+        double pn[PD + 1];
+        auto pload = [&](int s2) {
             const int s = h * HM + s2;
-            float pr[DS];
             const float *src = a.pq + size_t(s) * a.ks * DS;  // wave-uniform row base
 #pragma unroll
-            for (int i = 0; i < DS; ++i) pr[i] = src[jj * DS + i];
-            const double pnv = a.pq_norm[size_t(s) * a.ks + jj];
+            for (int i = 0; i < DS; ++i) pr[s2 % (PD + 1)][i] = src[jj * DS + i];
+            pn[s2 % (PD + 1)] = a.pq_norm[size_t(s) * a.ks + jj];
+        };
+#pragma unroll
+        for (int s2 = 0; s2 < PD && s2 < HM; ++s2) pload(s2);
+#pragma unroll
+        for (int s2 = 0; s2 < HM; ++s2) {
+            if (s2 + PD < HM) pload(s2 + PD);
+            const int s = h * HM + s2;
+            const float *p = pr[s2 % (PD + 1)];
 #pragma unroll
             for (int t = 0; t < TB; ++t) {
-                // residual sub-vector: broadcast LDS reads, 16 bytes at a time
-                float rr[DS];
-                const float4 *rv = reinterpret_cast<const float4 *>(resid + t * D + s * DS);
+                // residual sub-vector and its norm: uniform (scalar) loads
+                const float *rv = rres + row[t] * D + s * DS;
+                // dot = 0f; dot = fl32(dot + fl32(r_i p_i)).  Starting from
+                // the first product instead of 0f can only change the sign
+                // of an all-zero dot, which the entry below does not see:
+                // fl32(-2 (+-0) + c) = fl32(c) for c >= +0.
+                float dot = rv[0] * p[0];
 #pragma unroll
-                for (int i4 = 0; i4 < DS / 4; ++i4) {
-                    const float4 x = rv[i4];
-                    rr[4 * i4] = x.x;
-                    rr[4 * i4 + 1] = x.y;
-                    rr[4 * i4 + 2] = x.z;
-                    rr[4 * i4 + 3] = x.w;
-                }
-                float dot = 0.0f;
-#pragma unroll
-                for (int i = 0; i < DS; ++i) {
-                    const float prod = rr[i] * pr[i];
+                for (int i = 1; i < DS; ++i) {
+                    const float prod = rv[i] * p[i];
                     dot = dot + prod;
                 }
-                const float c = float(pnv + double(rnorm[t * M + s]));
+                const float c = float(pn[s2 % (PD + 1)] + rnd[row[t] * M + s]);
                 // fl32(fl32(-2*dot) + c): the product by -2 is exact, so one
                 // fused rounding gives the same bits
                 const float e = __builtin_fmaf(-2.0f, dot, c);
                 tab[(t * M + s) * 256 + j] = jok ? e : __builtin_huge_valf();
             }
-            __builtin_amdgcn_sched_barrier(0);  // one subquantizer's operands live at a time
+            __builtin_amdgcn_sched_barrier(0);  // PD + 1 subquantizers' operands live at a time
         }
-        __syncthreads();
+        // the first code steps are issued after the P loads: loads complete in
+        // order, so issued earlier they would hold up every table wait
+#pragma unroll
+        for (int r = 0; r + 1 < RD; ++r) issue(r, ring[r]);
+        lds_barrier();  // tables complete; the code loads stay in flight
 
         // ---- scan the batch's lists back to back; the prefetch runs one
         //      step ahead, across list boundaries
@@ -393,13 +420,21 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a) {
                 fill += C.inc[rot];
                 ++step;
                 if (fill > CAP - STEP) {
-                    fill = compact<CAP>(C, cand, fill, conv, kq, a.list_ids, &thr);
-                    conv = fill;
-                    thr_f = key_hi_dist(uint32_t(thr >> 32));
+                    // distance-only while no id is resolved; the exact form
+                    // when the kept set leaves no room for a step (many equal
+                    // distances) or ids already are
+                    const int nf = conv == 0 ? compact_dist<CAP>(C, cand, fill, kq, &thr_f) : CAP;
+                    if (nf > CAP - STEP) {
+                        fill = compact<CAP>(C, cand, conv == 0 ? nf : fill, conv, kq, a.list_ids, &thr);
+                        conv = fill;
+                        thr_f = key_hi_dist(uint32_t(thr >> 32));
+                    } else {
+                        fill = nf;
+                    }
                 }
             }
         }
-        __syncthreads();  // tables and residuals are rewritten by the next batch
+        lds_barrier();  // the next batch rewrites the tables
     }
 
     // final selection (also resolves the last positions to ids)
@@ -465,6 +500,41 @@ __global__ __launch_bounds__(QO_NT) void query_order_kernel(const int32_t *__res
     for (int q = tid; q < nq; q += QO_NT) order[atomicAdd(&hist[bucket(q)], 1)] = q;
 }
 
+// Residual rows r = fl32(q - c_l) (unless the caller passed residuals) and
+// their f32 sub-vector norms, summed in dimension order and stored as f64,
+// for every valid (query, probe) pair.  Two pairs per 256-thread block, one
+// dimension per thread.
+template <int M>
+__global__ __launch_bounds__(256) void resid_kernel(ScanArgs a, float *__restrict__ rres, double *__restrict__ rnd) {
+    constexpr int DS = D / M;
+    static_assert(DS <= 64, "a sub-vector must fit in one wavefront");
+    const int64_t pair = int64_t(blockIdx.x) * 2 + (threadIdx.x >> 7);
+    if (pair >= int64_t(a.nq) * a.wstride) return;  // whole wavefronts
+    const int q = int(pair / a.wstride);
+    const int p = int(pair - int64_t(q) * a.wstride);
+    if (p >= (a.wq ? a.wq[q] : a.w)) return;
+    const int l = a.probes[pair];
+    if (l < 0 || l >= a.nlist) return;
+    const int i = threadIdx.x & (D - 1);
+    float r;
+    if (a.resid_mode) {
+        r = a.queries[pair * D + i];
+    } else {
+        r = a.queries[int64_t(q) * D + i] - a.coarse[int64_t(l) * D + i];
+        rres[pair * D + i] = r;
+    }
+    // f32 norm of each sub-vector, summed in dimension order
+    const float sq = r * r;
+    const int g0 = (i & 63) & ~(DS - 1);
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < DS; ++t) {
+        const float v = __shfl(sq, g0 + t, 64);
+        acc = acc + v;
+    }
+    if ((i & (DS - 1)) == 0) rnd[pair * M + i / DS] = double(acc);
+}
+
 template <int M, int KM>
 hipError_t launch_mk(const ScanArgs &a, hipStream_t s) {
     constexpr int CAP = KM <= 128 ? 2048 : 4096;
@@ -477,7 +547,7 @@ hipError_t launch_mk(const ScanArgs &a, hipStream_t s) {
         attr_set = true;
     }
     const unsigned grid = unsigned(int64_t(a.nq) * a.ngroups);
-    scan_kernel<M, KM><<<grid, NT, lds, s>>>(a);
+    scan_kernel<M, KM><<<grid, NT, lds, s>>>(a, a.resid_mode ? a.queries : a.rres, a.rnd);
     return hipGetLastError();
 }
 
@@ -508,6 +578,18 @@ hipError_t launch_query_order(const int32_t *probes, int wstride, int nq, int nl
 hipError_t launch_scan(const ScanArgs &a, int m, hipStream_t s) {
     if (a.nq <= 0) return hipSuccess;
     if (a.kstride > 1024) return hipErrorInvalidValue;
+    if (!a.rnd || (!a.resid_mode && !a.rres)) return hipErrorInvalidValue;
+    const int64_t pairs = int64_t(a.nq) * a.wstride;
+    const unsigned rgrid = unsigned((pairs + 1) / 2);
+    switch (m) {
+        case 4: resid_kernel<4><<<rgrid, 256, 0, s>>>(a, a.rres, a.rnd); break;
+        case 8: resid_kernel<8><<<rgrid, 256, 0, s>>>(a, a.rres, a.rnd); break;
+        case 16: resid_kernel<16><<<rgrid, 256, 0, s>>>(a, a.rres, a.rnd); break;
+        case 32: resid_kernel<32><<<rgrid, 256, 0, s>>>(a, a.rres, a.rnd); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     switch (m) {
         case 4: return launch_m<4>(a, s);
         case 8: return launch_m<8>(a, s);
