@@ -399,20 +399,30 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a, const float *__
                 else
                     adc_step<M, U>(ring[r], uint32_t(L::tab), base + (tid & ~63), ln, accs);
                 // ties with the threshold distance pass too; ids are compared
-                // exactly at the next selection
+                // exactly at the final selection.  One LDS atomic per
+                // wavefront and step reserves the slots of all U vectors.
+                uint64_t mask[U];
+                int tot = 0;
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int v = base + u * NT + tid;
-                    const bool pass = (v < ln) && accs[u] <= thr_f;
-                    const uint64_t mask = __ballot(pass);
-                    if (mask) {
-                        const int lane = __lane_id();
-                        const int before = __popcll(mask & ((uint64_t(1) << lane) - 1));
-                        int pos = 0;
-                        if (lane == 0) pos = atomicAdd(&C.inc[rot], __popcll(mask));
-                        pos = fill + __builtin_amdgcn_readfirstlane(pos) + before;  // lane 0's count
-                        if (pass && pos < CAP)
-                            cand[pos] = (uint64_t(fkey(accs[u])) << 32) | uint32_t(bg + v);
+                    mask[u] = __ballot((v < ln) && accs[u] <= thr_f);
+                    tot += __popcll(mask[u]);
+                }
+                if (tot) {  // wave-uniform
+                    const int lane = __lane_id();
+                    const uint64_t below = (uint64_t(1) << lane) - 1;
+                    int pos = 0;
+                    if (lane == 0) pos = atomicAdd(&C.inc[rot], tot);
+                    pos = fill + __builtin_amdgcn_readfirstlane(pos);  // lane 0's reservation
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int at = pos + __popcll(mask[u] & below);
+                        if (((mask[u] >> lane) & 1) && at < CAP) {
+                            const int v = base + u * NT + tid;
+                            cand[at] = (uint64_t(fkey(accs[u])) << 32) | uint32_t(bg + v);
+                        }
+                        pos += __popcll(mask[u]);
                     }
                 }
                 // LDS-only: the code loads of the next steps stay in flight
@@ -448,7 +458,7 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a, const float *__
     const int np = next_pow2(c > 0 ? c : 1);
     for (int i = c + tid; i < np; i += NT) cand[i] = kNoKey;
     __syncthreads();
-    block_bitonic_sort<NT>(cand, np);
+    block_bitonic_sort<NT, true>(cand, np);
     int32_t *oi = a.out_ids + int64_t(q) * a.kstride;
     float *od = a.out_dis + int64_t(q) * a.kstride;
     for (int i = tid; i < a.kstride; i += NT) {

@@ -189,8 +189,9 @@ __device__ SelectResultT<K> block_select_lds(KeyFn keyfn, int n, int k, SelectSh
     return SelectResultT<K>{K(~K(0)), 0};  // not reached
 }
 
-// In-place ascending bitonic sort of s[0..n), n a power of two.
-template <int NT>
+// In-place ascending bitonic sort of s[0..n), n a power of two, in LDS.
+// LdsOnly: LDS-only barriers (the caller exchanges no global data around it).
+template <int NT, bool LdsOnly = false>
 __device__ void block_bitonic_sort(uint64_t *s, int n) {
     const int tid = threadIdx.x;
     for (int size = 2; size <= n; size <<= 1) {
@@ -205,7 +206,10 @@ __device__ void block_bitonic_sort(uint64_t *s, int n) {
                     s[hi] = x;
                 }
             }
-            __syncthreads();
+            if (LdsOnly)
+                lds_barrier();
+            else
+                __syncthreads();
         }
     }
 }
