@@ -271,10 +271,8 @@ int search_core(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, cons
         MIVF_TRY(ws.probes.ensure(sizeof(int32_t) * size_t(nq) * wmax));
         MIVF_TRY(ws.coadis.ensure(sizeof(float) * size_t(nq) * wmax));
         MIVF_TRY(coarse_stage(ix, ws, d_q, nq, wmax, ws.probes.as<int32_t>(), ws.coadis.as<float>(), st, &tm));
-        MIVF_TRY(tm.mark());
         probes = ws.probes.as<int32_t>();
     } else {
-        MIVF_TRY(tm.mark());
         MIVF_TRY(tm.mark());
     }
     int ppg = 0;
@@ -320,6 +318,9 @@ int search_core(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, cons
         a.out_ids = d_ids;
         a.out_dis = d_dis;
     }
+    // the select stage ends with the scan's order and residual rows
+    MIVF_CHECK_HIP(launch_scan_prologue(a, ix->m, st));
+    MIVF_TRY(tm.mark());
     MIVF_CHECK_HIP(launch_scan(a, ix->m, st));
     if (!ix->scan_checked) {  // once per index: the kernel's LDS-layout precondition
         int err = 0;

@@ -148,8 +148,10 @@ struct ScanArgs {
     float *rres;              // [nq*wstride][d] residual rows (written by the scan prologue; unused in resid_mode)
     double *rnd;              // [nq*wstride][m] f64 sub-vector norms of the residual rows
 };
-// Prologue (residual rows and their sub-vector norms of every valid
-// (query, probe) pair) followed by the fused table + ADC + top-k scan.
+// Prologue: residual rows and their sub-vector norms of every valid
+// (query, probe) pair (a.rres unless resid_mode, a.rnd).
+hipError_t launch_scan_prologue(const ScanArgs &a, int m, hipStream_t s);
+// Fused table + ADC + top-k scan; reads what the prologue wrote.
 hipError_t launch_scan(const ScanArgs &a, int m, hipStream_t s);
 // Queries ordered by their first probed list (counting sort; the order
 // inside a list is arbitrary).  Only the schedule of the scan depends on it.

@@ -512,37 +512,38 @@ __global__ __launch_bounds__(QO_NT) void query_order_kernel(const int32_t *__res
 
 // Residual rows r = fl32(q - c_l) (unless the caller passed residuals) and
 // their f32 sub-vector norms, summed in dimension order and stored as f64,
-// for every valid (query, probe) pair.  Two pairs per 256-thread block, one
-// dimension per thread.
+// for every valid (query, probe) pair.  One 256-thread block per query: each
+// half takes every second probe, one dimension per thread.
 template <int M>
 __global__ __launch_bounds__(256) void resid_kernel(ScanArgs a, float *__restrict__ rres, double *__restrict__ rnd) {
     constexpr int DS = D / M;
     static_assert(DS <= 64, "a sub-vector must fit in one wavefront");
-    const int64_t pair = int64_t(blockIdx.x) * 2 + (threadIdx.x >> 7);
-    if (pair >= int64_t(a.nq) * a.wstride) return;  // whole wavefronts
-    const int q = int(pair / a.wstride);
-    const int p = int(pair - int64_t(q) * a.wstride);
-    if (p >= (a.wq ? a.wq[q] : a.w)) return;
-    const int l = a.probes[pair];
-    if (l < 0 || l >= a.nlist) return;
+    const int q = blockIdx.x;
     const int i = threadIdx.x & (D - 1);
-    float r;
-    if (a.resid_mode) {
-        r = a.queries[pair * D + i];
-    } else {
-        r = a.queries[int64_t(q) * D + i] - a.coarse[int64_t(l) * D + i];
-        rres[pair * D + i] = r;
-    }
-    // f32 norm of each sub-vector, summed in dimension order
-    const float sq = r * r;
-    const int g0 = (i & 63) & ~(DS - 1);
-    float acc = 0.0f;
+    const int wq = a.wq ? a.wq[q] : a.w;
+    const float qi = a.resid_mode ? 0.0f : a.queries[int64_t(q) * D + i];
+    for (int p = threadIdx.x >> 7; p < wq; p += 2) {  // whole wavefronts
+        const int64_t pair = int64_t(q) * a.wstride + p;
+        const int l = a.probes[pair];
+        if (l < 0 || l >= a.nlist) continue;
+        float r;
+        if (a.resid_mode) {
+            r = a.queries[pair * D + i];
+        } else {
+            r = qi - a.coarse[int64_t(l) * D + i];
+            rres[pair * D + i] = r;
+        }
+        // f32 norm of each sub-vector, summed in dimension order
+        const float sq = r * r;
+        const int g0 = (i & 63) & ~(DS - 1);
+        float acc = 0.0f;
 #pragma unroll
-    for (int t = 0; t < DS; ++t) {
-        const float v = __shfl(sq, g0 + t, 64);
-        acc = acc + v;
+        for (int t = 0; t < DS; ++t) {
+            const float v = __shfl(sq, g0 + t, 64);
+            acc = acc + v;
+        }
+        if ((i & (DS - 1)) == 0) rnd[pair * M + i / DS] = double(acc);
     }
-    if ((i & (DS - 1)) == 0) rnd[pair * M + i / DS] = double(acc);
 }
 
 template <int M, int KM>
@@ -585,12 +586,10 @@ hipError_t launch_query_order(const int32_t *probes, int wstride, int nq, int nl
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const ScanArgs &a, int m, hipStream_t s) {
+hipError_t launch_scan_prologue(const ScanArgs &a, int m, hipStream_t s) {
     if (a.nq <= 0) return hipSuccess;
-    if (a.kstride > 1024) return hipErrorInvalidValue;
     if (!a.rnd || (!a.resid_mode && !a.rres)) return hipErrorInvalidValue;
-    const int64_t pairs = int64_t(a.nq) * a.wstride;
-    const unsigned rgrid = unsigned((pairs + 1) / 2);
+    const unsigned rgrid = unsigned(a.nq);
     switch (m) {
         case 4: resid_kernel<4><<<rgrid, 256, 0, s>>>(a, a.rres, a.rnd); break;
         case 8: resid_kernel<8><<<rgrid, 256, 0, s>>>(a, a.rres, a.rnd); break;
@@ -598,8 +597,13 @@ hipError_t launch_scan(const ScanArgs &a, int m, hipStream_t s) {
         case 32: resid_kernel<32><<<rgrid, 256, 0, s>>>(a, a.rres, a.rnd); break;
         default: return hipErrorInvalidValue;
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const ScanArgs &a, int m, hipStream_t s) {
+    if (a.nq <= 0) return hipSuccess;
+    if (a.kstride > 1024) return hipErrorInvalidValue;
+    if (!a.rnd || (!a.resid_mode && !a.rres)) return hipErrorInvalidValue;
     switch (m) {
         case 4: return launch_m<4>(a, s);
         case 8: return launch_m<8>(a, s);
