@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <memory>
 #include <string>
 #include <vector>
@@ -16,12 +17,23 @@ namespace mivf {
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    // When set before the first allocation: the only stream that uses the
+    // buffer; it then grows stream-ordered (hipFreeAsync / hipMallocAsync),
+    // without the device-wide wait of hipFree.
+    hipStream_t ord = nullptr;
+    bool async = false;
     int ensure(size_t want) {
         if (want <= bytes) return kOk;
-        if (p) hipFree(p);
+        if (p) {
+            if (async)
+                hipFreeAsync(p, ord);
+            else
+                hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
-        hipError_t e = hipMalloc(&p, want);
+        async = ord != nullptr;
+        hipError_t e = async ? hipMallocAsync(&p, want, ord) : hipMalloc(&p, want);
         if (e != hipSuccess) {
             set_error(std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
             return kNoMem;
@@ -66,6 +78,16 @@ struct Timer {
 // serving slot owns another, so searches on different streams can overlap.
 struct Ws {
     DevBuf qn, dmat, probes, coadis, partial, wq, kq, redo, tmp_ids, tmp_dis, qorder, rres, rnd;
+    // every buffer grows stream-ordered on s (the only stream these calls use)
+    void order_on(hipStream_t s) {
+        for (DevBuf *b : all()) b->ord = s;
+    }
+    void release() {
+        for (DevBuf *b : all()) b->release();
+    }
+    std::array<DevBuf *, 13> all() {
+        return {&qn, &dmat, &probes, &coadis, &partial, &wq, &kq, &redo, &tmp_ids, &tmp_dis, &qorder, &rres, &rnd};
+    }
 };
 
 }  // namespace mivf

@@ -69,6 +69,8 @@ struct mivf_server {
     ~mivf_server() {
         for (auto &s : slots) {
             if (s.st) hipStreamSynchronize(s.st);
+            s.ws.release();
+            for (DevBuf *b : {&s.q, &s.ids, &s.dis, &s.dw, &s.dk}) b->release();
             if (s.ev) hipEventDestroy(s.ev);
             if (s.st) hipStreamDestroy(s.st);
             if (s.h_wk) hipHostFree(s.h_wk);
@@ -79,10 +81,11 @@ struct mivf_server {
 namespace {
 
 // Device state a request holds while in flight: its share of the coarse
-// distance matrix, probes, partial top-k lists and the slot's query/output rows.
+// distance matrix, probes, residual rows, partial top-k lists and the slot's
+// query/output rows.
 int64_t request_mem(const mivf_index *ix, int nq, int w, int k) {
     const int64_t per_q = int64_t(ix->nlist) * 4 + int64_t(w) * 8 + 8 + int64_t(ix->d) * 4 +
-                          int64_t(k) * 8 * (1 + 8);
+                          int64_t(k) * 8 * (1 + 8) + int64_t(w) * (int64_t(ix->d) * 4 + int64_t(ix->m) * 8);
     return per_q * nq;
 }
 
@@ -201,6 +204,10 @@ int mivf_server_create(mivf_index *ix, int slots, int max_batch, int64_t mem_bud
             set_error("server_create: stream/event creation failed");
             return kHip;
         }
+        // a slot's buffers are used on its stream only: they grow stream-ordered,
+        // so a larger batch never waits for the other slots' work
+        s.ws.order_on(s.st);
+        for (DevBuf *b : {&s.q, &s.ids, &s.dis, &s.dw, &s.dk}) b->ord = s.st;
     }
     *out = sv;
     return kOk;
