@@ -816,9 +816,11 @@ __device__ float seed_table(const float *__restrict__ queries, const float *__re
     return mg;
 }
 
-constexpr int SCAP = 8192;  // seed keys held in LDS
+constexpr int SCAP = 8192;  // seed keys held in LDS (at most)
 
-template <int M>
+// SC: keys held in LDS.  A capped seed (cap <= 1024) runs with SC = 1024,
+// a third of the LDS, so that several more blocks share a CU.
+template <int M, int SC>
 __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__ queries,
                                                        const float *__restrict__ coarse,
                                                        const float *__restrict__ pq, const float *__restrict__ pnorm,
@@ -833,7 +835,7 @@ __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__
     __shared__ float r[D];
     __shared__ float As[M];
     __shared__ uint32_t hist[256];
-    __shared__ uint32_t skeys[SCAP];
+    __shared__ uint32_t skeys[SC];
     __shared__ int sdig, sbelow;
     const int64_t q = blockIdx.x;
     const int tid = threadIdx.x;
@@ -853,7 +855,7 @@ __global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__
         if (tid == 0) tau[q] = __builtin_huge_valf();
         return;
     }
-    const bool inl = tot <= SCAP;  // keys computed once, kept in LDS
+    const bool inl = tot <= SC;  // keys computed once, kept in LDS
     float marg = 0.0f;
     if (inl) {
         int off = 0;
@@ -1144,22 +1146,23 @@ hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *ran
     return hipGetLastError();
 }
 
+template <int M, int SC>
+hipError_t launch_seed_m(const SeedArgs &a, hipStream_t s) {
+    seed_tau_kernel<M, SC><<<a.nq, 256, 0, s>>>(a.queries, a.coarse, a.pq, a.pnorm, a.pmax, a.list_off, a.codes,
+                                                 a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.klq, a.cap, a.tau);
+    return hipGetLastError();
+}
+
 hipError_t launch_seed_tau(const SeedArgs &a, int m, hipStream_t s) {
     if (a.nq <= 0) return hipSuccess;
     if (a.kl < 1) return hipErrorInvalidValue;
     if (a.cap > 0 && (a.klq || a.cap < a.kl || a.cap > SCAP)) return hipErrorInvalidValue;
+    const bool small = a.cap > 0 && a.cap <= 1024;
     switch (m) {
-        case 8:
-            seed_tau_kernel<8><<<a.nq, 256, 0, s>>>(a.queries, a.coarse, a.pq, a.pnorm, a.pmax, a.list_off, a.codes,
-                                                     a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.klq, a.cap, a.tau);
-            break;
-        case 16:
-            seed_tau_kernel<16><<<a.nq, 256, 0, s>>>(a.queries, a.coarse, a.pq, a.pnorm, a.pmax, a.list_off, a.codes,
-                                                      a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.klq, a.cap, a.tau);
-            break;
+        case 8: return small ? launch_seed_m<8, 1024>(a, s) : launch_seed_m<8, SCAP>(a, s);
+        case 16: return small ? launch_seed_m<16, 1024>(a, s) : launch_seed_m<16, SCAP>(a, s);
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_fast_scan(const FastArgs &a, int m, int grid, hipStream_t s) {

@@ -859,12 +859,13 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_CHECK_HIP(launch_fast_select(kl, ix->f_cand.as<uint64_t>(), ix->f_ccnt.as<int>(), capq, nq,
                                       ix->f_short.as<int32_t>(), st));
     MIVF_TRY(tm.mark());
+    // the coarse stage left the queries' f64 norms in ws.qn
     if (d_base)
         MIVF_CHECK_HIP(launch_rerank_pos(d_base, nbase, false, ix->ids.as<int32_t>(), d_q, nq, d,
-                                         ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
+                                         ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
     else
         MIVF_CHECK_HIP(launch_rerank_pos(ix->rows.as<float>(), ix->ntotal, true, ix->ids.as<int32_t>(), d_q, nq, d,
-                                         ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
+                                         ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
     MIVF_TRY(tm.mark());
     MIVF_TRY(finish_timer(ix, tm));
     if (!ix->scan_checked || ix->profile) {

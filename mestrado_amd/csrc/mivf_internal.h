@@ -263,10 +263,11 @@ hipError_t launch_rerank(const float *base, int64_t nbase, const float *queries,
                          int32_t *out_ids, float *out_dis, hipStream_t s);
 
 // Same with a shortlist of list positions (id = list_ids[pos]); rows are
-// base[id], or base[pos] of a list-ordered copy when list_ordered.
+// base[id], or base[pos] of a list-ordered copy when list_ordered.  qnorm:
+// the queries' f64-accumulated norms (launch_norm_f64), or nullptr.
 hipError_t launch_rerank_pos(const float *rows, int64_t nrows, bool list_ordered, const int32_t *list_ids,
-                             const float *queries, int nq, int d, const int32_t *short_pos, int kl, int k,
-                             int32_t *out_ids, float *out_dis, hipStream_t s);
+                             const float *queries, int nq, int d, const double *qnorm, const int32_t *short_pos,
+                             int kl, int k, int32_t *out_ids, float *out_dis, hipStream_t s);
 // out[r] = base[ids[r]] (rows of d floats)
 hipError_t launch_gather_rows(const float *base, int d, const int32_t *ids, int64_t n, float *out, hipStream_t s);
 

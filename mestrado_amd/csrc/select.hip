@@ -89,11 +89,14 @@ __global__ __launch_bounds__(NT) void merge_kernel(const uint64_t *__restrict__ 
 // as knn_full(2, 1, ktmp, d, k, nearest, query) in sequential/ivf_pq/ivf_search.cpp:130.
 // MODE 0: the shortlist holds ids, rows base[id].  MODE 1: it holds list
 // positions, id = list_ids[pos], rows base[id].  MODE 2: list positions, rows
-// from the list-ordered copy base[pos] (nbase = its row count).
+// from the list-ordered copy base[pos] (nbase = its row count).  qnorm: the
+// queries' f64-accumulated norms when the caller has them (the same sum as
+// below), else nullptr.
 template <int MODE>
 __global__ __launch_bounds__(NT) void rerank_kernel(const float *__restrict__ base, int64_t nbase,
                                                     const int32_t *__restrict__ list_ids,
                                                     const float *__restrict__ queries, int d,
+                                                    const double *__restrict__ qnorm,
                                                     const int32_t *__restrict__ short_ids,
                                                     int kl, int k, int32_t *__restrict__ out_ids,
                                                     float *__restrict__ out_dis) {
@@ -108,12 +111,16 @@ __global__ __launch_bounds__(NT) void rerank_kernel(const float *__restrict__ ba
     for (int t = tid; t < d; t += NT) qv[t] = queries[int64_t(q) * d + t];
     __syncthreads();
     if (tid == 0) {
-        double s = 0.0;
-        for (int t = 0; t < d; ++t) {
-            float p = qv[t] * qv[t];
-            s = s + double(p);
+        if (qnorm) {
+            qn = qnorm[q];
+        } else {
+            double s = 0.0;
+            for (int t = 0; t < d; ++t) {
+                float p = qv[t] * qv[t];
+                s = s + double(p);
+            }
+            qn = s;
         }
-        qn = s;
     }
     __syncthreads();
     const double nb = qn;
@@ -200,22 +207,22 @@ hipError_t launch_rerank(const float *base, int64_t nbase, const float *queries,
                          int32_t *out_ids, float *out_dis, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     if (kl < 1 || kl > KBUF || k < 1 || k > KBUF || d > 512) return hipErrorInvalidValue;
-    rerank_kernel<0><<<unsigned(nq), NT, 0, s>>>(base, nbase, nullptr, queries, d, short_ids, kl, k, out_ids,
-                                                 out_dis);
+    rerank_kernel<0><<<unsigned(nq), NT, 0, s>>>(base, nbase, nullptr, queries, d, nullptr, short_ids, kl, k,
+                                                 out_ids, out_dis);
     return hipGetLastError();
 }
 
 hipError_t launch_rerank_pos(const float *rows, int64_t nrows, bool list_ordered, const int32_t *list_ids,
-                             const float *queries, int nq, int d, const int32_t *short_pos, int kl, int k,
-                             int32_t *out_ids, float *out_dis, hipStream_t s) {
+                             const float *queries, int nq, int d, const double *qnorm, const int32_t *short_pos,
+                             int kl, int k, int32_t *out_ids, float *out_dis, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     if (kl < 1 || kl > KBUF || k < 1 || k > KBUF || d > 512) return hipErrorInvalidValue;
     if (list_ordered)
-        rerank_kernel<2><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, short_pos, kl, k, out_ids,
-                                                     out_dis);
+        rerank_kernel<2><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm, short_pos, kl, k,
+                                                     out_ids, out_dis);
     else
-        rerank_kernel<1><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, short_pos, kl, k, out_ids,
-                                                     out_dis);
+        rerank_kernel<1><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm, short_pos, kl, k,
+                                                     out_ids, out_dis);
     return hipGetLastError();
 }
 
