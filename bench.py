@@ -35,7 +35,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=8, help="timed passes of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gt-queries", type=int, default=10000, help="queries with an FP64 ground truth")
-    ap.add_argument("--fast-kl", type=int, default=128, help="shortlist of the reduced-precision form")
+    ap.add_argument("--fast-kl", type=int, default=0, help="shortlist of the reduced-precision form (0: sweep --fast-kls)")
+    ap.add_argument("--fast-kls", default="128,160,176,192", help="shortlist sizes of the reduced-precision sweep")
     ap.add_argument("--fast-seed", type=int, default=1, help="lists scanned exactly to seed its filter")
     ap.add_argument("--fast-seed-cap", type=int, default=-1,
                     help="vectors entering the seed bound (0 = whole seed lists, -1 = library default)")
@@ -146,9 +147,12 @@ def main():
     fast = None
     if not args.no_fast:
         # The reduced-precision form is reported at the plain form's recall:
-        # probes w' in a short sweep, the fastest whose recall@100 is within
-        # 0.5 points of the plain form's (or the --fast-w given).
-        kl = args.fast_kl
+        # probes w' and shortlist sizes kl in a short sweep, the fastest
+        # whose recall@100 is within 0.5 points of the plain form's (or the
+        # --fast-w / --fast-kl given).
+        kls = [args.fast_kl] if args.fast_kl else [int(x) for x in args.fast_kls.split(",")]
+        if args.profile_only:
+            kls = kls[:1]
         f_ids = M.DeviceArray((nq, k), np.int32)
         f_dis = M.DeviceArray((nq, k), np.float32)
         ix.set_option("fast_seed_lists", args.fast_seed)
@@ -160,9 +164,9 @@ def main():
         elif args.profile_only:
             sweep_w = [w]
         else:
-            sweep_w = sorted({max(1, w - d) for d in (0, 4, 6, 8, 9, 10, 12)}, reverse=True)
+            sweep_w = sorted({max(1, w - d) for d in (0, 4, 8, 10, 11, 12, 13, 14, 15, 16)}, reverse=True)
         sweep = []
-        for fw in sweep_w:
+        for kl, fw in [(kl, fw) for kl in kls for fw in sweep_w]:
             for _ in range(args.warmup):
                 ix.search_fast(query, fw, kl, k, out=(f_ids, f_dis))
             M.synchronize()
@@ -186,11 +190,11 @@ def main():
                                zip(("coarse", "seed", "pairs", "scan8", "select", "rerank"), fstage)}}
             f.update(recall(f_ids.numpy(), f_dis.numpy()))
             sweep.append(f)
-            log(f"fast form w={fw}: qps {f['qps']} recall@100 {f.get('recall@100')}")
+            log(f"fast form w={fw} kl={kl}: qps {f['qps']} recall@100 {f.get('recall@100')}")
         ok = [f for f in sweep if f.get("recall@100", 0.0) >= rec.get("recall@100", 0.0) - 0.5]
         fast = dict(max(ok, key=lambda f: f["qps"]) if ok else sweep[0])
-        fast["recall_rule"] = "fastest probe count whose recall@100 is within 0.5 points of the plain form's"
-        fast["sweep"] = [{"w": f["w"], "qps": f["qps"], "recall@100": f.get("recall@100")} for f in sweep]
+        fast["recall_rule"] = "fastest (probes, shortlist) whose recall@100 is within 0.5 points of the plain form's"
+        fast["sweep"] = [{"w": f["w"], "kl": f["kl"], "qps": f["qps"], "recall@100": f.get("recall@100")} for f in sweep]
         log(f"fast form: {fast}")
     uneven = None
     if not args.no_uneven:
