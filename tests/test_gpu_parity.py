@@ -191,6 +191,33 @@ def test_ties_and_empty_lists(mivf, orc, c1):
     assert (g[0][:, -1] == -1).all() and np.isinf(g[1][:, -1]).all()
 
 
+def test_ties_long_lists(mivf, orc, c1):
+    """Long lists of one repeated code: thousands of candidates at exactly
+    the k-th distance, more than the scan's candidate buffer keeps after a
+    distance-only shrink, so it falls back to the exact (distance, id)
+    selection; lower ids win."""
+    q = c1["gold_q"]
+    nlist = q["coarse"].shape[0]
+    rng = np.random.default_rng(12)
+    sizes = rng.integers(0, 20, nlist)
+    sizes[rng.choice(nlist, 8, replace=False)] += 3000
+    off = np.zeros(nlist + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    n = int(off[-1])
+    ids = rng.permutation(n).astype(np.int32)
+    codes = np.zeros((n, 8), np.uint8)  # one code for most vectors: equal distances within a list
+    few = rng.random(n) < 0.05
+    codes[few] = rng.integers(0, 256, (int(few.sum()), 8)).astype(np.uint8)
+    ix = mivf.Index(q["coarse"], q["pq"])
+    ix.set_lists(off, ids, codes)
+    index = (q["coarse"], q["pq"], off, ids, codes)
+    query = c1["query"][:20]
+    for w, k in ((nlist, 10), (nlist, 100), (64, 1000)):
+        g = ix.search(query, w, k)
+        r = orc.search(index, query, w, k)
+        assert_same_results(g[0], g[1], r[0], r[1], f"long ties w={w} k={k}")
+
+
 def test_int32_codes_upload(mivf, orc, c1, c1_index):
     """Reference layout (int32 per subquantizer) uploads to the same index."""
     q = c1["gold_q"]
