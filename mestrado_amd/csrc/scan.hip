@@ -65,6 +65,7 @@ struct ScanCtl {
     int newcnt;
     int eqt;
     uint32_t maxhi;
+    uint32_t seed;  // first-step threshold key (max over the wavefronts)
 };
 
 constexpr int TB = 2;  // tables built per phase (probes scanned per phase)
@@ -280,6 +281,7 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a, const float *__
     const bool jok = j < a.ks;  // table rows past ks are never indexed by valid codes
 
     if (tid < 3) C.inc[tid] = 0;
+    if (tid == 0) C.seed = 0u;
     int fill = 0;      // candidates held at the start of a step (uniform)
     int conv = 0;      // cand[0..conv) already carry ids
     int step = 0;      // global step counter (uniform)
@@ -398,6 +400,30 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a, const float *__
                     adc_step<M, U>(ring[r], uint32_t(L::tab) + uint32_t(M * 256 * 4), base + (tid & ~63), ln, accs);
                 else
                     adc_step<M, U>(ring[r], uint32_t(L::tab), base + (tid & ~63), ln, accs);
+                // First step of the workgroup, threshold still +inf: every
+                // wavefront finds the ceil(kq / 8)-th smallest of its own
+                // distances by a bitwise search over ballot counts (no LDS,
+                // no selection pass); the largest of the eight bounds the
+                // kq-th smallest distance of the step, so only the distances
+                // within it are appended, instead of the whole step.
+                if (step == 0) {  // workgroup-uniform
+                    const int need = (kq + NT / 64 - 1) / (NT / 64);
+                    uint32_t key[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        key[u] = base + u * NT + tid < ln ? fkey(accs[u]) : 0xFFFFFFFFu;
+                    uint32_t ans = 0u;  // need-th smallest key of the wavefront (all ones: fewer keys)
+                    for (int bit = 31; bit >= 0; --bit) {
+                        const uint32_t t = ans | ((1u << bit) - 1u);
+                        int cnt = 0;
+#pragma unroll
+                        for (int u = 0; u < U; ++u) cnt += __popcll(__ballot(key[u] <= t));
+                        if (cnt < need) ans |= 1u << bit;
+                    }
+                    if (__lane_id() == 0) atomicMax(&C.seed, ans);
+                    lds_barrier();
+                    thr_f = key_hi_dist(C.seed);
+                }
                 // ties with the threshold distance pass too; ids are compared
                 // exactly at the final selection.  One LDS atomic per
                 // wavefront and step reserves the slots of all U vectors.
