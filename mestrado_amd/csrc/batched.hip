@@ -211,7 +211,7 @@ int autotune(mivf_index *ix, XWs &x, const float *d_q, int nq, int wmax, int kma
             if (rc != kOk) break;
             if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = kHip; break; }
             float ms = 0;
-            hipEventElapsedTime(&ms, e0, e1);
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = kHip; break; }
             if (rep > 0) tot += ms;
         }
         if (rc != kOk) break;
@@ -220,8 +220,8 @@ int autotune(mivf_index *ix, XWs &x, const float *d_q, int nq, int wmax, int kma
             b = c;
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc != kOk) return rc;
     *best = b;
     if (best_ms) *best_ms = bms;
@@ -246,11 +246,11 @@ struct mivf_plan {
     hipGraphExec_t exec = nullptr;
     float tuned_ms = 0;
     ~mivf_plan() {
-        if (st) hipStreamSynchronize(st);
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
-        if (st) hipStreamDestroy(st);
-        if (h_wk) hipHostFree(h_wk);
+        if (st) (void)hipStreamSynchronize(st);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (st) (void)hipStreamDestroy(st);
+        if (h_wk) (void)hipHostFree(h_wk);
     }
 };
 

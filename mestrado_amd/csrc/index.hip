@@ -562,7 +562,7 @@ int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, cons
     {
         int dev = 0;
         if (hipGetDevice(&dev) == hipSuccess)
-            hipDeviceGetAttribute(&ix->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipDeviceGetAttribute(&ix->ncu, hipDeviceAttributeMultiprocessorCount, dev);  // else the default below
         if (ix->ncu <= 0) ix->ncu = 256;
     }
     if ((rc = upload_lists(ix))) return fail(rc);
@@ -572,7 +572,7 @@ int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, cons
 
 int mivf_index_free(mivf_index *ix) {
     if (!ix) return kOk;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();  // buffers may still be in use by queued work
     delete ix;
     return kOk;
 }
@@ -733,8 +733,8 @@ int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
         float ms = 0;
         MIVF_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
         ix->timings[4] = ms;
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     }
     return kOk;
 }

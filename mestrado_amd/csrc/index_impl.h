@@ -26,9 +26,9 @@ struct DevBuf {
         if (want <= bytes) return kOk;
         if (p) {
             if (async)
-                hipFreeAsync(p, ord);
+                (void)hipFreeAsync(p, ord);
             else
-                hipFree(p);
+                (void)hipFree(p);
         }
         p = nullptr;
         bytes = 0;
@@ -44,7 +44,7 @@ struct DevBuf {
     template <typename T>
     T *as() const { return static_cast<T *>(p); }
     void release() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
     }
@@ -141,10 +141,10 @@ struct mivf_index {
     hipEvent_t aux_ev[2] = {};
     ~mivf_index() {
         for (auto &e : ev)
-            if (e) hipEventDestroy(e);
+            if (e) (void)hipEventDestroy(e);
         for (auto &e : aux_ev)
-            if (e) hipEventDestroy(e);
-        if (aux) hipStreamDestroy(aux);
+            if (e) (void)hipEventDestroy(e);
+        if (aux) (void)hipStreamDestroy(aux);
     }
 };
 

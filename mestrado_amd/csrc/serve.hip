@@ -68,12 +68,12 @@ struct mivf_server {
     int64_t batches = 0;
     ~mivf_server() {
         for (auto &s : slots) {
-            if (s.st) hipStreamSynchronize(s.st);
+            if (s.st) (void)hipStreamSynchronize(s.st);
             s.ws.release();
             for (DevBuf *b : {&s.q, &s.ids, &s.dis, &s.dw, &s.dk}) b->release();
-            if (s.ev) hipEventDestroy(s.ev);
-            if (s.st) hipStreamDestroy(s.st);
-            if (s.h_wk) hipHostFree(s.h_wk);
+            if (s.ev) (void)hipEventDestroy(s.ev);
+            if (s.st) (void)hipStreamDestroy(s.st);
+            if (s.h_wk) (void)hipHostFree(s.h_wk);
         }
     }
 };
