@@ -473,7 +473,13 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a, const float *__
         lds_barrier();  // the next batch rewrites the tables
     }
 
-    // final selection (also resolves the last positions to ids)
+    // final selection: a distance-only shrink first (no id loads), then the
+    // remaining positions become ids and, if equal distances leave more than
+    // kq, the exact (distance, id) selection
+    if (conv == 0 && fill > kq) {
+        const int nf = compact_dist<CAP>(C, cand, fill, kq, &thr_f);
+        fill = nf;
+    }
     fill = compact<CAP>(C, cand, fill, conv, kq, a.list_ids, &thr);
     const int c = fill;
     if (a.ngroups > 1) {
@@ -481,12 +487,32 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a, const float *__
         for (int i = tid; i < a.kstride; i += NT) dst[i] = i < c ? cand[i] : kNoKey;
         return;
     }
-    const int np = next_pow2(c > 0 ? c : 1);
-    for (int i = c + tid; i < np; i += NT) cand[i] = kNoKey;
-    __syncthreads();
-    block_bitonic_sort<NT, true>(cand, np);
     int32_t *oi = a.out_ids + int64_t(q) * a.kstride;
     float *od = a.out_dis + int64_t(q) * a.kstride;
+    if (c <= NT) {
+        // key i goes to its rank: the count of smaller keys, plus equal keys
+        // before it (equal keys only come from repeated ids); every thread
+        // reads the keys in the same order
+        if (tid < c) {
+            const uint64_t key = cand[tid];
+            int rank = 0;
+            for (int j = 0; j < c; ++j) {
+                const uint64_t o = cand[j];
+                rank += o < key || (o == key && j < tid);
+            }
+            oi[rank] = key_id(key);
+            od[rank] = key_dist(key);
+        }
+        for (int i = c + tid; i < a.kstride; i += NT) {
+            oi[i] = -1;
+            od[i] = __builtin_huge_valf();
+        }
+        return;
+    }
+    const int np = next_pow2(c > 0 ? c : 1);
+    for (int i = c + tid; i < np; i += NT) cand[i] = kNoKey;
+    lds_barrier();
+    block_bitonic_sort<NT, true>(cand, np);
     for (int i = tid; i < a.kstride; i += NT) {
         if (i < c) {
             oi[i] = key_id(cand[i]);
