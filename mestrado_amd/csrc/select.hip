@@ -33,8 +33,30 @@ __device__ int collect(KeyFn keyfn, int n, SelectResult r, uint64_t *buf, int *c
     return min(*cnt, KBUF);
 }
 
+// buf[0..c) holds the selected keys (complete: the caller's barrier).
 __device__ void sort_and_emit(uint64_t *buf, int c, int k, int32_t *idx, float *dis) {
     const int tid = threadIdx.x;
+    if (c <= NT) {
+        // key i goes to its rank: the count of smaller keys, plus equal keys
+        // before it; every thread reads the keys in the same order
+        if (tid < c) {
+            const uint64_t key = buf[tid];
+            int rank = 0;
+            for (int j = 0; j < c; ++j) {
+                const uint64_t o = buf[j];
+                rank += o < key || (o == key && j < tid);
+            }
+            if (rank < k) {
+                idx[rank] = key_id(key);
+                dis[rank] = key_dist(key);
+            }
+        }
+        for (int i = c + tid; i < k; i += NT) {
+            idx[i] = -1;
+            dis[i] = __builtin_huge_valf();
+        }
+        return;
+    }
     const int np = next_pow2(c > 0 ? c : 1);
     for (int i = c + tid; i < np; i += NT) buf[i] = kNoKey;
     __syncthreads();
