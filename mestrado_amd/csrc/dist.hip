@@ -165,10 +165,11 @@ hipError_t launch_norm_f32(const float *x, int64_t n, int d, int ldx, float *out
     return hipGetLastError();
 }
 
-// Same sums, rows staged through LDS: 64 rows per workgroup, 32 dimensions
-// per chunk loaded coalesced by all 256 threads, then each of 64 threads
-// continues its row's sequential f64 sum over the chunk (t ascending).
-constexpr int NR = 64, NCH = 32;
+// Same sums, rows staged through LDS: 16 rows per workgroup (a 10k-query
+// batch spreads over 625 workgroups), 32 dimensions per chunk loaded
+// coalesced by all 256 threads, then each of 16 threads continues its row's
+// sequential f64 sum over the chunk (t ascending).
+constexpr int NR = 16, NCH = 32;
 __global__ __launch_bounds__(256) void norm_f64_rows_kernel(const float *__restrict__ x, int64_t n, int d, int ldx,
                                                             double *__restrict__ out) {
     __shared__ float tile[NR][NCH + 1];
