@@ -565,36 +565,56 @@ __global__ __launch_bounds__(QO_NT) void query_order_kernel(const int32_t *__res
 // Residual rows r = fl32(q - c_l) (unless the caller passed residuals) and
 // their f32 sub-vector norms, summed in dimension order and stored as f64,
 // for every valid (query, probe) pair.  One 256-thread block per query: each
-// half takes every second probe, one dimension per thread.
+// half takes every second probe, one dimension per thread, eight probes'
+// loads at a time.
 template <int M>
 __global__ __launch_bounds__(256) void resid_kernel(ScanArgs a, float *__restrict__ rres, double *__restrict__ rnd) {
     constexpr int DS = D / M;
+    constexpr int RB = 8;  // probes per half and batch: their loads are issued together
     static_assert(DS <= 64, "a sub-vector must fit in one wavefront");
     const int q = blockIdx.x;
     const int i = threadIdx.x & (D - 1);
+    const int half = threadIdx.x >> 7;
     const int wq = a.wq ? a.wq[q] : a.w;
     const float qi = a.resid_mode ? 0.0f : a.queries[int64_t(q) * D + i];
-    for (int p = threadIdx.x >> 7; p < wq; p += 2) {  // whole wavefronts
-        const int64_t pair = int64_t(q) * a.wstride + p;
-        const int l = a.probes[pair];
-        if (l < 0 || l >= a.nlist) continue;
-        float r;
-        if (a.resid_mode) {
-            r = a.queries[pair * D + i];
-        } else {
-            r = qi - a.coarse[int64_t(l) * D + i];
-            rres[pair * D + i] = r;
-        }
-        // f32 norm of each sub-vector, summed in dimension order
-        const float sq = r * r;
-        const int g0 = (i & 63) & ~(DS - 1);
-        float acc = 0.0f;
+    for (int p0 = half; p0 < wq; p0 += 2 * RB) {  // whole wavefronts
+        int lid[RB];
+        float val[RB];
 #pragma unroll
-        for (int t = 0; t < DS; ++t) {
-            const float v = __shfl(sq, g0 + t, 64);
-            acc = acc + v;
+        for (int b = 0; b < RB; ++b) {
+            const int p = p0 + 2 * b;
+            lid[b] = p < wq ? a.probes[int64_t(q) * a.wstride + p] : -1;
         }
-        if ((i & (DS - 1)) == 0) rnd[pair * M + i / DS] = double(acc);
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            const int p = p0 + 2 * b;
+            const bool ok = lid[b] >= 0 && lid[b] < a.nlist;
+            val[b] = !ok ? 0.0f
+                         : (a.resid_mode ? a.queries[(int64_t(q) * a.wstride + p) * D + i]
+                                         : a.coarse[int64_t(lid[b]) * D + i]);
+        }
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            if (lid[b] < 0 || lid[b] >= a.nlist) continue;  // wave-uniform
+            const int64_t pair = int64_t(q) * a.wstride + p0 + 2 * b;
+            float r;
+            if (a.resid_mode) {
+                r = val[b];
+            } else {
+                r = qi - val[b];
+                rres[pair * D + i] = r;
+            }
+            // f32 norm of each sub-vector, summed in dimension order
+            const float sq = r * r;
+            const int g0 = (i & 63) & ~(DS - 1);
+            float acc = 0.0f;
+#pragma unroll
+            for (int t = 0; t < DS; ++t) {
+                const float v = __shfl(sq, g0 + t, 64);
+                acc = acc + v;
+            }
+            if ((i & (DS - 1)) == 0) rnd[pair * M + i / DS] = double(acc);
+        }
     }
 }
 
