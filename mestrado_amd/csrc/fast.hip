@@ -976,11 +976,30 @@ __global__ __launch_bounds__(256) void fast_select_kernel(int kl, const uint64_t
         for (int i = tid; i < kl; i += 256) out[i] = i < n ? key_id(C[i]) : -1;
         return;
     }
+    // up to 2048 candidates are read once, into registers; more are re-read
+    constexpr int FR = 8;
+    const bool inreg = n <= 256 * FR;  // uniform
+    uint64_t rk[FR];
+#pragma unroll
+    for (int e = 0; e < FR; ++e) {
+        const int i = tid + 256 * e;
+        rk[e] = inreg && i < n ? C[i] : kNoKey;
+    }
     uint32_t mn = 0xFFFFFFFFu, mx = 0u;
-    for (int i = tid; i < n; i += 256) {
-        const uint32_t h = uint32_t(C[i] >> 32);
-        mn = min(mn, h);
-        mx = max(mx, h);
+    if (inreg) {
+#pragma unroll
+        for (int e = 0; e < FR; ++e) {
+            if (rk[e] == kNoKey) continue;
+            const uint32_t h = uint32_t(rk[e] >> 32);
+            mn = min(mn, h);
+            mx = max(mx, h);
+        }
+    } else {
+        for (int i = tid; i < n; i += 256) {
+            const uint32_t h = uint32_t(C[i] >> 32);
+            mn = min(mn, h);
+            mx = max(mx, h);
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -1003,7 +1022,13 @@ __global__ __launch_bounds__(256) void fast_select_kernel(int kl, const uint64_t
     // any monotone map of the distance bits to bins keeps the order
     const float scale = 256.0f / (float(mx - mn) + 1.0f);
     auto bin = [&](uint64_t k) { return min(255, int(float(uint32_t(k >> 32) - mn) * scale)); };
-    for (int i = tid; i < n; i += 256) atomicAdd(&hist[bin(C[i])], 1u);
+    if (inreg) {
+#pragma unroll
+        for (int e = 0; e < FR; ++e)
+            if (rk[e] != kNoKey) atomicAdd(&hist[bin(rk[e])], 1u);
+    } else {
+        for (int i = tid; i < n; i += 256) atomicAdd(&hist[bin(C[i])], 1u);
+    }
     __syncthreads();
     if (tid < 64) {
         const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
@@ -1043,11 +1068,17 @@ __global__ __launch_bounds__(256) void fast_select_kernel(int kl, const uint64_t
         }
         return;
     }
-    for (int i = tid; i < n; i += 256) {
-        const uint64_t key = C[i];
+    auto place = [&](uint64_t key) {
         const int bk = bin(key);
         if (bk < b) out[atomicAdd(&cnt, 1)] = key_id(key);
         else if (bk == b) buf[atomicAdd(&nb, 1)] = key;
+    };
+    if (inreg) {
+#pragma unroll
+        for (int e = 0; e < FR; ++e)
+            if (rk[e] != kNoKey) place(rk[e]);
+    } else {
+        for (int i = tid; i < n; i += 256) place(C[i]);
     }
     __syncthreads();
     const int m = nb, need = kl - below;
