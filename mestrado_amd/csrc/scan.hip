@@ -396,10 +396,17 @@ __global__ __launch_bounds__(NT, 6) void scan_kernel(ScanArgs a, const float *__
                 // so every table address is a compile-time constant and a
                 // gather is one SDWA byte shift plus a ds_read
                 float accs[U];
-                if (t1)
+                // The two calls differ only in the table base.  The distinct
+                // empty asm statements stop the optimizer from merging their
+                // common tails into one block with a run-time base, which
+                // would cost a second VALU instruction per gather there.
+                if (t1) {
                     adc_step<M, U>(ring[r], uint32_t(L::tab) + uint32_t(M * 256 * 4), base + (tid & ~63), ln, accs);
-                else
+                    asm volatile("; second table" ::);
+                } else {
                     adc_step<M, U>(ring[r], uint32_t(L::tab), base + (tid & ~63), ln, accs);
+                    asm volatile("; first table" ::);
+                }
                 // First step of the workgroup, threshold still +inf: every
                 // wavefront finds the ceil(kq / 8)-th smallest of its own
                 // distances by a bitwise search over ballot counts (no LDS,
