@@ -122,19 +122,42 @@ __global__ __launch_bounds__(NTC) void coarse_approx_kernel(const __bf16 *__rest
             }
         }
     }
-    // accumulator r of lane l: query row 16 qt + 4h + r, centroid column 16 t + l % 16
+    // accumulator r of lane l: query row 16 qt + 4h + r, centroid column 16 t + l % 16.
+    // Each 16 x 64 block of the wave goes through LDS so that a store writes a
+    // whole row piece (64 centroids, 256 contiguous bytes).  Row stride TS = 68
+    // floats puts the four rows a write touches (h = 0..3) in distinct banks.
+    constexpr int TS = 68;
+    __syncthreads();  // every wave is done with the query tile
+    float *T = lds + wave * (16 * TS);
+    double cnc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int c = c0 + 16 * t + i16;
-        if (c >= nlist) continue;
-        const double cnc = double(cn[c]);
+        cnc[t] = c < nlist ? double(cn[c]) : 0.0;
+    }
+    const int cl = c0 + lane;
 #pragma unroll
-        for (int qt = 0; qt < 4; ++qt)
+    for (int qt = 0; qt < 4; ++qt) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t q = q0 + 16 * qt + 4 * h + r;
-                if (q < nq) out[q * ldo + c] = float(qn[q] + cnc) - 2.0f * acc[qt][t][r];
+        for (int r = 0; r < 4; ++r) {
+            const int64_t q = q0 + 16 * qt + 4 * h + r;
+            const double qq = q < nq ? qn[q] : 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) T[(4 * h + r) * TS + 16 * t + i16] = float(qq + cnc[t]) - 2.0f * acc[qt][t][r];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (cl < nlist) {
+#pragma unroll
+            for (int row = 0; row < 16; ++row) {
+                const int64_t q = q0 + 16 * qt + row;
+                if (q < nq) out[q * ldo + cl] = T[row * TS + lane];
             }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
