@@ -421,6 +421,8 @@ __global__ __launch_bounds__(FNT, 4) void fast_scan_kernel(FastArgs a) {  // 4 w
         const uint8_t *lc = a.codes + beg * M;
         const int r16 = lane & 15, kb = lane >> 4;
         const uint32_t lim = stmp_sh[r16];
+        // a sum acc + 128 M lies in [0, 255 M], so sum < lim is acc < limi
+        const int limi = int(min(lim, uint32_t(255 * M + 1))) - 128 * M;
         const uint32_t tb = uint32_t(L::t8) + uint32_t(kb * NJ) * 4096u;
         const int wv0 = wave * (UT * 16);  // first vector of this wave's tiles in a step
         // code loads run one step ahead (addresses clamped to the list, so
@@ -462,16 +464,23 @@ __global__ __launch_bounds__(FNT, 4) void fast_scan_kernel(FastArgs a) {  // 4 w
                     for (int u = 0; u < UT; ++u)
                         acc[u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(xg[j][u], bsel, acc[u], 0, 0, 0);
             }
-            // pass bits: bit 4u + i = vector base + wv0 + 16u + 4kb + i
+            // pass bits: bit 4u + i = vector base + wv0 + 16u + 4kb + i; the
+            // row test is needed only in a list's last, partial step
             uint32_t pm = 0u;
             const int vrow = base + wv0 + 4 * kb;
+            if (base + FNT <= len) {  // uniform
 #pragma unroll
-            for (int u = 0; u < UT; ++u)
+                for (int u = 0; u < UT; ++u)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const uint32_t sum = uint32_t(acc[u][i] + 128 * M);
-                    pm |= uint32_t(vrow + 16 * u + i < len && sum < lim) << (4 * u + i);
-                }
+                    for (int i = 0; i < 4; ++i) pm |= uint32_t(acc[u][i] < limi) << (4 * u + i);
+            } else {
+                const int rem = len - vrow;
+#pragma unroll
+                for (int u = 0; u < UT; ++u)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        pm |= uint32_t(16 * u + i < rem && acc[u][i] < limi) << (4 * u + i);
+            }
             if (__ballot(pm != 0u) != 0) {  // wave-uniform
                 // wave prefix sum of the per-lane pass counts (<= 16: five bit planes)
                 const uint32_t n = uint32_t(__popc(pm));
