@@ -499,10 +499,19 @@ __global__ __launch_bounds__(FNT, 4) void fast_scan_kernel(FastArgs a) {  // 4 w
                 while (pm) {
                     const int bit = __ffs(pm) - 1;
                     pm &= pm - 1u;
-                    // select without dynamic register indexing (bit varies per lane)
-                    int sv = acc[0][0];
+                    // select without dynamic register indexing (bit varies per
+                    // lane): a tree over the bits of bit = 4u + i
+                    int cu[UT];
 #pragma unroll
-                    for (int k = 1; k < 4 * UT; ++k) sv = bit == k ? acc[k >> 2][k & 3] : sv;
+                    for (int u = 0; u < UT; ++u) {
+                        const int a01 = (bit & 1) ? acc[u][1] : acc[u][0];
+                        const int a23 = (bit & 1) ? acc[u][3] : acc[u][2];
+                        cu[u] = (bit & 2) ? a23 : a01;
+                    }
+                    static_assert(UT == 4, "the tree below selects one of four tiles");
+                    const int s01 = (bit & 4) ? cu[1] : cu[0];
+                    const int s23 = (bit & 4) ? cu[3] : cu[2];
+                    const int sv = (bit & 8) ? s23 : s01;
                     const uint32_t sum = uint32_t(sv + 128 * M);
                     const int v = vrow + 16 * (bit >> 2) + (bit & 3);
                     if (slot < POOL) {
