@@ -287,12 +287,12 @@ def main():
         pts = []
         for frac in [float(x) for x in args.serve_fracs.split(",")]:
             st = M.serve_bench(ix, query, rate=frac * qps_peak / mean_nq, nreq=args.serve_reqs, seed=11,
-                               bmax=1024, w_choices=(w,), k_choices=(k,), slots=4, max_batch=1024)
+                               bmax=1024, w_choices=(w,), k_choices=(k,), slots=3, max_batch=1024)
             pts.append({"load": frac, "offered_qps": round(st["offered_qps"], 1), "qps": round(st["qps"], 1),
                         "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                         "mean_ms": round(st["mean_ms"], 3), "batches": int(st["batches"])})
             log(f"serving at {frac:.2f} of peak: {pts[-1]}")
-        serving = {"form": "plain exact FP32 (per-request w, k)", "w": w, "k": k, "slots": 4, "max_batch": 1024,
+        serving = {"form": "plain exact FP32 (per-request w, k)", "w": w, "k": k, "slots": 3, "max_batch": 1024,
                    "requests_per_point": args.serve_reqs, "batch_sizes": "log-uniform 1..1024",
                    "arrivals": "Poisson (open loop)", "points": pts}
     if args.profile_only:
