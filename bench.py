@@ -32,7 +32,7 @@ def main():
     ap.add_argument("--workload", default="ivfpq_10m_l4096_m16")
     ap.add_argument("--nbase", type=int, default=0, help="override base size (smaller runs)")
     ap.add_argument("--cpu-queries", type=int, default=10000, help="queries in the CPU-baseline sample")
-    ap.add_argument("--cpu-reps", type=int, default=8, help="timed passes of the CPU-baseline sample")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed passes of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gt-queries", type=int, default=10000, help="queries with an FP64 ground truth")
     ap.add_argument("--fast-kl", type=int, default=0, help="shortlist of the reduced-precision form (0: sweep --fast-kls)")
@@ -328,24 +328,38 @@ def main():
         fast["speedup_vs_plain"] = round(fast["qps"] / qps, 3)
 
     cpu = None
+    parity = None
     if not args.no_cpu:
         from oracle import pyoracle as O
         ncpu = len(os.sched_getaffinity(0))
         nc = min(args.cpu_queries, nq)
         qh = query.numpy()[:nc]
         index = (coarse, pq, off, ids_l, codes_l)
-        O.search(index, qh[: min(64, nc)], w, k, nthreads=ncpu)  # warm-up
-        times = []
+        O.search_phases(index, qh[: min(64, nc)], w, k, nthreads=ncpu)  # warm-up
+        times, phases = [], []
         for _ in range(max(1, args.cpu_reps)):
             t0 = time.perf_counter()
-            O.search(index, qh, w, k, nthreads=ncpu)
+            r_ids, r_dis, ph = O.search_phases(index, qh, w, k, nthreads=ncpu)
             times.append(time.perf_counter() - t0)
+            phases.append(ph)
         dt = float(np.median(times))
+        tot = {n: sum(p[n] for p in phases) for n in phases[0]}
+        allp = sum(tot.values())
         cpu = {"value": round(nc / dt, 2), "unit": "queries/s", "cores": ncpu, "kind": "port",
                "sample": f"first {nc} of the {nq} queries, same index, w={w}, k={k}, median of "
                          f"{len(times)} passes ({sum(times):.1f} s of CPU work), OpenMP over queries "
-                         f"(oracle restatement of the parallel/ OpenMP search loop)"}
+                         f"(oracle restatement of the parallel/ OpenMP search loop: per probe a table, "
+                         f"sumidxtab2 into a concatenated array, then one k-selection)",
+               "phase_split": {n: round(v / allp, 4) for n, v in tot.items()}}
         log(f"cpu baseline: {cpu}")
+        # the oracle's results against the GPU's plain output on the same queries
+        g_ids, g_dis = out_ids.numpy()[:nc], out_dis.numpy()[:nc]
+        parity = {"queries": nc, "w": w, "k": k,
+                  "bit_identical": bool(np.array_equal(g_ids, r_ids) and
+                                        np.array_equal(g_dis.view(np.uint32), r_dis.view(np.uint32))),
+                  "rows_differing": int(((g_ids != r_ids).any(1) |
+                                         (g_dis.view(np.uint32) != r_dis.view(np.uint32)).any(1)).sum())}
+        log(f"parity vs oracle: {parity}")
 
     res = {
         "metric": "QPS at fixed recall@100 (10M x128 IVF4096,PQ16); ADC-scan HBM GB/s; p99 ms",
@@ -361,7 +375,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Gaussian mixture, integer-valued float32)",
         "config": {"workload": args.workload, "nbase": cfg["nbase"], "nlist": cfg["nlist"], "m": m,
-                   "nquery": nq, "w": w, "k": k, "form": "plain exact FP32", **rec},
+                   "nquery": nq, "w": w, "k": k, "form": "plain exact FP32",
+                   "kmeans_iterations": {"coarse": cfg["niter_coarse"], "pq": cfg["niter_pq"]}, **rec},
         "stages_ms": {"coarse": round(float(stage[0]), 4), "select": round(float(stage[1]), 4),
                       "scan": round(scan_ms, 4), "merge": round(float(stage[3]), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
@@ -370,6 +385,7 @@ def main():
                      "traffic": traffic,
                      "kernel": "scan_kernel", "algorithmic_bytes_per_launch": scan_bytes},
         "cpu_baseline": cpu,
+        "parity_vs_oracle": parity,
         # the same exact arithmetic, list-major: the batched form's scan kernel on the headline batch
         "roofline_batched_exact": (uneven or {}).get("uniform_headline", {}).get("scan_kernel_roofline"),
         "fast_form": fast,

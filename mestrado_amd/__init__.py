@@ -161,6 +161,14 @@ class DeviceArray:
             _check(lib().mivf_memcpy_dtoh(_ptr(out), self.ptr, self.nbytes))
         return out
 
+    def head(self, n):
+        """Host copy of the first n rows (along the first axis)."""
+        n = min(int(n), self.shape[0])
+        out = np.empty((n,) + self.shape[1:], self.dtype)
+        if out.nbytes:
+            _check(lib().mivf_memcpy_dtoh(_ptr(out), self.ptr, out.nbytes))
+        return out
+
     def offset(self, nbytes):
         return ctypes.c_void_p(self.ptr.value + int(nbytes))
 

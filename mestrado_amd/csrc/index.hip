@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mivf.h"
@@ -55,6 +56,17 @@ int upload_lists(mivf_index *ix) {
 
 namespace {
 
+// Run fn(j, nth) for j < nth on nth host threads (at most the hardware's).
+template <typename Fn>
+void host_parallel(int64_t want, Fn fn) {
+    const int hw = int(std::max(1u, std::thread::hardware_concurrency()));
+    const int nth = int(std::max<int64_t>(1, std::min<int64_t>(want, hw)));
+    std::vector<std::thread> th;
+    for (int j = 1; j < nth; ++j) th.emplace_back(fn, j, nth);
+    fn(0, nth);
+    for (auto &t : th) t.join();
+}
+
 // ---------------------------------------------------------------------------
 // k-means: GPU assignment with the reference distance arithmetic (a = centroid,
 // b = point, strict '<' -> lowest index), host update in double precision
@@ -94,16 +106,22 @@ int kmeans_dev(const float *d_x, const float *h_x, int64_t n, int d, int ldx, in
                                           dxn.as<double>(), n, d, dkeys.as<uint64_t>(), nullptr));
         MIVF_CHECK_HIP(launch_keys_to_idx(dkeys.as<uint64_t>(), n, dasg.as<int32_t>(), nullptr));
         MIVF_CHECK_HIP(hipMemcpy(assign.data(), dasg.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i)
+            MIVF_REQUIRE(assign[i] >= 0 && assign[i] < k, kInvalid, "kmeans: non-finite distances in training data");
         std::fill(sums.begin(), sums.end(), 0.0);
         std::fill(cnt.begin(), cnt.end(), 0);
-        for (int64_t i = 0; i < n; ++i) {
-            const int c = assign[i];
-            MIVF_REQUIRE(c >= 0 && c < k, kInvalid, "kmeans: non-finite distances in training data");
-            cnt[c]++;
-            double *s = sums.data() + int64_t(c) * d;
-            const float *xi = h_x + i * ldx;
-            for (int t = 0; t < d; ++t) s[t] += double(xi[t]);
-        }
+        // each host thread owns the clusters c with c % nth == j and adds its
+        // points in point order: the same sums as one sequential pass
+        host_parallel(std::min<int64_t>(k, 16), [&](int j, int nth) {
+            for (int64_t i = 0; i < n; ++i) {
+                const int c = assign[i];
+                if (c % nth != j) continue;
+                cnt[c]++;
+                double *s = sums.data() + int64_t(c) * d;
+                const float *xi = h_x + i * ldx;
+                for (int t = 0; t < d; ++t) s[t] += double(xi[t]);
+            }
+        });
         for (int c = 0; c < k; ++c)
             if (cnt[c] > 0)
                 for (int t = 0; t < d; ++t)

@@ -3,7 +3,9 @@
 Every workload draws d = 128 float32 rows holding integers 0..255 from a
 seeded mixture of 1000 Gaussians (centres uniform in [0, 128], sigma = 20 per
 dimension, rounded and clipped); train, base and query rows come from separate
-streams of the same generator.
+streams of the same generator.  Every index is trained with the k-means
+iteration counts of the reference's ivfpq_new (50 coarse, 100 per
+subquantizer; parallel/ivf_pq/ivf_training.cpp:77, parallel/pq-utils/pq_new.cpp:53).
 """
 D = 128
 NCOMP = 1000
@@ -20,10 +22,10 @@ WORKLOADS = {
                               w=8, k=100, zipf=None, niter_coarse=50, niter_pq=100),
     # plain GPU drop-in: 1M base, 10k queries, 100k training, 1024 lists, m=8, w=16, k=100
     "ivfpq_1m_l1024_m8": dict(seed=102, ntrain=100_000, nbase=1_000_000, nquery=10_000, nlist=1024,
-                              m=8, w=16, k=100, zipf=None, niter_coarse=20, niter_pq=20),
+                              m=8, w=16, k=100, zipf=None, niter_coarse=50, niter_pq=100),
     # headline: 10M base (Zipf s=1.1 mixture weights), 4096 lists, m=16,
     # 10k queries, w=32, k=100; trained on 1M (= base/10)
     "ivfpq_10m_l4096_m16": dict(seed=103, ntrain=1_000_000, nbase=10_000_000, nquery=10_000,
-                                nlist=4096, m=16, w=32, k=100, zipf=1.1, niter_coarse=16,
-                                niter_pq=16),
+                                nlist=4096, m=16, w=32, k=100, zipf=1.1, niter_coarse=50,
+                                niter_pq=100),
 }

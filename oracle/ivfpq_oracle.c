@@ -405,6 +405,85 @@ int orc_search(int d, int nlist, int m, int ks, const float *coarse, const float
     return rc;
 }
 
+/* The same search staged as the parallel/ CPU loop is, with a wall-clock
+   split per phase (parallel/ivf_pq/ivf_search.cpp:97-137 and its per-thread
+   log :150-162): coarse top-w (knn_full, ivf_assign.cpp:28), then per probe
+   the residual and table (ivf_search.cpp:206-209) and sumidxtab2 into a
+   distance array (:211, :451-468), the arrays of the w probes concatenated
+   (:114-128), then the k-selection over the concatenation (:131-137, exact
+   (distance, id) order in place of my_k_min).  Results equal orc_search.
+   phase_s[4] receives the seconds summed over threads: coarse, residual +
+   tables, ADC + concatenation, selection. */
+int orc_search_phases(int d, int nlist, int m, int ks, const float *coarse, const float *pq,
+                      const int64_t *offsets, const int32_t *ids, const uint8_t *codes,
+                      const float *queries, int nq, int w, int k,
+                      int32_t *out_ids, float *out_dis, double *phase_s, int nthreads)
+{
+    if (w < 1 || w > nlist || k < 1 || d % m) return 1;
+    int64_t maxlen = 0;
+    for (int l = 0; l < nlist; l++)
+        if (offsets[l + 1] - offsets[l] > maxlen) maxlen = offsets[l + 1] - offsets[l];
+    float *cn = (float *)malloc(sizeof(float) * nlist);
+    for (int c = 0; c < nlist; c++) cn[c] = norm_f32(coarse + (int64_t)c * d, d);
+    double tot[4] = {0, 0, 0, 0};
+    set_threads(nthreads);
+#pragma omp parallel
+    {
+        double t[4] = {0, 0, 0, 0};
+        uint64_t *ck = (uint64_t *)malloc(sizeof(uint64_t) * nlist);
+        uint64_t *pb = (uint64_t *)malloc(sizeof(uint64_t) * nlist);
+        uint64_t *hb = (uint64_t *)malloc(sizeof(uint64_t) * (k + 1));
+        uint64_t *ob = (uint64_t *)malloc(sizeof(uint64_t) * (k + 1));
+        float *tab = (float *)malloc(sizeof(float) * m * ks);
+        float *r = (float *)malloc(sizeof(float) * d);
+        int64_t cap = maxlen * w + 1;
+        float *cdis = (float *)malloc(sizeof(float) * cap);
+        int32_t *cid = (int32_t *)malloc(sizeof(int32_t) * cap);
+#pragma omp for schedule(dynamic, 1)
+        for (int q = 0; q < nq; q++) {
+            const float *x = queries + (int64_t)q * d;
+            double t0 = omp_get_wtime();
+            double nb = norm_f64(x, d);
+            kheap C = {pb, 0, w};
+            for (int c = 0; c < nlist; c++) {
+                float dd = yael_dist(coarse + (int64_t)c * d, cn[c], x, nb, d);
+                kh_push(&C, orc_ckey(dd, c));
+            }
+            int nprobe = kh_finish(&C, ck);
+            double t1 = omp_get_wtime();
+            t[0] += t1 - t0;
+            int64_t n = 0;
+            for (int p = 0; p < nprobe; p++) {
+                int l = (int)(ck[p] & 0xFFFFFFFFu);
+                double a = omp_get_wtime();
+                orc_residual(d, x, coarse + (int64_t)l * d, r);
+                orc_tables(d, m, ks, r, pq, tab);
+                double b = omp_get_wtime();
+                int64_t len = offsets[l + 1] - offsets[l];
+                orc_adc(m, ks, tab, codes + offsets[l] * m, len, cdis + n);
+                memcpy(cid + n, ids + offsets[l], sizeof(int32_t) * len);
+                n += len;
+                double c2 = omp_get_wtime();
+                t[1] += b - a;
+                t[2] += c2 - b;
+            }
+            double t2 = omp_get_wtime();
+            kheap H = {hb, 0, k};
+            for (int64_t i = 0; i < n; i++) kh_push(&H, orc_ckey(cdis[i], cid[i]));
+            int cnt = kh_finish(&H, ob);
+            emit(ob, cnt, k, out_ids + (int64_t)q * k, out_dis + (int64_t)q * k);
+            t[3] += omp_get_wtime() - t2;
+        }
+#pragma omp critical
+        for (int i = 0; i < 4; i++) tot[i] += t[i];
+        free(ck); free(pb); free(hb); free(ob); free(tab); free(r); free(cdis); free(cid);
+    }
+    free(cn);
+    if (phase_s)
+        for (int i = 0; i < 4; i++) phase_s[i] = tot[i];
+    return 0;
+}
+
 /* parallel2/ivf_pq/mycuda.cu:119-188 semantics: residuals already computed */
 int orc_search_residual(int d, int m, int ks, const float *pq,
                         const int64_t *offsets, const int32_t *ids,

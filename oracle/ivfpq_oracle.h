@@ -72,6 +72,13 @@ int orc_search(int d, int nlist, int m, int ks, const float *coarse, const float
                const float *queries, int nq, int w, int k,
                int32_t *out_ids, float *out_dis, int32_t *coaidx, float *coadis,
                int nthreads);
+/* the same search staged as the parallel/ CPU loop (coarse, tables, ADC into
+   a concatenated array, selection), with phase_s[4] = seconds per phase
+   summed over threads; results equal orc_search */
+int orc_search_phases(int d, int nlist, int m, int ks, const float *coarse, const float *pq,
+                      const int64_t *offsets, const int32_t *ids, const uint8_t *codes,
+                      const float *queries, int nq, int w, int k,
+                      int32_t *out_ids, float *out_dis, double *phase_s, int nthreads);
 /* per-query w and k (rows of out_* are kmax wide) */
 int orc_search_var(int d, int nlist, int m, int ks, const float *coarse,
                    const float *pq, const int64_t *offsets, const int32_t *ids,

@@ -28,6 +28,7 @@ _SIGS = {
     "orc_tables": (None, [_i, _i, _i, _vp, _vp, _vp]),
     "orc_adc": (None, [_i, _i, _vp, _vp, _i64, _vp]),
     "orc_search": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i]),
+    "orc_search_phases": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i]),
     "orc_search_var": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _i]),
     "orc_search_residual": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _i]),
     "orc_search_rerank": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i, _i, _i, _i,
@@ -165,6 +166,21 @@ def search(index, q, w, k, nthreads=0, want_coarse=False):
                           P(oi), P(od), P(ci), P(cd), nthreads)
     assert rc == 0, rc
     return (oi, od, ci, cd) if want_coarse else (oi, od)
+
+
+def search_phases(index, q, w, k, nthreads=0):
+    """orc_search staged as the reference's parallel/ loop; returns
+    (ids, dis, seconds per phase {coarse, tables, adc, select} summed over threads)."""
+    d, nlist, m, ks, coarse, pq, off, ids, codes = _ix(index)
+    q = f32(q)
+    nq = q.shape[0]
+    oi = np.empty((nq, k), np.int32)
+    od = np.empty((nq, k), np.float32)
+    ph = np.zeros(4, np.float64)
+    rc = lib().orc_search_phases(d, nlist, m, ks, P(coarse), P(pq), P(off), P(ids), P(codes), P(q), nq, w, k,
+                                 P(oi), P(od), P(ph), nthreads)
+    assert rc == 0, rc
+    return oi, od, dict(zip(("coarse", "tables", "adc", "select"), ph.tolist()))
 
 
 def search_var(index, q, wq, kq, kmax, nthreads=0):

@@ -253,6 +253,18 @@ def small_index(orc, seed=10, nlist=32, m=8, nbase=3000):
     return orc.build_index(coarse, pq, base), base, q
 
 
+def test_search_phases_equals_search(orc):
+    """The staged search (tables, ADC into a concatenated array, then the
+    selection) returns exactly the fused search's results, and the four phase
+    times are non-negative."""
+    index, base, q = small_index(orc)
+    for w, k in ((1, 5), (8, 100), (32, 700)):
+        a_i, a_d = orc.search(index, q, w, k)
+        b_i, b_d, ph = orc.search_phases(index, q, w, k, nthreads=2)
+        assert np.array_equal(a_i, b_i) and np.array_equal(a_d.view(np.uint32), b_d.view(np.uint32))
+        assert set(ph) == {"coarse", "tables", "adc", "select"} and min(ph.values()) >= 0.0
+
+
 def test_search_all_lists_is_exhaustive_adc(orc):
     """w = nlist probes every list: the result is the exhaustive ADC top-k
     (brute force over every base vector with its own list's table)."""
