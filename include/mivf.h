@@ -121,6 +121,21 @@ int mivf_coarse(mivf_index *ix, const float *d_q, int nq, int w, int32_t *d_idx,
 int mivf_search_residual(mivf_index *ix, const float *d_residual,
                          const int32_t *d_list_of, int nq, int w, int k,
                          int32_t *d_ids, float *d_dis, void *stream);
+/* Per-list search of the parallel/ fork, ivfpq_search(ivf_t*, float*,
+ * pqtipo, int, double*, double*) (parallel/ivf_pq/ivf_search.cpp:178-228):
+ * the m x ks table of one residual d_residual [d] (compute_cross_distances,
+ * :206-209) and the ADC distance of each of n codes d_codes [n][m] (uint8,
+ * sumidxtab2, :451-468) into d_dis [n], in code order, with the index's PQ.
+ * Every code must be < ks (all are when ks = 256).                         */
+int mivf_adc(mivf_index *ix, const float *d_residual, const uint8_t *d_codes, int64_t n, float *d_dis,
+             void *stream);
+/* Replaces my_k_min(dis_t, int, float*, int*) (parallel/ivf_pq/k_min.cpp:3-73)
+ * by an exact selection: the k smallest (distance, id) pairs of
+ * d_dis/d_ids [n], ascending, ties by lower id, into d_out_ids/d_out_dis [k];
+ * entries past n are (-1, +inf).  k <= 1024.  (The reference's heap returns
+ * other than the k smallest, e.g. 1, 2, 6 ... for the input 1 ... 7.)       */
+int mivf_kmin(const float *d_dis, const int32_t *d_ids, int64_t n, int k, int32_t *d_out_ids, float *d_out_dis,
+              void *stream);
 /* RERANK path (sequential/ivf_pq/ivf_search.cpp:98-134): ADC shortlist of kl,
  * then exact FP32 distances to the raw base rows d_base [nbase][d], best k
  * by (distance, id).  Distances are returned too.                          */

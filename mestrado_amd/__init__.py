@@ -55,6 +55,8 @@ _SIGS = {
     "mivf_coarse": [_vp, _vp, _i, _i, _vp, _vp, _vp],
     "mivf_search_residual": [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp],
     "mivf_search_rerank": [_vp, _vp, _i, _i, _i, _i, _vp, _i64, _vp, _vp, _vp],
+    "mivf_adc": [_vp, _vp, _vp, _i64, _vp, _vp],
+    "mivf_kmin": [_vp, _vp, _i64, _i, _vp, _vp, _vp],
     "mivf_search_fast": [_vp, _vp, _i, _i, _i, _i, _vp, _i64, _vp, _vp, _vp],
     "mivf_fast_stats": [_vp, _i64p, _i64p],
     "mivf_index_attach_base": [_vp, _vp, _i64],
@@ -266,6 +268,17 @@ def fast_select(cand, cnt, kl):
     return out.numpy()
 
 
+def kmin(dis, ids, k):
+    """Exact k smallest (distance, id) pairs of one candidate array, ascending
+    (my_k_min done exactly); padded with (-1, +inf) past len(dis)."""
+    dd, _ = _dev(dis, np.float32)
+    di, _ = _dev(ids, np.int32)
+    oi = DeviceArray((k,), np.int32)
+    od = DeviceArray((k,), np.float32)
+    _check(lib().mivf_kmin(dd.ptr, di.ptr, dd.shape[0], k, oi.ptr, od.ptr, None))
+    return oi.numpy(), od.numpy()
+
+
 class Index:
     """Device-resident IVF-PQ index (coarse + PQ quantizers and inverted lists)."""
 
@@ -375,6 +388,15 @@ class Index:
         dis = DeviceArray((nq, k), np.float32)
         _check(lib().mivf_search_residual(self.h, rd.ptr, ld.ptr, nq, w, k, ids.ptr, dis.ptr, None))
         return ids.numpy(), dis.numpy()
+
+    def adc(self, residual, codes):
+        """ADC distances of one residual over codes [n][m] (uint8), in code order."""
+        rd, _ = _dev(residual, np.float32)
+        cd, _ = _dev(codes, np.uint8)
+        n = cd.shape[0]
+        out = DeviceArray((max(n, 1),), np.float32)
+        _check(lib().mivf_adc(self.h, rd.ptr, cd.ptr, n, out.ptr, None))
+        return out.numpy()[:n]
 
     def attach_base(self, base):
         """Keep a list-ordered copy of the raw vectors for search_fast(base=None)."""

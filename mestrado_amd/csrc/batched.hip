@@ -239,6 +239,7 @@ struct mivf_plan {
     XCfg cfg;
     int nq = 0, wmax = 0, kmax = 0;
     bool tune = false, captured = false;
+    uint64_t list_gen = 0;  // the index's list upload the graph was captured against
     DevBuf q, ids, dis;
     int32_t *h_wk = nullptr;  // pinned staging of w and k
     hipStream_t st = nullptr;
@@ -263,7 +264,7 @@ XWs &index_xws(mivf_index *ix) {
 }  // namespace
 
 int mivf_search_batched(mivf_index *ix, const float *d_q, int nq, const int32_t *w, const int32_t *k, int kmax,
-                        int32_t *d_ids, float *d_dis, void *stream) {
+                        int32_t *d_ids, float *d_dis, void *stream) try {
     MIVF_REQUIRE(ix, kInvalid, "null index");
     if (nq <= 0) return kOk;
     MIVF_REQUIRE(d_q && w && k && d_ids && d_dis, kInvalid, "search_batched: null pointer");
@@ -297,15 +298,15 @@ int mivf_search_batched(mivf_index *ix, const float *d_q, int nq, const int32_t 
     std::copy(kept, kept + 8, ix->timings);
     ix->batched_last_overflow = nov;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_batched_stats(mivf_index *ix, int64_t *overflowed_queries) {
+int mivf_batched_stats(mivf_index *ix, int64_t *overflowed_queries) try {
     MIVF_REQUIRE(ix, kInvalid, "null index");
     if (overflowed_queries) *overflowed_queries = ix->batched_last_overflow;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_plan_create(mivf_index *ix, int nq, int wmax, int kmax, int autotune_flag, mivf_plan **out) {
+int mivf_plan_create(mivf_index *ix, int nq, int wmax, int kmax, int autotune_flag, mivf_plan **out) try {
     MIVF_REQUIRE(ix && out && nq >= 1, kInvalid, "plan_create: bad arguments");
     MIVF_REQUIRE(wmax >= 1 && wmax <= ix->nlist && wmax <= 1024 && kmax >= 1 && kmax <= 1024, kInvalid,
                  "plan_create: wmax must be in 1..min(nlist,1024), kmax in 1..1024");
@@ -336,15 +337,15 @@ int mivf_plan_create(mivf_index *ix, int nq, int wmax, int kmax, int autotune_fl
     }
     *out = p;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_plan_free(mivf_plan *p) {
+int mivf_plan_free(mivf_plan *p) try {
     delete p;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 int mivf_plan_search(mivf_plan *p, const float *d_q, const int32_t *w, const int32_t *k, int32_t *d_ids, float *d_dis,
-                     void *stream) {
+                     void *stream) try {
     MIVF_REQUIRE(p && d_q && w && k && d_ids && d_dis, kInvalid, "plan_search: null argument");
     mivf_index *ix = p->ix;
     const int nq = p->nq;
@@ -362,6 +363,18 @@ int mivf_plan_search(mivf_plan *p, const float *d_q, const int32_t *w, const int
         MIVF_CHECK_HIP(hipEventRecord(e, user));
         MIVF_CHECK_HIP(hipStreamWaitEvent(st, e, 0));
         MIVF_CHECK_HIP(hipEventDestroy(e));
+    }
+    // The graph holds the device addresses of the lists: after an add or
+    // set_lists, upload the new lists and capture again.
+    MIVF_TRY(upload_lists(ix));
+    if (p->captured && p->list_gen != ix->list_gen) {
+        MIVF_CHECK_HIP(hipStreamSynchronize(st));
+        MIVF_CHECK_HIP(hipGraphExecDestroy(p->exec));
+        MIVF_CHECK_HIP(hipGraphDestroy(p->graph));
+        p->exec = nullptr;
+        p->graph = nullptr;
+        p->captured = false;
+        p->tune = false;  // keep the tuned work-item shape
     }
     MIVF_CHECK_HIP(hipMemcpyAsync(p->q.p, d_q, sizeof(float) * size_t(nq) * ix->d, hipMemcpyDeviceToDevice, st));
     MIVF_CHECK_HIP(hipMemcpyAsync(p->x.wq.p, p->h_wk, sizeof(int32_t) * nq, hipMemcpyHostToDevice, st));
@@ -384,6 +397,7 @@ int mivf_plan_search(mivf_plan *p, const float *d_q, const int32_t *w, const int
         p->graph = g;
         MIVF_CHECK_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
         p->captured = true;
+        p->list_gen = ix->list_gen;
     }
     MIVF_CHECK_HIP(hipGraphLaunch(p->exec, st));
     int nov = 0;
@@ -394,17 +408,17 @@ int mivf_plan_search(mivf_plan *p, const float *d_q, const int32_t *w, const int
     MIVF_CHECK_HIP(hipMemcpyAsync(d_dis, p->dis.p, sizeof(float) * size_t(nq) * p->kmax, hipMemcpyDeviceToDevice, st));
     MIVF_CHECK_HIP(hipStreamSynchronize(st));
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_plan_info(const mivf_plan *p, int *g, int *threads, float *tuned_ms) {
+int mivf_plan_info(const mivf_plan *p, int *g, int *threads, float *tuned_ms) try {
     MIVF_REQUIRE(p, kInvalid, "plan_info: null plan");
     if (g) *g = p->cfg.g;
     if (threads) *threads = p->cfg.nt;
     if (tuned_ms) *tuned_ms = p->tuned_ms;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_exact_tables(mivf_index *ix, const float *d_q, int n, int list, float *d_out) {
+int mivf_exact_tables(mivf_index *ix, const float *d_q, int n, int list, float *d_out) try {
     MIVF_REQUIRE(ix && d_q && d_out && n >= 1 && n <= 8 && list >= 0 && list < ix->nlist, kInvalid,
                  "exact_tables: bad arguments");
     ExactArgs a{};
@@ -416,9 +430,9 @@ int mivf_exact_tables(mivf_index *ix, const float *d_q, int n, int list, float *
     MIVF_CHECK_HIP(launch_exact_tables(a, ix->m, list, n, d_out, nullptr));
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_exact_adc(mivf_index *ix, const float *d_q, int n, int list, float *d_out) {
+int mivf_exact_adc(mivf_index *ix, const float *d_q, int n, int list, float *d_out) try {
     MIVF_REQUIRE(ix && d_q && d_out && n >= 1 && n <= 8 && list >= 0 && list < ix->nlist, kInvalid,
                  "exact_adc: bad arguments");
     MIVF_REQUIRE(ix->m == 16, kUnsupported, "exact_adc: compiled for m = 16");
@@ -435,5 +449,5 @@ int mivf_exact_adc(mivf_index *ix, const float *d_q, int n, int list, float *d_o
     MIVF_CHECK_HIP(launch_exact_adc(a, ix->m, list, n, beg, len, d_out, nullptr));
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
-}
+} MIVF_CATCH_ALL
 

@@ -47,6 +47,7 @@ int upload_lists(mivf_index *ix) {
     ix->h_off = off;
     ix->ntotal = n;
     ix->dirty = false;
+    ix->list_gen++;
     ix->rows.release();  // list positions changed: an attached copy is stale
     ix->rows_ok = false;
     return kOk;
@@ -431,12 +432,12 @@ extern "C" {
 const char *mivf_last_error(void) { return last_error(); }
 int mivf_version(void) { return 100; }
 
-int mivf_device_count(int *count) {
+int mivf_device_count(int *count) try {
     MIVF_REQUIRE(count, kInvalid, "null count");
     MIVF_CHECK_HIP(hipGetDeviceCount(count));
     return kOk;
-}
-int mivf_malloc(void **p, size_t bytes) {
+} MIVF_CATCH_ALL
+int mivf_malloc(void **p, size_t bytes) try {
     MIVF_REQUIRE(p, kInvalid, "null pointer");
     hipError_t e = hipMalloc(p, bytes ? bytes : 1);
     if (e != hipSuccess) {
@@ -444,56 +445,56 @@ int mivf_malloc(void **p, size_t bytes) {
         return kNoMem;
     }
     return kOk;
-}
-int mivf_free(void *p) {
+} MIVF_CATCH_ALL
+int mivf_free(void *p) try {
     MIVF_CHECK_HIP(hipFree(p));
     return kOk;
-}
-int mivf_memcpy_htod(void *dst, const void *src, size_t bytes) {
+} MIVF_CATCH_ALL
+int mivf_memcpy_htod(void *dst, const void *src, size_t bytes) try {
     MIVF_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return kOk;
-}
-int mivf_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+} MIVF_CATCH_ALL
+int mivf_memcpy_dtoh(void *dst, const void *src, size_t bytes) try {
     MIVF_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return kOk;
-}
-int mivf_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+} MIVF_CATCH_ALL
+int mivf_memcpy_dtod(void *dst, const void *src, size_t bytes) try {
     MIVF_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return kOk;
-}
-int mivf_synchronize(void) {
+} MIVF_CATCH_ALL
+int mivf_synchronize(void) try {
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 int mivf_kmeans(const float *x, int64_t n, int d, int ldx, int k, int niter, uint64_t seed,
-                float *cent_out) {
+                float *cent_out) try {
     MIVF_REQUIRE(x && cent_out && n > 0 && d > 0 && ldx >= d, kInvalid, "kmeans: bad arguments");
     DevBuf dx;
     MIVF_TRY(dx.ensure(sizeof(float) * size_t(n) * ldx));
     MIVF_CHECK_HIP(hipMemcpy(dx.p, x, sizeof(float) * size_t(n) * ldx, hipMemcpyHostToDevice));
     return kmeans_dev(dx.as<float>(), x, n, d, ldx, k, niter, seed, cent_out);
-}
+} MIVF_CATCH_ALL
 
 int mivf_train(const float *x, int64_t n, int d, int nlist, int m, int ks, int niter_coarse,
-               int niter_pq, uint64_t seed, float *coarse_out, float *pq_out) {
+               int niter_pq, uint64_t seed, float *coarse_out, float *pq_out) try {
     MIVF_REQUIRE(x && coarse_out && pq_out && n > 0 && d > 0 && nlist > 0, kInvalid, "train: bad arguments");
     DevBuf dx;
     MIVF_TRY(dx.ensure(sizeof(float) * size_t(n) * d));
     MIVF_CHECK_HIP(hipMemcpy(dx.p, x, sizeof(float) * size_t(n) * d, hipMemcpyHostToDevice));
     return train_impl(dx.as<float>(), x, n, d, nlist, m, ks, niter_coarse, niter_pq, seed, coarse_out, pq_out);
-}
+} MIVF_CATCH_ALL
 
 int mivf_train_device(const float *d_x, int64_t n, int d, int nlist, int m, int ks, int niter_coarse,
-                      int niter_pq, uint64_t seed, float *coarse_out, float *pq_out) {
+                      int niter_pq, uint64_t seed, float *coarse_out, float *pq_out) try {
     MIVF_REQUIRE(d_x && coarse_out && pq_out && n > 0 && d > 0 && nlist > 0, kInvalid, "train: bad arguments");
     std::vector<float> hx(size_t(n) * d);
     MIVF_CHECK_HIP(hipMemcpy(hx.data(), d_x, sizeof(float) * hx.size(), hipMemcpyDeviceToHost));
     return train_impl(d_x, hx.data(), n, d, nlist, m, ks, niter_coarse, niter_pq, seed, coarse_out, pq_out);
-}
+} MIVF_CATCH_ALL
 
 int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, const float *pq,
-                      mivf_index **out) {
+                      mivf_index **out) try {
     MIVF_REQUIRE(out && coarse && pq, kInvalid, "index_create: null argument");
     MIVF_REQUIRE(nlist >= 1 && m >= 1 && d >= 1 && d % m == 0, kInvalid, "index_create: d must be a multiple of m");
     MIVF_REQUIRE(ks >= 1 && ks <= 256, kInvalid, "index_create: ks must be in 1..256 (8-bit codes)");
@@ -586,16 +587,16 @@ int mivf_index_create(int d, int nlist, int m, int ks, const float *coarse, cons
     if ((rc = upload_lists(ix))) return fail(rc);
     *out = ix;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_index_free(mivf_index *ix) {
+int mivf_index_free(mivf_index *ix) try {
     if (!ix) return kOk;
     (void)hipDeviceSynchronize();  // buffers may still be in use by queued work
     delete ix;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_index_info(const mivf_index *ix, int *d, int *nlist, int *m, int *ks, int64_t *ntotal) {
+int mivf_index_info(const mivf_index *ix, int *d, int *nlist, int *m, int *ks, int64_t *ntotal) try {
     MIVF_TRY(check_index(ix));
     if (d) *d = ix->d;
     if (nlist) *nlist = ix->nlist;
@@ -607,19 +608,19 @@ int mivf_index_info(const mivf_index *ix, int *d, int *nlist, int *m, int *ks, i
         *ntotal = n;
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_index_set_lists(mivf_index *ix, const int64_t *offsets, const int32_t *ids, const uint8_t *codes) {
+int mivf_index_set_lists(mivf_index *ix, const int64_t *offsets, const int32_t *ids, const uint8_t *codes) try {
     MIVF_TRY(check_index(ix));
     return set_lists_impl(ix, offsets, ids, codes, nullptr);
-}
+} MIVF_CATCH_ALL
 
-int mivf_index_set_lists_i32(mivf_index *ix, const int64_t *offsets, const int32_t *ids, const int32_t *codes) {
+int mivf_index_set_lists_i32(mivf_index *ix, const int64_t *offsets, const int32_t *ids, const int32_t *codes) try {
     MIVF_TRY(check_index(ix));
     return set_lists_impl(ix, offsets, ids, nullptr, codes);
-}
+} MIVF_CATCH_ALL
 
-int mivf_index_get_lists(mivf_index *ix, int64_t *offsets, int32_t *ids, uint8_t *codes) {
+int mivf_index_get_lists(mivf_index *ix, int64_t *offsets, int32_t *ids, uint8_t *codes) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(offsets, kInvalid, "get_lists: null offsets");
     int64_t o = 0;
@@ -632,9 +633,9 @@ int mivf_index_get_lists(mivf_index *ix, int64_t *offsets, int32_t *ids, uint8_t
         offsets[l + 1] = o;
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_index_add(mivf_index *ix, const float *x, int64_t n, int32_t id0) {
+int mivf_index_add(mivf_index *ix, const float *x, int64_t n, int32_t id0) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(n >= 0 && (n == 0 || x), kInvalid, "add: bad arguments");
     MIVF_REQUIRE(int64_t(id0) + n <= (int64_t(1) << 31), kInvalid, "add: ids exceed int32");
@@ -648,25 +649,25 @@ int mivf_index_add(mivf_index *ix, const float *x, int64_t n, int32_t id0) {
         MIVF_TRY(add_impl(ix, dx.as<float>(), nn, id0 + int32_t(r0)));
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_index_add_device(mivf_index *ix, const float *d_x, int64_t n, int32_t id0) {
+int mivf_index_add_device(mivf_index *ix, const float *d_x, int64_t n, int32_t id0) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(n >= 0 && (n == 0 || d_x), kInvalid, "add: bad arguments");
     MIVF_REQUIRE(int64_t(id0) + n <= (int64_t(1) << 31), kInvalid, "add: ids exceed int32");
     if (n == 0) return kOk;
     return add_impl(ix, d_x, n, id0);
-}
+} MIVF_CATCH_ALL
 
 int mivf_index_encode(mivf_index *ix, const float *d_x, int64_t n, int32_t *d_assign, uint8_t *d_codes,
-                      void *stream) {
+                      void *stream) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(n >= 0 && (n == 0 || (d_x && d_assign && d_codes)), kInvalid, "encode: bad arguments");
     return encode_impl(ix, d_x, n, d_assign, d_codes, static_cast<hipStream_t>(stream));
-}
+} MIVF_CATCH_ALL
 
 int mivf_search(mivf_index *ix, const float *d_q, int nq, int w, int k, int32_t *d_ids, float *d_dis,
-                void *stream) {
+                void *stream) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(nq >= 0, kInvalid, "search: nq < 0");
     MIVF_REQUIRE(w >= 1 && w <= ix->nlist, kInvalid, "search: w must be in 1..nlist");
@@ -675,10 +676,10 @@ int mivf_search(mivf_index *ix, const float *d_q, int nq, int w, int k, int32_t 
     if (nq == 0) return kOk;
     MIVF_REQUIRE(d_q && d_ids && d_dis, kInvalid, "search: null pointer");
     return search_entry(ix, ix->ws, d_q, nq, w, nullptr, nullptr, k, k, d_ids, d_dis, static_cast<hipStream_t>(stream));
-}
+} MIVF_CATCH_ALL
 
 int mivf_search_var(mivf_index *ix, const float *d_q, int nq, const int32_t *w, const int32_t *k,
-                    int32_t *d_ids, float *d_dis, int kmax, void *stream) {
+                    int32_t *d_ids, float *d_dis, int kmax, void *stream) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(nq >= 0 && kmax >= 1 && kmax <= 1024, kInvalid, "search_var: kmax must be in 1..1024");
     if (nq == 0) return kOk;
@@ -695,9 +696,9 @@ int mivf_search_var(mivf_index *ix, const float *d_q, int nq, const int32_t *w, 
     MIVF_CHECK_HIP(hipMemcpy(ix->ws.kq.p, k, sizeof(int32_t) * nq, hipMemcpyHostToDevice));
     return search_entry(ix, ix->ws, d_q, nq, wmax, ix->ws.wq.as<int32_t>(), ix->ws.kq.as<int32_t>(), kmax, kmax, d_ids,
                         d_dis, static_cast<hipStream_t>(stream));
-}
+} MIVF_CATCH_ALL
 
-int mivf_coarse(mivf_index *ix, const float *d_q, int nq, int w, int32_t *d_idx, float *d_dis, void *stream) {
+int mivf_coarse(mivf_index *ix, const float *d_q, int nq, int w, int32_t *d_idx, float *d_dis, void *stream) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(w >= 1 && w <= ix->nlist && w <= 1024, kInvalid, "coarse: w must be in 1..min(nlist,1024)");
     if (nq <= 0) return kOk;
@@ -711,10 +712,10 @@ int mivf_coarse(mivf_index *ix, const float *d_q, int nq, int w, int32_t *d_idx,
         MIVF_TRY(coarse_stage(ix, ix->ws, d_q + q0 * ix->d, nc, w, d_idx + q0 * w, d_dis + q0 * w, st, nullptr));
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 int mivf_search_residual(mivf_index *ix, const float *d_residual, const int32_t *d_list_of, int nq, int w,
-                         int k, int32_t *d_ids, float *d_dis, void *stream) {
+                         int k, int32_t *d_ids, float *d_dis, void *stream) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(w >= 1 && k >= 1 && k <= 1024, kInvalid, "search_residual: bad w or k");
     if (nq <= 0) return kOk;
@@ -724,10 +725,30 @@ int mivf_search_residual(mivf_index *ix, const float *d_residual, const int32_t 
     MIVF_TRY(tm.start(false, static_cast<hipStream_t>(stream), nullptr));
     return search_core(ix, ix->ws, d_residual, nq, w, nullptr, nullptr, k, k, d_ids, d_dis,
                        static_cast<hipStream_t>(stream), d_list_of, tm);
-}
+} MIVF_CATCH_ALL
+
+int mivf_adc(mivf_index *ix, const float *d_residual, const uint8_t *d_codes, int64_t n, float *d_dis,
+             void *stream) try {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(n >= 0, kInvalid, "adc: negative code count");
+    if (n == 0) return kOk;
+    MIVF_REQUIRE(d_residual && d_codes && d_dis, kInvalid, "adc: null pointer");
+    MIVF_CHECK_HIP(launch_adc_codes(d_residual, ix->m, ix->ks, ix->ds, ix->pq.as<float>(), ix->pq_n64.as<double>(),
+                                    d_codes, n, d_dis, static_cast<hipStream_t>(stream)));
+    return kOk;
+} MIVF_CATCH_ALL
+
+int mivf_kmin(const float *d_dis, const int32_t *d_ids, int64_t n, int k, int32_t *d_out_ids, float *d_out_dis,
+              void *stream) try {
+    MIVF_REQUIRE(k >= 1 && k <= 1024, kInvalid, "kmin: k must be in 1..1024");
+    MIVF_REQUIRE(n >= 0 && n <= INT32_MAX, kInvalid, "kmin: n must be in 0..2^31-1");
+    MIVF_REQUIRE(d_out_ids && d_out_dis && (n == 0 || (d_dis && d_ids)), kInvalid, "kmin: null pointer");
+    MIVF_CHECK_HIP(launch_kmin(d_dis, d_ids, n, k, d_out_ids, d_out_dis, static_cast<hipStream_t>(stream)));
+    return kOk;
+} MIVF_CATCH_ALL
 
 int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, int k, const float *d_base,
-                       int64_t nbase, int32_t *d_ids, float *d_dis, void *stream) {
+                       int64_t nbase, int32_t *d_ids, float *d_dis, void *stream) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(w >= 1 && w <= ix->nlist && w <= 1024, kInvalid, "rerank: w must be in 1..min(nlist,1024)");
     MIVF_REQUIRE(kl >= 1 && kl <= 1024 && k >= 1 && k <= 1024, kInvalid, "rerank: kl and k must be in 1..1024");
@@ -755,10 +776,10 @@ int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
         (void)hipEventDestroy(e1);
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, int k, const float *d_base,
-                     int64_t nbase, int32_t *d_ids, float *d_dis, void *stream) {
+                     int64_t nbase, int32_t *d_ids, float *d_dis, void *stream) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(ix->m == 8 || ix->m == 16, kUnsupported, "search_fast: compiled for m in {8, 16}");
     MIVF_REQUIRE(ix->nlist <= 8192, kUnsupported, "search_fast: compiled for nlist <= 8192");
@@ -903,9 +924,9 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         ix->f_last_overflow = over;
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_index_attach_base(mivf_index *ix, const float *d_base, int64_t nbase) {
+int mivf_index_attach_base(mivf_index *ix, const float *d_base, int64_t nbase) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(d_base && nbase >= 0, kInvalid, "attach_base: null base");
     MIVF_TRY(upload_lists(ix));
@@ -918,9 +939,9 @@ int mivf_index_attach_base(mivf_index *ix, const float *d_base, int64_t nbase) {
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     ix->rows_ok = true;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t *d_t8, float *d_delta) {
+int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t *d_t8, float *d_delta) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(ix->m == 8 || ix->m == 16, kUnsupported, "fast_tables: compiled for m in {8, 16}");
     MIVF_REQUIRE(n >= 1 && n <= fast_group_size(), kInvalid, "fast_tables: need 1 <= n <= 16 queries");
@@ -935,9 +956,9 @@ int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t 
     MIVF_CHECK_HIP(launch_fast_tables(a, ix->m, list, n, d_t8, d_delta, nullptr));
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, float *d_tau) {
+int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, float *d_tau) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(ix->m == 8 || ix->m == 16, kUnsupported, "fast_seed_tau: compiled for m in {8, 16}");
     MIVF_REQUIRE(w >= 1 && w <= ix->nlist && w <= 1024 && kl >= 1, kInvalid, "fast_seed_tau: bad w or kl");
@@ -972,25 +993,25 @@ int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
     MIVF_CHECK_HIP(launch_seed_tau(a, ix->m, nullptr));
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_fast_stats(mivf_index *ix, int64_t *candidates, int64_t *overflowed_queries) {
+int mivf_fast_stats(mivf_index *ix, int64_t *candidates, int64_t *overflowed_queries) try {
     MIVF_TRY(check_index(ix));
     if (candidates) *candidates = ix->f_last_cands;
     if (overflowed_queries) *overflowed_queries = ix->f_last_overflow;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int nq, int kl, int32_t *d_short) {
+int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int nq, int kl, int32_t *d_short) try {
     MIVF_REQUIRE(capq >= 1 && nq >= 0 && kl >= 1, kInvalid, "fast_select: bad sizes");
     MIVF_REQUIRE(d_cand && d_cnt && d_short, kInvalid, "fast_select: null pointer");
     if (nq == 0) return kOk;
     MIVF_CHECK_HIP(launch_fast_select(kl, d_cand, d_cnt, capq, nq, d_short, nullptr));
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_set_option(mivf_index *ix, const char *name, int64_t value) {
+int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(name, kInvalid, "set_option: null name");
     const std::string n(name);
@@ -1026,24 +1047,24 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) {
         return kInvalid;
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_get_timings(mivf_index *ix, float *ms, int n) {
+int mivf_get_timings(mivf_index *ix, float *ms, int n) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(ms && n >= 0, kInvalid, "get_timings: bad arguments");
     for (int i = 0; i < n && i < 8; ++i) ms[i] = ix->timings[i];
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 int mivf_synth_rows(uint64_t seed, uint64_t stream_id, int64_t row0, int64_t n, int d, const float *d_centres,
-                    int ncomp, const uint64_t *d_thr, float sigma, float *d_out, void *stream) {
+                    int ncomp, const uint64_t *d_thr, float sigma, float *d_out, void *stream) try {
     MIVF_REQUIRE(n >= 0 && d > 0 && ncomp > 0 && d_centres && d_out, kInvalid, "synth_rows: bad arguments");
     MIVF_CHECK_HIP(launch_gen_rows(seed, stream_id, row0, n, d, d_centres, ncomp, d_thr, sigma, d_out,
                                    static_cast<hipStream_t>(stream)));
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_synth_centres(uint64_t seed, int ncomp, int d, float *centres) {
+int mivf_synth_centres(uint64_t seed, int ncomp, int d, float *centres) try {
     MIVF_REQUIRE(ncomp > 0 && d > 0 && centres, kInvalid, "synth_centres: bad arguments");
     const uint64_t b = synth_stream_base(seed, 1);
     for (int64_t i = 0; i < int64_t(ncomp) * d; ++i) {
@@ -1051,9 +1072,9 @@ int mivf_synth_centres(uint64_t seed, int ncomp, int d, float *centres) {
         centres[i] = float(double(u) * (1.0 / 9007199254740992.0) * 128.0);
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_zipf_thresholds(uint64_t seed, int ncomp, double s, uint64_t *thr) {
+int mivf_zipf_thresholds(uint64_t seed, int ncomp, double s, uint64_t *thr) try {
     MIVF_REQUIRE(ncomp > 0 && thr, kInvalid, "zipf_thresholds: bad arguments");
     std::vector<int> perm(ncomp), rank(ncomp);
     const uint64_t b = synth_stream_base(seed, 0x7a697066ull);
@@ -1073,10 +1094,10 @@ int mivf_zipf_thresholds(uint64_t seed, int ncomp, double s, uint64_t *thr) {
         thr[c] = (c == ncomp - 1 || t > 9007199254740992.0) ? (uint64_t(1) << 53) : uint64_t(t);
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 int mivf_bruteforce_nn(const float *d_base, int64_t nb, const float *d_q, int nq, int d, int32_t *d_ids,
-                       double *d_dis) {
+                       double *d_dis) try {
     MIVF_REQUIRE(d == 128, kUnsupported, "bruteforce_nn: d = 128 only");
     MIVF_REQUIRE(nb > 0 && nq >= 0 && d_base && d_q && d_ids && d_dis, kInvalid, "bruteforce_nn: bad arguments");
     if (nq == 0) return kOk;
@@ -1088,6 +1109,6 @@ int mivf_bruteforce_nn(const float *d_base, int64_t nb, const float *d_q, int nq
                                         nchunks, nullptr));
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 }  // extern "C"

@@ -104,6 +104,7 @@ struct mivf_index {
     std::vector<std::vector<int32_t>> h_ids;
     std::vector<std::vector<uint8_t>> h_codes;
     bool dirty = true;
+    uint64_t list_gen = 0;   // bumped by every upload of the lists (captured graphs compare it)
     int64_t ntotal = 0;
     std::vector<int64_t> h_off;
     DevBuf off, ids, codes;
@@ -124,8 +125,8 @@ struct mivf_index {
     int ppg_opt = 0;
     int query_order = 1;      // scan queries grouped by nearest list, per XCD (scheduling only)
     bool fast_running = true;  // running per-query bound in the reduced-precision scan
-    int fast_seed = 1;
-    int fast_seed_cap = 1024; // vectors entering the seed bound (0 = all of the seed lists)        // lists scanned exactly to seed the reduced-precision filter
+    int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
+    int fast_seed_cap = 1024; // vectors entering the seed bound (0 = all of the seed lists)
     int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
     int batched_cap = 8192;   // candidate capacity per query of the batched exact form (>= 2 kmax)
     int batched_g = 8, batched_nt = 512;  // its work-item shape (queries per item, threads)

@@ -54,12 +54,25 @@ int open_or_fail(File &fh, const char *path) {
     return kOk;
 }
 
+// Bytes from the current position to the end of the file (-1 if unknown).
+int64_t bytes_left(FILE *f) {
+    const long pos = std::ftell(f);
+    if (pos < 0 || std::fseek(f, 0, SEEK_END) != 0) return -1;
+    const long end = std::ftell(f);
+    if (std::fseek(f, pos, SEEK_SET) != 0 || end < pos) return -1;
+    return int64_t(end) - int64_t(pos);
+}
+
 // Read one list record (ids + int32 codes) and append it in uint8 form.
 int read_list_record(FILE *f, int m, std::vector<int32_t> &ids, std::vector<uint8_t> &codes,
                      const char *what) {
     int32_t n = 0, n2 = 0, mm = 0;
     MIVF_TRY(read_exact(f, &n, 4, 1, what));
     MIVF_REQUIRE(n >= 0, kIo, std::string("negative list length in ") + what);
+    // a list of n vectors takes 4n (ids) + 8 + 4nm (codes) more bytes: check
+    // the file holds them before sizing anything by n
+    MIVF_REQUIRE(int64_t(n) * 4 * (1 + int64_t(m)) + 8 <= bytes_left(f), kIo,
+                 std::string("list length larger than the file in ") + what);
     const size_t i0 = ids.size();
     ids.resize(i0 + size_t(n));
     MIVF_TRY(read_exact(f, ids.data() + i0, 4, size_t(n), what));
@@ -112,7 +125,7 @@ int finish_lists(int nlist, int m, std::vector<std::vector<int32_t>> &lid,
 extern "C" {
 
 int mivf_write_cent(const char *header, const char *cent, const char *coa, int nsq, int ks, int ds,
-                    int coarsek, int d, const float *pq, const float *coarse) {
+                    int coarsek, int d, const float *pq, const float *coarse) try {
     MIVF_REQUIRE(nsq > 0 && ks > 0 && ds > 0 && coarsek > 0 && d > 0 && pq && coarse, kInvalid,
                  "write_cent: bad arguments");
     File h(header, "wb"), c(cent, "wb"), a(coa, "wb");
@@ -124,9 +137,9 @@ int mivf_write_cent(const char *header, const char *cent, const char *coa, int n
     MIVF_TRY(write_exact(c.f, pq, 4, size_t(nsq) * ks * ds, cent));
     MIVF_TRY(write_exact(a.f, coarse, 4, size_t(coarsek) * d, coa));
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_read_cent_header(const char *header, int *nsq, int *ks, int *ds, int *coarsek, int *d) {
+int mivf_read_cent_header(const char *header, int *nsq, int *ks, int *ds, int *coarsek, int *d) try {
     File h(header, "rb");
     MIVF_TRY(open_or_fail(h, header));
     int32_t hdr[8];
@@ -142,9 +155,9 @@ int mivf_read_cent_header(const char *header, int *nsq, int *ks, int *ds, int *c
     if (coarsek) *coarsek = hdr[5];
     if (d) *d = hdr[7];
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_read_cent(const char *header, const char *cent, const char *coa, float *pq, float *coarse) {
+int mivf_read_cent(const char *header, const char *cent, const char *coa, float *pq, float *coarse) try {
     int nsq, ks, ds, coarsek, d;
     MIVF_TRY(mivf_read_cent_header(header, &nsq, &ks, &ds, &coarsek, &d));
     MIVF_REQUIRE(pq && coarse, kInvalid, "read_cent: null output");
@@ -154,10 +167,10 @@ int mivf_read_cent(const char *header, const char *cent, const char *coa, float 
     MIVF_TRY(read_exact(c.f, pq, 4, size_t(nsq) * ks * ds, cent));
     MIVF_TRY(read_exact(a.f, coarse, 4, size_t(coarsek) * d, coa));
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 int mivf_write_ivf_dir(const char *dir, int coarsek, int m, const int64_t *offsets, const int32_t *ids,
-                       const uint8_t *codes) {
+                       const uint8_t *codes) try {
     MIVF_REQUIRE(dir && coarsek > 0 && m > 0 && offsets, kInvalid, "write_ivf_dir: bad arguments");
     for (int j = 0; j < coarsek; ++j) {
         const std::string path = std::string(dir) + "/" + std::to_string(j);
@@ -169,11 +182,11 @@ int mivf_write_ivf_dir(const char *dir, int coarsek, int m, const int64_t *offse
         MIVF_TRY(write_list_record(f.f, m, ids + b, codes + b * m, n, path.c_str()));
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 // offsets [coarsek+1] always written; ids/codes may be NULL to query sizes.
 int mivf_read_ivf_dir(const char *dir, int coarsek, int m, int64_t *offsets, int32_t *ids, uint8_t *codes,
-                      int64_t cap) {
+                      int64_t cap) try {
     MIVF_REQUIRE(dir && coarsek > 0 && m > 0 && offsets, kInvalid, "read_ivf_dir: bad arguments");
     std::vector<std::vector<int32_t>> lid(coarsek);
     std::vector<std::vector<uint8_t>> lcodes(coarsek);
@@ -187,10 +200,10 @@ int mivf_read_ivf_dir(const char *dir, int coarsek, int m, int64_t *offsets, int
         MIVF_TRY(read_list_record(f.f, m, lid[j], lcodes[j], path.c_str()));
     }
     return finish_lists(coarsek, m, lid, lcodes, offsets, ids, codes, cap);
-}
+} MIVF_CATCH_ALL
 
 int mivf_write_ivf_file(const char *path, int coarsek, int m, const int64_t *offsets, const int32_t *ids,
-                        const uint8_t *codes) {
+                        const uint8_t *codes) try {
     MIVF_REQUIRE(path && coarsek > 0 && m > 0 && offsets, kInvalid, "write_ivf_file: bad arguments");
     File f(path, "wb");
     MIVF_TRY(open_or_fail(f, path));
@@ -201,10 +214,10 @@ int mivf_write_ivf_file(const char *path, int coarsek, int m, const int64_t *off
         MIVF_TRY(write_list_record(f.f, m, ids + b, codes + b * m, n, path));
     }
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 int mivf_read_ivf_file(const char *path, int coarsek, int m, int64_t *offsets, int32_t *ids, uint8_t *codes,
-                       int64_t cap) {
+                       int64_t cap) try {
     MIVF_REQUIRE(path && coarsek > 0 && m > 0 && offsets, kInvalid, "read_ivf_file: bad arguments");
     File f(path, "rb");
     MIVF_TRY(open_or_fail(f, path));
@@ -215,6 +228,6 @@ int mivf_read_ivf_file(const char *path, int coarsek, int m, int64_t *offsets, i
     std::vector<std::vector<uint8_t>> lcodes(coarsek);
     for (int j = 0; j < coarsek; ++j) MIVF_TRY(read_list_record(f.f, m, lid[j], lcodes[j], path));
     return finish_lists(coarsek, m, lid, lcodes, offsets, ids, codes, cap);
-}
+} MIVF_CATCH_ALL
 
 }  // extern "C"

@@ -7,6 +7,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <new>
+#include <stdexcept>
 #include <string>
 
 namespace mivf {
@@ -43,6 +45,19 @@ const char *last_error();
             return (code);             \
         }                              \
     } while (0)
+
+// Handler list of the function-try-block around every C-ABI body: a C++
+// exception (host allocation failure, above all) becomes a status code
+// instead of ending the caller's process.
+#define MIVF_CATCH_ALL                                                      \
+    catch (const std::bad_alloc &) {                                        \
+        ::mivf::set_error("out of host memory");                            \
+        return ::mivf::kNoMem;                                              \
+    }                                                                       \
+    catch (const std::exception &e_) {                                      \
+        ::mivf::set_error(std::string("internal error: ") + e_.what());     \
+        return ::mivf::kInvalid;                                            \
+    }
 
 #define MIVF_TRY(expr)              \
     do {                            \
@@ -270,6 +285,15 @@ hipError_t launch_rerank_pos(const float *rows, int64_t nrows, bool list_ordered
                              int kl, int k, int32_t *out_ids, float *out_dis, hipStream_t s);
 // out[r] = base[ids[r]] (rows of d floats)
 hipError_t launch_gather_rows(const float *base, int d, const int32_t *ids, int64_t n, float *out, hipStream_t s);
+
+// ADC distances of one residual [m*ds] over n codes [n][m] (uint8), in code
+// order, with the reference table arithmetic (adc.hip).
+hipError_t launch_adc_codes(const float *resid, int m, int ks, int ds, const float *pq, const double *pq_n64,
+                            const uint8_t *codes, int64_t n, float *out, hipStream_t s);
+// k smallest (distance, id) of dis/ids [n], ascending, padded with (-1, +inf)
+// to k entries; k <= 1024 (select.hip).
+hipError_t launch_kmin(const float *dis, const int32_t *ids, int64_t n, int k, int32_t *out_ids, float *out_dis,
+                       hipStream_t s);
 
 // Synthetic mixture rows (bit-identical to the CPU generator).
 uint64_t synth_stream_base(uint64_t seed, uint64_t stream);

@@ -88,6 +88,21 @@ __global__ __launch_bounds__(NT) void row_topk_kernel(const float *__restrict__ 
     sort_and_emit(buf, c, k, idx + row * k, dis + row * k);
 }
 
+// k smallest (distance, id) pairs of one candidate array, ascending: the
+// selection of parallel/ivf_pq/k_min.cpp:3-73 done exactly.
+__global__ __launch_bounds__(NT) void kmin_kernel(const float *__restrict__ dis, const int32_t *__restrict__ ids,
+                                                  int n, int k, int32_t *__restrict__ out_ids,
+                                                  float *__restrict__ out_dis) {
+    __shared__ SelectShared sh;
+    __shared__ uint64_t buf[KBUF];
+    __shared__ int cnt, eqt;
+    auto keyfn = [&](int i) { return ckey(dis[i], ids[i]); };
+    const int kk = k < n ? k : n;
+    SelectResult r = block_select<NT>(keyfn, n, kk, sh);
+    int c = collect(keyfn, n, r, buf, &cnt, &eqt);
+    sort_and_emit(buf, c, k, out_ids, out_dis);
+}
+
 __global__ __launch_bounds__(NT) void merge_kernel(const uint64_t *__restrict__ partial,
                                                    int ngroups, int kstride,
                                                    const int32_t *__restrict__ kq, int k,
@@ -212,6 +227,13 @@ hipError_t launch_row_topk(const float *D, int64_t rows, int n, int64_t ld, int 
     if (rows <= 0) return hipSuccess;
     if (k < 1 || k > KBUF) return hipErrorInvalidValue;
     row_topk_kernel<<<unsigned(rows), NT, 0, s>>>(D, n, ld, k, idx, dis);
+    return hipGetLastError();
+}
+
+hipError_t launch_kmin(const float *dis, const int32_t *ids, int64_t n, int k, int32_t *out_ids, float *out_dis,
+                       hipStream_t s) {
+    if (k < 1 || k > KBUF || n < 0 || n > INT32_MAX) return hipErrorInvalidValue;
+    kmin_kernel<<<1, NT, 0, s>>>(dis, ids, int(n), k, out_ids, out_dis);
     return hipGetLastError();
 }
 

@@ -38,7 +38,7 @@ struct Req {
     int32_t *d_ids;
     float *d_dis;
     double t_submit = 0, t_done = 0;
-    int state = 0;  // 0 waiting, 1 in flight, 2 done
+    int state = 0;  // 0 waiting, 1 in flight, 2 done, 3 failed at launch
     int64_t mem = 0;
 };
 
@@ -109,9 +109,9 @@ int retire(mivf_server *sv, int *completed) {
     return kOk;
 }
 
+int launch_taken(mivf_server *sv, Slot &s, const std::vector<int64_t> &take, int nb, int wmax, int kmax);
+
 int launch(mivf_server *sv, Slot &s) {
-    mivf_index *ix = sv->ix;
-    const int d = ix->d;
     int nb = 0, wmax = 1, kmax = 1;
     int64_t mem = 0;
     std::vector<int64_t> take;
@@ -129,6 +129,25 @@ int launch(mivf_server *sv, Slot &s) {
         kmax = std::max(kmax, r.k);
     }
     if (take.empty()) return kOk;
+    const int rc = launch_taken(sv, s, take, nb, wmax, kmax);
+    if (rc != kOk) {
+        // the requests left the queue: mark them failed so that wait and
+        // latency report the error instead of waiting for them
+        for (int64_t id : take) sv->reqs[id].state = 3;
+        return rc;
+    }
+    s.members = std::move(take);
+    s.busy = true;
+    s.mem = mem;
+    sv->in_flight += mem;
+    ++sv->batches;
+    return kOk;
+}
+
+// Size the slot's buffers for the taken requests and enqueue their search.
+int launch_taken(mivf_server *sv, Slot &s, const std::vector<int64_t> &take, int nb, int wmax, int kmax) {
+    mivf_index *ix = sv->ix;
+    const int d = ix->d;
     MIVF_TRY(s.q.ensure(sizeof(float) * size_t(nb) * d));
     MIVF_TRY(s.ids.ensure(sizeof(int32_t) * size_t(nb) * kmax));
     MIVF_TRY(s.dis.ensure(sizeof(float) * size_t(nb) * kmax));
@@ -167,11 +186,6 @@ int launch(mivf_server *sv, Slot &s) {
         off += r.nq;
     }
     MIVF_CHECK_HIP(hipEventRecord(s.ev, s.st));
-    s.members = std::move(take);
-    s.busy = true;
-    s.mem = mem;
-    sv->in_flight += mem;
-    ++sv->batches;
     return kOk;
 }
 
@@ -186,7 +200,7 @@ int progress_impl(mivf_server *sv, int *completed) {
 
 }  // namespace
 
-int mivf_server_create(mivf_index *ix, int slots, int max_batch, int64_t mem_budget, mivf_server **out) {
+int mivf_server_create(mivf_index *ix, int slots, int max_batch, int64_t mem_budget, mivf_server **out) try {
     MIVF_REQUIRE(ix && out, kInvalid, "server_create: null argument");
     MIVF_REQUIRE(slots >= 1 && slots <= 16, kInvalid, "server_create: slots must be in 1..16");
     MIVF_REQUIRE(max_batch >= 1 && max_batch <= (1 << 16), kInvalid, "server_create: max_batch must be in 1..65536");
@@ -211,15 +225,15 @@ int mivf_server_create(mivf_index *ix, int slots, int max_batch, int64_t mem_bud
     }
     *out = sv;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_server_free(mivf_server *sv) {
+int mivf_server_free(mivf_server *sv) try {
     delete sv;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 int mivf_server_submit(mivf_server *sv, const float *d_q, int nq, int w, int k, int32_t *d_ids, float *d_dis,
-                       int64_t *ticket) {
+                       int64_t *ticket) try {
     MIVF_REQUIRE(sv && ticket, kInvalid, "server_submit: null argument");
     MIVF_REQUIRE(nq >= 1 && d_q && d_ids && d_dis, kInvalid, "server_submit: empty request or null pointer");
     MIVF_REQUIRE(w >= 1 && w <= sv->ix->nlist && w <= 1024 && k >= 1 && k <= 1024, kInvalid,
@@ -237,23 +251,27 @@ int mivf_server_submit(mivf_server *sv, const float *d_q, int nq, int w, int k, 
     sv->reqs.push_back(r);
     sv->waiting.push_back(*ticket);
     return progress_impl(sv, nullptr);
-}
+} MIVF_CATCH_ALL
 
-int mivf_server_progress(mivf_server *sv, int *completed) {
+int mivf_server_progress(mivf_server *sv, int *completed) try {
     MIVF_REQUIRE(sv, kInvalid, "server_progress: null server");
     int c = 0;
     MIVF_TRY(progress_impl(sv, &c));
     if (completed) *completed = c;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
-int mivf_server_wait(mivf_server *sv, int64_t ticket) {
+int mivf_server_wait(mivf_server *sv, int64_t ticket) try {
     MIVF_REQUIRE(sv && ticket >= 0 && ticket < int64_t(sv->reqs.size()), kInvalid, "server_wait: bad ticket");
-    while (sv->reqs[ticket].state != 2) MIVF_TRY(progress_impl(sv, nullptr));
-    return kOk;
-}
+    for (;;) {
+        const int st = sv->reqs[ticket].state;
+        if (st == 2) return kOk;
+        MIVF_REQUIRE(st != 3, kHip, "server_wait: the request's micro-batch failed to launch");
+        MIVF_TRY(progress_impl(sv, nullptr));
+    }
+} MIVF_CATCH_ALL
 
-int mivf_server_drain(mivf_server *sv) {
+int mivf_server_drain(mivf_server *sv) try {
     MIVF_REQUIRE(sv, kInvalid, "server_drain: null server");
     for (;;) {
         MIVF_TRY(progress_impl(sv, nullptr));
@@ -261,15 +279,16 @@ int mivf_server_drain(mivf_server *sv) {
         for (auto &s : sv->slots) busy = busy || s.busy;
         if (!busy) return kOk;
     }
-}
+} MIVF_CATCH_ALL
 
-int mivf_server_latency(mivf_server *sv, int64_t ticket, double *ms) {
+int mivf_server_latency(mivf_server *sv, int64_t ticket, double *ms) try {
     MIVF_REQUIRE(sv && ms && ticket >= 0 && ticket < int64_t(sv->reqs.size()), kInvalid, "server_latency: bad ticket");
     const Req &r = sv->reqs[ticket];
+    MIVF_REQUIRE(r.state != 3, kHip, "server_latency: the request's micro-batch failed to launch");
     MIVF_REQUIRE(r.state == 2, kInvalid, "server_latency: request not finished");
     *ms = r.t_done - r.t_submit;
     return kOk;
-}
+} MIVF_CATCH_ALL
 
 // Open-loop load: nreq requests with exponential inter-arrival times at
 // `rate` requests/s, nq log-uniform in 1..bmax, w and k drawn from the given
@@ -277,7 +296,7 @@ int mivf_server_latency(mivf_server *sv, int64_t ticket, double *ms) {
 // scheduled arrival (a request is submitted as soon as its time has come).
 int mivf_serve_bench(mivf_index *ix, int slots, int max_batch, int64_t mem_budget, const float *d_pool,
                      int64_t pool_n, double rate, int nreq, uint64_t seed, int bmax, const int32_t *w_choices,
-                     int nw, const int32_t *k_choices, int nk, double *stats) {
+                     int nw, const int32_t *k_choices, int nk, double *stats) try {
     MIVF_REQUIRE(ix && d_pool && stats && nreq >= 1 && rate > 0 && bmax >= 1 && pool_n >= bmax && nw >= 1 &&
                      nk >= 1 && w_choices && k_choices,
                  kInvalid, "serve_bench: bad arguments");
@@ -359,4 +378,4 @@ int mivf_serve_bench(mivf_index *ix, int slots, int max_batch, int64_t mem_budge
     stats[7] = double(nqt) / (plan.back().t / 1000.0);    // offered queries/s
     delete sv;
     return kOk;
-}
+} MIVF_CATCH_ALL

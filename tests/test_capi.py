@@ -45,8 +45,24 @@ def test_compat_symbols_have_reference_mangling(L):
                 "_Z10write_centPcS_S_5ivfpq",
                 "_Z9read_centPcS_S_P5ivfpq",
                 "_Z8read_ivf5ivfpqiiPc",
-                "_Z12ivfpq_search5ivfpqP3ivf3matS2_iiiPiPf"]:
+                "_Z12ivfpq_search5ivfpqP3ivf3matS2_iiiPiPf",
+                # parallel/ fork: per-list search and my_k_min (parallel/ivf_pq/ivf_search.h:39, k_min.h:15)
+                "_Z12ivfpq_searchP3ivfPf6pqtipoiPdS3_",
+                "_Z8my_k_min5dis_tiPfPi"]:
         assert hasattr(L, sym), sym
+
+
+@pytest.mark.parametrize("fork", ["parallel2", "sequential", "parallel"])
+def test_compat_callers_load(L, fork):
+    """The C++ callers built against each fork's header (tests/compat/) were
+    linked with --no-undefined against libmivf and load here; their entry
+    points exist.  (They run on the GPU in test_gpu_compat.py.)"""
+    path = os.path.join(ROOT, "tests", "compat", f"libcaller_{fork}.so")
+    assert os.path.exists(path), "build() compiles the callers"
+    C = ctypes.CDLL(path)
+    for fn in {"parallel2": ["run_core_gpu", "run_build", "run_read_ivf"], "sequential": ["run_seq_search"],
+               "parallel": ["run_par_search", "run_par_list", "run_kmin"]}[fork]:
+        assert hasattr(C, fn), fn
 
 
 def test_read_reference_centroid_files(L):
@@ -124,6 +140,21 @@ def test_ivf_roundtrip(L, tmp_path, kind):
     if n > 1:
         assert r(path, nlist, m, off2.ctypes.data, ids2.ctypes.data, codes2.ctypes.data,
                  ctypes.c_int64(n - 1)) == 1
+
+
+def test_ivf_file_oversized_list_length(L, tmp_path):
+    """A list length far beyond the file's size is reported as a corrupt file
+    (status 4) before anything is sized by it."""
+    path = tmp_path / "ivf"
+    np.array([2, 1 << 30, 5, 6], np.int32).tofile(str(path))  # coarsek 2, list 0 claims 2^30 vectors
+    off = np.empty(3, np.int64)
+    assert L.mivf_read_ivf_file(str(path).encode(), 2, 8, off.ctypes.data, None, None, ctypes.c_int64(0)) == 4
+    assert b"larger than the file" in L.mivf_last_error()
+    d = tmp_path / "dir"
+    d.mkdir()
+    for j in range(2):
+        np.array([2, 1 << 29, 1, 2], np.int32).tofile(str(d / str(j)))
+    assert L.mivf_read_ivf_dir(str(d).encode(), 2, 8, off.ctypes.data, None, None, ctypes.c_int64(0)) == 4
 
 
 def test_ivf_dir_layout_matches_reference(L, tmp_path):

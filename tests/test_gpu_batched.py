@@ -112,3 +112,30 @@ def test_exact_tables_and_scan_adc_bitexact(mivf, orc, zipf16):
             assert np.array_equal(t[:, :, g].view(np.uint32), ref.view(np.uint32)), f"table list {l} slot {g}"
             adc = orc.adc(ref, codes[off[l]:off[l + 1]])
             assert np.array_equal(d[:, g].view(np.uint32), adc.view(np.uint32)), f"ADC list {l} slot {g}"
+
+
+def test_plan_follows_list_changes(mivf, orc, c1):
+    """A captured plan holds the lists' device addresses: after an add (which
+    grows and moves the lists) and another search in between, the next
+    plan.search re-captures and returns the results of the new lists."""
+    q = c1["gold_q"]
+    ix = mivf.Index(q["coarse"], q["pq"])
+    base = c1["base"]
+    ix.add(base[:3000])
+    query = c1["query"]
+    w, k = uneven(query.shape[0], seed=41, wmax=32)
+    plan = mivf.Plan(ix, query.shape[0], 32, 100, autotune=False)
+    try:
+        a = plan.search(query, w, k)
+        b = ix.search_var(query, w, k, 100)
+        assert_same(a[0], a[1], b[0], b[1], "plan before the add")
+        ix.add(base[3000:], id0=3000)
+        ix.search(query[:5], 4, 10)  # uploads (and reallocates) the grown lists
+        a = plan.search(query, w, k)
+        index = (q["coarse"], q["pq"]) + tuple(ix.get_lists())
+        r = orc.search_var(index, query, w, k, 100)
+        assert_same(a[0], a[1], r[0], r[1], "plan after the add")
+        a2 = plan.search(query, w, k)  # replay of the new graph
+        assert_same(a2[0], a2[1], r[0], r[1], "plan replay after the add")
+    finally:
+        plan.close()
