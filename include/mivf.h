@@ -107,6 +107,22 @@ int mivf_index_encode(mivf_index *ix, const float *d_x, int64_t n, int32_t *d_as
  * (parallel/ivf_pq/ivf_search.cpp:97-137).  d_q [nq][d]; outputs [nq][k].   */
 int mivf_search(mivf_index *ix, const float *d_q, int nq, int w, int k,
                 int32_t *d_ids, float *d_dis, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mivf_search with its scratch (coarse distance matrix,
+ * probes, residual rows, partial top-k lists) carved out of one fixed device
+ * array of 64 MiB per GPU, held by the library until the process ends,
+ * instead of the index's workspace.  Same results as mivf_search.  It
+ * allocates nothing, so (1) the lists must already be on the device: after
+ * mivf_index_add / set_lists, one call of any other search uploads them
+ * (MIVF_EINVAL until then), and (2) a batch needing more than 64 MiB is
+ * rejected with MIVF_ENOMEM (per query about 4 nlist + w (4 d + 8 m + 8)
+ * bytes, plus 8 w k when nq < 2048 splits the probes over workgroups; e.g.
+ * about 1,000 queries at nlist 4096, w 32, k 100).  Every call on one GPU
+ * uses the same array and leaves its contents there: calls that overlap on
+ * one GPU (other streams, other threads) race and must be serialised.      */
+int mivf_search_noalloc(mivf_index *ix, const float *d_q, int nq, int w, int k,
+                        int32_t *d_ids, float *d_dis, void *stream);
 /* Per-query w and k (host arrays w[nq], k[nq]); output rows kmax wide.     */
 int mivf_search_var(mivf_index *ix, const float *d_q, int nq, const int32_t *w,
                     const int32_t *k, int32_t *d_ids, float *d_dis, int kmax,

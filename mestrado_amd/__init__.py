@@ -51,6 +51,7 @@ _SIGS = {
     "mivf_index_add_device": [_vp, _vp, _i64, ctypes.c_int32],
     "mivf_index_encode": [_vp, _vp, _i64, _vp, _vp, _vp],
     "mivf_search": [_vp, _vp, _i, _i, _i, _vp, _vp, _vp],
+    "mivf_search_noalloc": [_vp, _vp, _i, _i, _i, _vp, _vp, _vp],
     "mivf_search_var": [_vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp],
     "mivf_coarse": [_vp, _vp, _i, _i, _vp, _vp, _vp],
     "mivf_search_residual": [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp],

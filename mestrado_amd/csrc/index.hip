@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -424,6 +425,65 @@ int search_entry(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, con
 
 }  // namespace mivf
 
+// ---------------------------------------------------------------------------
+// Search without allocation (mivf_search_noalloc): the workspace of one plain
+// search carved out of a fixed device array.  Every GPU the library runs on
+// holds its own copy of the array (kNoAllocScratch bytes) until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device, never through the symbol's name.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratch = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_at[kMaxDevices] = {};  // per device: the address of `scratch` there
+
+// The current device's scratch address (looked up once per device).
+int scratch_address(void **out) {
+    int dev = 0;
+    MIVF_CHECK_HIP(hipGetDevice(&dev));
+    MIVF_REQUIRE(dev >= 0 && dev < kMaxDevices, kUnsupported, "search_noalloc: device index above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_at[dev]) {
+        void *p = nullptr;
+        MIVF_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        MIVF_REQUIRE(p != nullptr, kHip, "search_noalloc: no device address for the scratch array");
+        g_scratch_at[dev] = p;
+    }
+    *out = g_scratch_at[dev];
+    return kOk;
+}
+
+// Lay the buffers search_core asks for (nq queries, w probes, k results)
+// out from base, 256-byte aligned, as views in ws (base == nullptr: only
+// count); returns the bytes used.
+size_t noalloc_layout(const mivf_index *ix, int nq, int w, int k, unsigned char *base, Ws &ws) {
+    size_t off = 0;
+    auto take = [&](DevBuf &b, size_t bytes) {
+        bytes = (bytes + 255) & ~size_t(255);
+        if (base) b.view(base + off, bytes);
+        off += bytes;
+    };
+    const size_t q = size_t(nq), qw = q * size_t(w);
+    int ppg = 0;
+    const int ngroups = choose_groups(ix, nq, w, &ppg);
+    take(ws.qn, sizeof(double) * q);
+    take(ws.dmat, sizeof(float) * q * size_t(ix->nlist));
+    take(ws.probes, sizeof(int32_t) * qw);
+    take(ws.coadis, sizeof(float) * qw);
+    take(ws.redo, sizeof(int32_t) * (q + 1));
+    take(ws.qorder, sizeof(int32_t) * q);
+    take(ws.rres, sizeof(float) * qw * size_t(ix->d));
+    take(ws.rnd, sizeof(double) * qw * size_t(ix->m));
+    take(ws.partial, ngroups > 1 ? sizeof(uint64_t) * q * size_t(ngroups) * size_t(k) : 0);
+    return off;
+}
+
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -676,6 +736,33 @@ int mivf_search(mivf_index *ix, const float *d_q, int nq, int w, int k, int32_t 
     if (nq == 0) return kOk;
     MIVF_REQUIRE(d_q && d_ids && d_dis, kInvalid, "search: null pointer");
     return search_entry(ix, ix->ws, d_q, nq, w, nullptr, nullptr, k, k, d_ids, d_dis, static_cast<hipStream_t>(stream));
+} MIVF_CATCH_ALL
+
+int mivf_search_noalloc(mivf_index *ix, const float *d_q, int nq, int w, int k, int32_t *d_ids, float *d_dis,
+                        void *stream) try {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(nq >= 0, kInvalid, "search_noalloc: nq < 0");
+    MIVF_REQUIRE(w >= 1 && w <= ix->nlist, kInvalid, "search_noalloc: w must be in 1..nlist");
+    MIVF_REQUIRE(w <= 1024, kUnsupported, "search_noalloc: w > 1024");
+    MIVF_REQUIRE(k >= 1 && k <= 1024, kInvalid, "search_noalloc: k must be in 1..1024");
+    if (nq == 0) return kOk;
+    MIVF_REQUIRE(d_q && d_ids && d_dis, kInvalid, "search_noalloc: null pointer");
+    MIVF_REQUIRE(!ix->dirty, kInvalid,
+                 "search_noalloc: the lists changed since they were last put on the device; one call of any "
+                 "other search (which may allocate) uploads them");
+    Ws ws;
+    const size_t need = noalloc_layout(ix, nq, w, k, nullptr, ws);
+    MIVF_REQUIRE(need <= kNoAllocScratch, kNoMem,
+                 "search_noalloc: " + std::to_string(nq) + " queries at w = " + std::to_string(w) + ", k = " +
+                     std::to_string(k) + " need " + std::to_string(need) + " bytes of scratch, above the fixed " +
+                     std::to_string(kNoAllocScratch) + "; split the batch");
+    void *base = nullptr;
+    MIVF_TRY(scratch_address(&base));
+    noalloc_layout(ix, nq, w, k, static_cast<unsigned char *>(base), ws);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    Timer tm;
+    MIVF_TRY(tm.start(false, st, nullptr));
+    return search_core(ix, ws, d_q, nq, w, nullptr, nullptr, k, k, d_ids, d_dis, st, nullptr, tm);
 } MIVF_CATCH_ALL
 
 int mivf_search_var(mivf_index *ix, const float *d_q, int nq, const int32_t *w, const int32_t *k,

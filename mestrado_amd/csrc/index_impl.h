@@ -22,8 +22,20 @@ struct DevBuf {
     // without the device-wide wait of hipFree.
     hipStream_t ord = nullptr;
     bool async = false;
+    // A view into memory the buffer does not own (a slice of a fixed scratch
+    // array): it never allocates or frees, and a request beyond its size fails.
+    bool fixed = false;
+    void view(void *at, size_t n) {
+        p = at;
+        bytes = n;
+        fixed = true;
+    }
     int ensure(size_t want) {
         if (want <= bytes) return kOk;
+        if (fixed) {
+            set_error("scratch view of " + std::to_string(bytes) + " bytes, " + std::to_string(want) + " needed");
+            return kNoMem;
+        }
         if (p) {
             if (async)
                 (void)hipFreeAsync(p, ord);
@@ -44,7 +56,7 @@ struct DevBuf {
     template <typename T>
     T *as() const { return static_cast<T *>(p); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !fixed) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
     }
