@@ -2,9 +2,14 @@
 
 Workload (default): 10M x 128 base from the seeded Zipf (s=1.1) Gaussian
 mixture, IVF with 4096 lists, PQ m=16 x 8 bit, 10,000 queries, w=32, k=100,
-plain exact FP32 form.  One step = one search of the whole 10k-query batch
-(coarse assignment, probe selection, fused table + ADC scan + top-k, merge)
-with queries, index and outputs resident in HBM.  Prints one JSON line.
+plain exact FP32 form (ids and distances bit-identical to the reference
+arithmetic).  One step = one search of the whole 10k-query batch with
+queries, index and outputs resident in HBM; at this batch size mivf_search
+runs the list-major kernels (coarse top-w, seed bounds, list-major grouping,
+exact scan, select).  The per-query kernels are timed beside it, then the
+reduced-precision form (operating point chosen on held-out queries), the
+batched uneven form, serving, and the CPU baseline (the oracle on the host
+cores), whose results are compared with the GPU's.  Prints one JSON line.
 """
 import argparse
 import json
@@ -24,6 +29,10 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def same_bits(a_ids, a_dis, b_ids, b_dis):
+    return bool(np.array_equal(a_ids, b_ids) and np.array_equal(a_dis.view(np.uint32), b_dis.view(np.uint32)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -35,14 +44,15 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5, help="timed passes of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gt-queries", type=int, default=10000, help="queries with an FP64 ground truth")
+    ap.add_argument("--tune-queries", type=int, default=10000,
+                    help="held-out queries (the next rows of the query stream) that choose the "
+                         "reduced-precision operating point")
     ap.add_argument("--fast-kl", type=int, default=0, help="shortlist of the reduced-precision form (0: sweep --fast-kls)")
-    ap.add_argument("--fast-kls", default="128,160,176,192", help="shortlist sizes of the reduced-precision sweep")
+    ap.add_argument("--fast-kls", default="128,160,176,192,224", help="shortlist sizes of the reduced-precision sweep")
     ap.add_argument("--fast-seed", type=int, default=1, help="lists scanned exactly to seed its filter")
     ap.add_argument("--fast-seed-cap", type=int, default=-1,
                     help="vectors entering the seed bound (0 = whole seed lists, -1 = library default)")
-    ap.add_argument("--fast-w", type=int, default=0,
-                    help="probes of the reduced-precision form (0 = the workload's w); its recall@100 is "
-                         "reported beside the plain form's")
+    ap.add_argument("--fast-w", type=int, default=0, help="probes of the reduced-precision form (0 = sweep)")
     ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
@@ -78,7 +88,7 @@ def main():
 
     t0 = time.time()
     coarse, pq = M.train(train, cfg["nlist"], cfg["m"], 256, cfg["niter_coarse"], cfg["niter_pq"], seed)
-    log(f"trained in {time.time() - t0:.1f}s")
+    log(f"trained in {time.time() - t0:.1f}s ({cfg['niter_coarse']} coarse, {cfg['niter_pq']} PQ k-means iterations)")
     train.free()
     t0 = time.time()
     ix = M.Index(coarse, pq)
@@ -87,33 +97,60 @@ def main():
     log(f"encoded {ix.ntotal} vectors in {time.time() - t0:.1f}s")
 
     nq, w, k, m = cfg["nquery"], cfg["w"], cfg["k"], cfg["m"]
-    out_ids = M.DeviceArray((nq, k), np.int32)
-    out_dis = M.DeviceArray((nq, k), np.float32)
-
-    # algorithmic scan bytes: codes of every probed list, once per (query, probe)
-    probes, _ = ix.coarse(query, w)
     lens = np.diff(off)
+    probes, _ = ix.coarse(query, w)
+    # algorithmic scan bytes: codes of every probed list, once per (query, probe)
     scan_vectors = int(lens[probes].sum())
     scan_bytes = scan_vectors * m
+    # FP32 table entries read by an exact scan: 4 bytes per (vector, query, subquantizer)
+    table_bytes = scan_vectors * m * 4
 
-    for _ in range(args.warmup):
-        ix.search(query, w, k, out=(out_ids, out_dis))
-    M.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        ix.search(query, w, k, out=(out_ids, out_dis))
-    M.synchronize()
-    elapsed = time.perf_counter() - t_start
-    # per-stage times from HIP events in a separate pass (a profiled search
-    # waits for its last event, so the timed loop above runs without them)
-    ix.set_option("profile", 1)
-    stage = np.zeros(8)
-    for _ in range(args.steps):
-        ix.search(query, w, k, out=(out_ids, out_dis))
-        stage += ix.timings()
-    ix.set_option("profile", 0)
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        M.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        M.synchronize()
+        return (time.perf_counter() - t) / args.steps
 
-    # recall against the exact FP64 nearest neighbour
+    def stages(fn):
+        # per-stage times from HIP events in a separate pass (a profiled search
+        # waits for its last event, so the timed loops run without them)
+        ix.set_option("profile", 1)
+        st = np.zeros(8)
+        for _ in range(args.steps):
+            fn()
+            st += ix.timings()
+        ix.set_option("profile", 0)
+        return st / args.steps
+
+    # ---- plain exact form: mivf_search (list-major kernels at this batch size)
+    out_ids = M.DeviceArray((nq, k), np.int32)
+    out_dis = M.DeviceArray((nq, k), np.float32)
+    ix.set_option("list_major", 1)
+    plain_fn = lambda: ix.search(query, w, k, out=(out_ids, out_dis))  # noqa: E731
+    t_plain = timed(plain_fn)
+    st_plain = stages(plain_fn)
+    # ---- the same results from the per-query kernels (option list_major = 0)
+    pq_ids = M.DeviceArray((nq, k), np.int32)
+    pq_dis = M.DeviceArray((nq, k), np.float32)
+    ix.set_option("list_major", 0)
+    pq_fn = lambda: ix.search(query, w, k, out=(pq_ids, pq_dis))  # noqa: E731
+    t_pq = timed(pq_fn)
+    st_pq = stages(pq_fn)
+    ix.set_option("list_major", 1)
+    out_h = (out_ids.numpy(), out_dis.numpy())
+    paths_same = same_bits(*out_h, pq_ids.numpy(), pq_dis.numpy())
+    log(f"plain: {nq / t_plain:.0f} q/s (list-major), {nq / t_pq:.0f} q/s (per-query kernels), same {paths_same}")
+
+    # ---- the LDS read ceilings of this device (the scan kernels' table reads)
+    lds = {"random16": M.lds_read_rate(16, True), "peak16": M.lds_read_rate(16, False),
+           "random4": M.lds_read_rate(4, True), "peak4": M.lds_read_rate(4, False)}
+    log(f"LDS read rates GB/s: {lds}")
+
+    # ---- recall against the exact FP64 nearest neighbour
     ng = min(args.gt_queries, nq)
     if not args.profile_only:
         qsub = M.DeviceArray.from_numpy(query.numpy()[:ng])
@@ -121,11 +158,15 @@ def main():
         base_h = base.numpy()  # host copy for the FP64 distances of returned ids (untimed)
         q64 = qsub.numpy().astype(np.float64)
 
+    def recall_of(ids_h, g):
+        n = g.shape[0]
+        return {f"recall@{R}": round(100.0 * float((ids_h[:n, :R] == g[:, None]).any(1).mean()), 3)
+                for R in (1, 10, 100) if R <= k}
+
     def recall(ids_h, dis_h):
         if args.profile_only:
             return {}
-        r = {f"recall@{R}": round(100.0 * float(np.mean([gt[i] in ids_h[i, :R] for i in range(ng)])), 3)
-             for R in (1, 10, 100) if R <= k}
+        r = recall_of(ids_h, gt)
         # distance error against FP64: (1) the returned distance of each
         # returned id vs its FP64 squared distance, (2) the first result's
         # FP64 distance vs the exact nearest neighbour's
@@ -141,66 +182,107 @@ def main():
         r["top1_dist_gap_vs_fp64_nn"] = {"mean": float(f"{gap.mean():.3g}"), "max": float(f"{gap.max():.3g}")}
         return r
 
-    rec = recall(out_ids.numpy(), out_dis.numpy())
-    qps = nq * args.steps / elapsed
+    rec = recall(*out_h)
 
+    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
+    # recorded for this workload (null when none is recorded)
+    def recorded(fname, key):
+        path = os.path.join(ROOT, "profiles", f"{fname}_{args.workload}.md")
+        if os.path.exists(path) and not args.nbase:
+            for line in open(path):
+                if line.startswith(f"{key}:"):
+                    return float(line.split(":", 1)[1])
+        return None
+
+    def rf(x, nd=4):
+        return None if x is None else round(x, nd)
+
+    # ---- reduced-precision form + exact rerank, operating point chosen on
+    # held-out queries (the next rows of the query stream)
     fast = None
     if not args.no_fast:
-        # The reduced-precision form is reported at the plain form's recall:
-        # probes w' and shortlist sizes kl in a short sweep, the fastest
-        # whose recall@100 is within 0.5 points of the plain form's (or the
-        # --fast-w / --fast-kl given).
         kls = [args.fast_kl] if args.fast_kl else [int(x) for x in args.fast_kls.split(",")]
-        if args.profile_only:
-            kls = kls[:1]
-        f_ids = M.DeviceArray((nq, k), np.int32)
-        f_dis = M.DeviceArray((nq, k), np.float32)
-        ix.set_option("fast_seed_lists", args.fast_seed)
-        if args.fast_seed_cap >= 0:
-            ix.set_option("fast_seed_cap", args.fast_seed_cap)
-        ix.attach_base(base)  # list-ordered copy of the raw vectors (index build, untimed)
         if args.fast_w:
             sweep_w = [args.fast_w]
         elif args.profile_only:
             sweep_w = [w]
         else:
             sweep_w = sorted({max(1, w - d) for d in (0, 4, 8, 10, 11, 12, 13, 14, 15, 16)}, reverse=True)
+        if args.profile_only:
+            kls = kls[:1]
+        ix.set_option("fast_seed_lists", args.fast_seed)
+        if args.fast_seed_cap >= 0:
+            ix.set_option("fast_seed_cap", args.fast_seed_cap)
+        ix.attach_base(base)  # list-ordered copy of the raw vectors (index build, untimed)
+        f_ids = M.DeviceArray((nq, k), np.int32)
+        f_dis = M.DeviceArray((nq, k), np.float32)
         sweep = []
-        for kl, fw in [(kl, fw) for kl in kls for fw in sweep_w]:
-            for _ in range(args.warmup):
-                ix.search_fast(query, fw, kl, k, out=(f_ids, f_dis))
-            M.synchronize()
-            t_f = time.perf_counter()
-            for _ in range(args.steps):
-                ix.search_fast(query, fw, kl, k, out=(f_ids, f_dis))
-            M.synchronize()
-            f_elapsed = time.perf_counter() - t_f
-            ix.set_option("profile", 1)  # stage times and candidate counts, separate pass
-            fstage = np.zeros(8)
-            for _ in range(args.steps):
-                ix.search_fast(query, fw, kl, k, out=(f_ids, f_dis))
-                fstage += ix.timings()
-            ix.set_option("profile", 0)
-            fstage /= args.steps
-            cands, over = ix.fast_stats()
-            f = {"qps": round(nq * args.steps / f_elapsed, 1), "ms_per_step": round(f_elapsed * 1000 / args.steps, 4),
-                 "w": fw, "kl": kl, "seed_lists": args.fast_seed, "candidates_per_query": round(cands / nq, 1),
-                 "overflowed_queries": over,
-                 "stages_ms": {n: round(float(v), 4) for n, v in
-                               zip(("coarse", "seed", "pairs", "scan8", "select", "rerank"), fstage)}}
-            f.update(recall(f_ids.numpy(), f_dis.numpy()))
-            sweep.append(f)
-            log(f"fast form w={fw} kl={kl}: qps {f['qps']} recall@100 {f.get('recall@100')}")
-        ok = [f for f in sweep if f.get("recall@100", 0.0) >= rec.get("recall@100", 0.0) - 0.5]
-        fast = dict(max(ok, key=lambda f: f["qps"]) if ok else sweep[0])
-        fast["recall_rule"] = "fastest (probes, shortlist) whose recall@100 is within 0.5 points of the plain form's"
-        fast["sweep"] = [{"w": f["w"], "kl": f["kl"], "qps": f["qps"], "recall@100": f.get("recall@100")} for f in sweep]
+        if len(kls) * len(sweep_w) > 1 and not args.profile_only:
+            ntune = args.tune_queries
+            qt = M.synth_rows(seed, STREAM_QUERY, nq, ntune, cent, thr, SIGMA)
+            gtt, _ = M.bruteforce_nn(base, qt)
+            t_ids = M.DeviceArray((ntune, k), np.int32)
+            t_dis = M.DeviceArray((ntune, k), np.float32)
+            ix.search(qt, w, k, out=(t_ids, t_dis))
+            plain_tune = recall_of(t_ids.numpy(), gtt)["recall@100"]
+            for kl, fw in [(kl, fw) for kl in kls for fw in sweep_w]:
+                tt = timed(lambda: ix.search_fast(qt, fw, kl, k, out=(t_ids, t_dis)))
+                r = recall_of(t_ids.numpy(), gtt)
+                sweep.append({"w": fw, "kl": kl, "qps": round(ntune / tt, 1), "recall@100": r["recall@100"]})
+                log(f"fast form (tuning queries) w={fw} kl={kl}: {ntune / tt:.0f} q/s recall@100 {r['recall@100']}")
+            ok = [f for f in sweep if f["recall@100"] >= plain_tune - 0.5]
+            pick = max(ok, key=lambda f: f["qps"]) if ok else max(sweep, key=lambda f: f["recall@100"])
+            fw, kl = pick["w"], pick["kl"]
+            qt.free()
+        else:
+            fw, kl, plain_tune = sweep_w[0], kls[0], None
+        fast_fn = lambda: ix.search_fast(query, fw, kl, k, out=(f_ids, f_dis))  # noqa: E731
+        t_fast = timed(fast_fn)
+        fst = stages(fast_fn)
+        cands, over = ix.fast_stats()
+        fast = {"qps": round(nq / t_fast, 1), "ms_per_step": round(t_fast * 1000, 4), "w": fw, "kl": kl,
+                "seed_lists": args.fast_seed, "candidates_per_query": round(cands / nq, 1),
+                "overflowed_queries": over,
+                "stages_ms": {n: round(float(v), 4) for n, v in
+                              zip(("coarse", "seed", "pairs", "scan8", "select", "rerank"), fst)}}
+        fast.update(recall(f_ids.numpy(), f_dis.numpy()))
+        if sweep:
+            fast["operating_point_rule"] = (
+                "fastest (probes, shortlist) whose recall@100 on the held-out tuning queries (rows "
+                f"{nq}..{nq + args.tune_queries - 1} of the query stream) is within 0.5 points of the plain "
+                "form's on the same queries; qps and recall above are then measured on the evaluation queries")
+            fast["plain_recall@100_tuning"] = plain_tune
+            fast["tuning_sweep"] = sweep
+        # roofline of its dominant kernel (fast_scan_kernel): the list-major
+        # (list, <= 16 queries) items past each query's seed lists
+        fprobes, _ = ix.coarse(query, fw)
+        plen = lens[fprobes]
+        csum = np.cumsum(plen, axis=1)
+        ns = np.maximum(args.fast_seed, (csum < kl).sum(1) + 1).clip(max=fw)
+        mask = np.arange(fw)[None, :] >= ns[:, None]
+        f_vectors = int(plen[mask].sum())
+        per_list = np.bincount(fprobes[mask], minlength=cfg["nlist"])
+        visits = int((np.ceil(per_list / 16.0) * lens).sum())
+        f_ms = float(fst[3])
+        fast["roofline"] = {
+            "kernel": "fast_scan_kernel", "bound": "lds",
+            "achieved": rf(visits * m * 16 / (f_ms * 1e-3) / 1e9, 1) if f_ms > 0 else None,
+            "peak": round(lds["random16"], 1), "unit": "GB/s",
+            "frac": rf(visits * m * 16 / (f_ms * 1e-3) / 1e9 / lds["random16"]) if f_ms > 0 else None,
+            "lds_bytes_per_launch": visits * m * 16,
+            "algorithmic_code_bytes_per_launch": f_vectors * m,
+            "algorithmic_GBps_vs_hbm_peak": rf(f_vectors * m / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+            if f_ms > 0 else None,
+            "int8_mfma_busy_recorded": recorded("counters", "mfma_busy fast_scan_kernel"),
+            "note": "one 16-byte LDS read per (vector, subquantizer) of a work item serves its 16 queries' "
+                    "8-bit entries; the peak is this device's measured rate of random 16-byte LDS reads"}
         log(f"fast form: {fast}")
+
+    # ---- batched uneven exact form
     uneven = None
     if not args.no_uneven:
         if args.batched_cap:
             ix.set_option("batched_capacity", args.batched_cap)
-        # batched uneven exact search: per-query w ~ U{1..64}, k ~ {1, 10, 100}
         rng = np.random.default_rng(seed)
         wq = rng.integers(1, 65, nq).astype(np.int32)
         kq = rng.choice([1, 10, 100], nq).astype(np.int32)
@@ -208,82 +290,39 @@ def main():
         v_dis = M.DeviceArray((nq, 100), np.float32)
         b_ids = M.DeviceArray((nq, 100), np.int32)
         b_dis = M.DeviceArray((nq, 100), np.float32)
-        for _ in range(args.warmup):
-            ix.search_var(query, wq, kq, 100, out=(v_ids, v_dis))
-        M.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            ix.search_var(query, wq, kq, 100, out=(v_ids, v_dis))
-        M.synchronize()
-        t_var = (time.perf_counter() - t0) / args.steps
+        t_var = timed(lambda: ix.search_var(query, wq, kq, 100, out=(v_ids, v_dis)))
         plan = M.Plan(ix, nq, 64, 100, autotune=True)
-        for _ in range(max(1, args.warmup)):
-            plan.search(query, wq, kq, out=(b_ids, b_dis))
-        M.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            plan.search(query, wq, kq, out=(b_ids, b_dis))
-        M.synchronize()
-        t_b = (time.perf_counter() - t0) / args.steps
-        same = bool(np.array_equal(v_ids.numpy(), b_ids.numpy()) and
-                    np.array_equal(v_dis.numpy().view(np.uint32), b_dis.numpy().view(np.uint32)))
-        uneven = {"w": "U{1..64}", "k": "{1,10,100}", "plain_var_qps": round(nq / t_var, 1),
+        plan.search(query, wq, kq, out=(b_ids, b_dis))  # autotune + capture
+        t_b = timed(lambda: plan.search(query, wq, kq, out=(b_ids, b_dis)))
+        uneven = {"w": "U{1..64}", "k": "{1,10,100}", "per_query_kernels_qps": round(nq / t_var, 1),
                   "batched_graph_qps": round(nq / t_b, 1), "speedup": round(t_var / t_b, 3),
-                  "bit_identical_to_plain": same, "overflowed_queries": ix.batched_stats(),
+                  "bit_identical_to_per_query": same_bits(v_ids.numpy(), v_dis.numpy(), b_ids.numpy(),
+                                                          b_dis.numpy()),
+                  "overflowed_queries": ix.batched_stats(),
                   **{f"tuned_{k2}": v for k2, v in plan.info().items()}}
         plan.close()
         log(f"batched uneven: {uneven}")
-        # the same batched exact form on the headline batch (uniform w and k)
+        # the headline batch through an autotuned graph plan
         wu = np.full(nq, w, np.int32)
         ku = np.full(nq, k, np.int32)
         u_ids = M.DeviceArray((nq, k), np.int32)
         u_dis = M.DeviceArray((nq, k), np.float32)
         plan = M.Plan(ix, nq, w, k, autotune=True)
-        for _ in range(max(1, args.warmup)):
-            plan.search(query, wu, ku, out=(u_ids, u_dis))
-        M.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            plan.search(query, wu, ku, out=(u_ids, u_dis))
-        M.synchronize()
-        t_u = (time.perf_counter() - t0) / args.steps
-        same_u = bool(np.array_equal(out_ids.numpy(), u_ids.numpy()) and
-                      np.array_equal(out_dis.numpy().view(np.uint32), u_dis.numpy().view(np.uint32)))
-        uneven["uniform_headline"] = {"w": w, "k": k, "batched_graph_qps": round(nq / t_u, 1),
-                                      "ms_per_step": round(t_u * 1000, 4),
-                                      "algorithmic_scan_GBps_whole_search": round(scan_bytes / t_u / 1e9, 1),
-                                      "bit_identical_to_plain": same_u, "overflowed_queries": ix.batched_stats(),
-                                      **{f"tuned_{k2}": v for k2, v in plan.info().items()}}
-        info = plan.info()
+        plan.search(query, wu, ku, out=(u_ids, u_dis))
+        t_u = timed(lambda: plan.search(query, wu, ku, out=(u_ids, u_dis)))
+        uneven["uniform_headline_plan"] = {"w": w, "k": k, "qps": round(nq / t_u, 1),
+                                           "ms_per_step": round(t_u * 1000, 4),
+                                           "bit_identical_to_plain": same_bits(*out_h, u_ids.numpy(),
+                                                                               u_dis.numpy()),
+                                           **{f"tuned_{k2}": v for k2, v in plan.info().items()}}
         plan.close()
-        # its scan kernel's time (HIP events, same work-item shape as the plan)
-        ix.set_option("batched_group", info["group"])
-        ix.set_option("batched_threads", info["threads"])
-        ix.set_option("profile", 1)
-        bst = np.zeros(8)
-        for _ in range(args.steps):
-            ix.search_batched(query, wu, ku, k, out=(u_ids, u_dis))
-            bst += ix.timings()
-        ix.set_option("profile", 0)
-        bst /= args.steps
-        b_scan = float(bst[2])
-        uneven["uniform_headline"]["stages_ms"] = {n: round(float(v), 4) for n, v in
-                                                   zip(("coarse", "seed_grouping", "scan", "select"), bst)}
-        uneven["uniform_headline"]["scan_kernel_roofline"] = {
-            "kernel": "exact_scan_kernel", "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
-            "achieved": round(scan_bytes / (b_scan * 1e-3) / 1e9, 1) if b_scan > 0 else None,
-            "frac": round(scan_bytes / (b_scan * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if b_scan > 0 else None,
-            "algorithmic_bytes_per_launch": scan_bytes,
-            "note": "codes of every (query, probed list) pair, as for the plain scan; each list is read once "
-                    "per group of queries"}
-        log(f"batched uniform: {uneven['uniform_headline']}")
+        log(f"batched uniform plan: {uneven['uniform_headline_plan']}")
 
+    # ---- serving: open-loop Poisson requests against the continuous-batching server
     serving = None
     if not args.no_serve and not args.profile_only:
-        # open-loop Poisson requests (nq log-uniform 1..1024, mean ~148) against the
-        # continuous-batching server, exact form, w and k of the headline
-        qps_peak = nq * args.steps / elapsed
-        mean_nq = 147.6
+        qps_peak = nq / t_plain
+        mean_nq = 147.6  # mean of log-uniform 1..1024
         pts = []
         for frac in [float(x) for x in args.serve_fracs.split(",")]:
             st = M.serve_bench(ix, query, rate=frac * qps_peak / mean_nq, nreq=args.serve_reqs, seed=11,
@@ -291,10 +330,11 @@ def main():
             pts.append({"load": frac, "offered_qps": round(st["offered_qps"], 1), "qps": round(st["qps"], 1),
                         "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                         "mean_ms": round(st["mean_ms"], 3), "batches": int(st["batches"])})
-            log(f"serving at {frac:.2f} of peak: {pts[-1]}")
+            log(f"serving at {frac:.2f} of the batch QPS: {pts[-1]}")
         serving = {"form": "plain exact FP32 (per-request w, k)", "w": w, "k": k, "slots": 3, "max_batch": 1024,
                    "requests_per_point": args.serve_reqs, "batch_sizes": "log-uniform 1..1024",
-                   "arrivals": "Poisson (open loop)", "points": pts}
+                   "arrivals": "Poisson (open loop)", "load_reference_qps": round(qps_peak, 1), "points": pts}
+
     if args.profile_only:
         # a plain device-to-device copy of known size: the reference point
         # that scales the HBM byte counters of the search kernels
@@ -302,31 +342,10 @@ def main():
         dst = M.DeviceArray((1 << 28,), np.float32)
         M.lib().mivf_memcpy_dtod(dst.ptr, src.ptr, src.nbytes)
         M.synchronize()
-        log(f"profile-only: {args.steps} steps, {elapsed * 1000 / args.steps:.3f} ms/step, copy 1 GiB")
+        log(f"profile-only: {args.steps} steps, {t_plain * 1000:.3f} ms/step plain, copy 1 GiB")
         return
-    ms_per_step = elapsed * 1000.0 / args.steps
-    stage /= args.steps
-    qps = nq * args.steps / elapsed
-    scan_ms = float(stage[2])
-    achieved = scan_bytes / (scan_ms * 1e-3) / 1e9 if scan_ms > 0 else None
-    # HBM bytes per launch of the scan kernel from the FETCH_SIZE / WRITE_SIZE
-    # counter runs recorded for this workload (null when none is recorded)
-    def recorded_traffic(fname, kernel):
-        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{fname}_{args.workload}.md")
-        if os.path.exists(path) and not args.nbase:
-            for line in open(path):
-                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
-                    return int(line.split(":")[1])
-        return None
 
-    traffic = recorded_traffic("counters", "scan_kernel")
-    rb = (uneven or {}).get("uniform_headline", {}).get("scan_kernel_roofline")
-    if rb is not None:
-        rb["traffic"] = recorded_traffic("batched_exact_scan", "exact_scan_kernel")
-
-    if fast is not None:
-        fast["speedup_vs_plain"] = round(fast["qps"] / qps, 3)
-
+    # ---- CPU baseline: the oracle on the host cores, and parity of its results
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -352,14 +371,54 @@ def main():
                          f"sumidxtab2 into a concatenated array, then one k-selection)",
                "phase_split": {n: round(v / allp, 4) for n, v in tot.items()}}
         log(f"cpu baseline: {cpu}")
-        # the oracle's results against the GPU's plain output on the same queries
-        g_ids, g_dis = out_ids.numpy()[:nc], out_dis.numpy()[:nc]
-        parity = {"queries": nc, "w": w, "k": k,
-                  "bit_identical": bool(np.array_equal(g_ids, r_ids) and
-                                        np.array_equal(g_dis.view(np.uint32), r_dis.view(np.uint32))),
+        g_ids, g_dis = out_h[0][:nc], out_h[1][:nc]
+        parity = {"queries": nc, "w": w, "k": k, "bit_identical": same_bits(g_ids, g_dis, r_ids, r_dis),
                   "rows_differing": int(((g_ids != r_ids).any(1) |
-                                         (g_dis.view(np.uint32) != r_dis.view(np.uint32)).any(1)).sum())}
+                                         (g_dis.view(np.uint32) != r_dis.view(np.uint32)).any(1)).sum()),
+                  "per_query_kernels_bit_identical": paths_same}
         log(f"parity vs oracle: {parity}")
+
+    qps = nq / t_plain
+    scan_ms = float(st_plain[2])
+    pq_scan_ms = float(st_pq[2])
+
+    def hbm(ms):
+        return scan_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
+
+    def ldsrate(ms):
+        return table_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
+
+    traffic = recorded("counters", "traffic_bytes_per_launch exact_scan_kernel")
+    roofline = {
+        "bound": "hbm", "achieved": rf(hbm(scan_ms), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": rf(hbm(scan_ms) / HBM_PEAK_GBS) if scan_ms > 0 else None,
+        "traffic": int(traffic) if traffic else None,
+        "kernel": "exact_scan_kernel", "algorithmic_bytes_per_launch": scan_bytes,
+        "binding_unit": "lds",
+        "lds": {"achieved": rf(ldsrate(scan_ms), 1), "peak": round(lds["random16"], 1), "unit": "GB/s",
+                "frac": rf(ldsrate(scan_ms) / lds["random16"]) if scan_ms > 0 else None,
+                "conflict_free_peak": round(lds["peak16"], 1), "bytes_per_launch": table_bytes},
+        "note": "achieved = codes of every (query, probed list) pair / kernel time; the kernel reads each list "
+                "once per work item of 4 queries (HBM traffic far below the algorithmic bytes) and is bound by "
+                "its random 16-byte FP32 table reads from LDS: lds.peak is this device's measured rate of such "
+                "reads, conflict_free_peak the same reads on distinct banks"}
+    pq_traffic = recorded("counters", "traffic_bytes_per_launch scan_kernel")
+    per_query = {
+        "qps": round(nq / t_pq, 1), "ms_per_step": round(t_pq * 1000, 4),
+        "stages_ms": {n: round(float(v), 4) for n, v in zip(("coarse", "select", "scan", "merge"), st_pq)},
+        "roofline": {"bound": "hbm", "achieved": rf(hbm(pq_scan_ms), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": rf(hbm(pq_scan_ms) / HBM_PEAK_GBS) if pq_scan_ms > 0 else None,
+                     "traffic": int(pq_traffic) if pq_traffic else None, "kernel": "scan_kernel",
+                     "binding_unit": "lds",
+                     "lds": {"achieved": rf(ldsrate(pq_scan_ms), 1), "peak": round(lds["random4"], 1),
+                             "unit": "GB/s",
+                             "frac": rf(ldsrate(pq_scan_ms) / lds["random4"]) if pq_scan_ms > 0 else None,
+                             "conflict_free_peak": round(lds["peak4"], 1)}},
+        "note": "the per-query kernels (one workgroup per query and probe group, one 4-byte table read per "
+                "code byte): mivf_search below 2048 queries, or with option list_major = 0"}
+    if fast is not None:
+        fast["speedup_vs_plain"] = round(fast["qps"] / qps, 3)
+        fast["speedup_vs_per_query_kernels"] = round(fast["qps"] * t_pq / nq, 3)
 
     res = {
         "metric": "QPS at fixed recall@100 (10M x128 IVF4096,PQ16); ADC-scan HBM GB/s; p99 ms",
@@ -368,26 +427,20 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(t_plain * 1000, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded Gaussian mixture, integer-valued float32)",
         "config": {"workload": args.workload, "nbase": cfg["nbase"], "nlist": cfg["nlist"], "m": m,
-                   "nquery": nq, "w": w, "k": k, "form": "plain exact FP32",
+                   "nquery": nq, "w": w, "k": k, "form": "plain exact FP32 (mivf_search: list-major kernels)",
                    "kmeans_iterations": {"coarse": cfg["niter_coarse"], "pq": cfg["niter_pq"]}, **rec},
-        "stages_ms": {"coarse": round(float(stage[0]), 4), "select": round(float(stage[1]), 4),
-                      "scan": round(scan_ms, 4), "merge": round(float(stage[3]), 4)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": traffic,
-                     "kernel": "scan_kernel", "algorithmic_bytes_per_launch": scan_bytes},
+        "stages_ms": {n: round(float(v), 4) for n, v in zip(("coarse", "seed_grouping", "scan", "select"), st_plain)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
-        # the same exact arithmetic, list-major: the batched form's scan kernel on the headline batch
-        "roofline_batched_exact": (uneven or {}).get("uniform_headline", {}).get("scan_kernel_roofline"),
+        "per_query_kernels": per_query,
         "fast_form": fast,
         "batched_uneven": uneven,
         "serving": serving,

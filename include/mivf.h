@@ -104,7 +104,11 @@ int mivf_index_encode(mivf_index *ix, const float *d_x, int64_t n, int32_t *d_as
  * arithmetic; results sorted by (distance, id); missing results are padded
  * with id -1 and distance +inf.  Replaces the sequential ivfpq_search
  * without rerank and the per-rank search loop of parallel_search
- * (parallel/ivf_pq/ivf_search.cpp:97-137).  d_q [nq][d]; outputs [nq][k].   */
+ * (parallel/ivf_pq/ivf_search.cpp:97-137).  d_q [nq][d]; outputs [nq][k].
+ * Batches of at least "list_major_min_queries" (default 2048) 16-byte
+ * aligned queries run on the list-major kernels of mivf_search_batched
+ * (each list read once per group of queries; the same results bit for bit);
+ * option "list_major" 0 keeps every batch on the per-query kernels.        */
 int mivf_search(mivf_index *ix, const float *d_q, int nq, int w, int k,
                 int32_t *d_ids, float *d_dis, void *stream);
 /* No counterpart in the reference.  For callers that cannot allocate device
@@ -213,14 +217,18 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * mivf_search_fast, default 1), "fast_capacity", "fast_running_bound" (1 =
  * the 8-bit scan tightens each query's filter as its candidates accumulate;
  * results unchanged, default 1), "batched_group", "batched_threads",
- * "batched_capacity" (mivf_search_batched).                                 */
+ * "batched_capacity" (mivf_search_batched), "list_major" (1 = mivf_search
+ * sends large batches to the list-major kernels, default 1),
+ * "list_major_min_queries" (default 2048).                                  */
 int mivf_set_option(mivf_index *ix, const char *name, int64_t value);
 /* Per-stage milliseconds of the last profiled search:
- * [0] coarse distances, [1] probe select, [2] scan, [3] merge (mivf_search);
+ * [0] coarse distances, [1] probe select, [2] scan, [3] merge (mivf_search on
+ * the per-query kernels);
  * [0] coarse, [1] seed scan, [2] wait for the pair grouping (which runs on a
  * second stream beside the seed), [3] 8-bit scan, [4] shortlist, [5] rerank
  * (mivf_search_fast); [0] coarse, [1] probe masking + seed + grouping,
- * [2] exact scan, [3] select (mivf_search_batched).                         */
+ * [2] exact scan, [3] select (mivf_search_batched, and mivf_search on a
+ * batch it sends to the list-major kernels).                                */
 int mivf_get_timings(mivf_index *ix, float *ms, int n);
 
 /* ---- batched exact form ----------------------------------------------------
@@ -287,6 +295,15 @@ int mivf_server_latency(mivf_server *sv, int64_t ticket, double *ms);
 int mivf_serve_bench(mivf_index *ix, int slots, int max_batch, int64_t mem_budget, const float *d_pool,
                      int64_t pool_n, double rate, int nreq, uint64_t seed, int bmax,
                      const int32_t *w_choices, int nw, const int32_t *k_choices, int nk, double *stats);
+
+/* ---- calibration ------------------------------------------------------------
+ * LDS read rate of the current device in GB/s: every lane of 512-thread
+ * workgroups (two per CU) reads `bytes` (4 or 16) per step from a 64 KiB
+ * table laid out as the scan kernels' tables, at random rows (random_rows = 1:
+ * data-driven table gathers, bank conflicts included) or at rows on distinct
+ * banks (0: the conflict-free peak).  The ceiling of the scan kernels' table
+ * reads; no reference counterpart.                                         */
+int mivf_lds_read_rate(int bytes, int random_rows, double *gbps);
 
 /* ---- synthetic data and ground truth ------------------------------------ */
 /* Seeded Gaussian mixture rows; bit-identical to the CPU generator.

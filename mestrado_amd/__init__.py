@@ -87,6 +87,7 @@ _SIGS = {
     "mivf_synth_centres": [_u64, _i, _i, _vp],
     "mivf_zipf_thresholds": [_u64, _i, ctypes.c_double, _vp],
     "mivf_bruteforce_nn": [_vp, _i64, _vp, _i, _i, _vp, _vp],
+    "mivf_lds_read_rate": [_i, _i, ctypes.POINTER(ctypes.c_double)],
     "mivf_write_cent": [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _i, _i, _i, _i, _i, _vp, _vp],
     "mivf_read_cent_header": [ctypes.c_char_p, _i32p, _i32p, _i32p, _i32p, _i32p],
     "mivf_read_cent": [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _vp, _vp],
@@ -243,6 +244,14 @@ def synth_rows(seed, stream, row0, n, centres, thr=None, sigma=20.0, out=None):
                                  td.ptr if td is not None else None, sigma, out.ptr, None))
     synchronize()
     return out
+
+
+def lds_read_rate(nbytes, random_rows):
+    """Measured LDS read rate (GB/s) of 4- or 16-byte reads at random or
+    conflict-free rows (the table-gather ceiling of the scan kernels)."""
+    v = ctypes.c_double(0)
+    _check(lib().mivf_lds_read_rate(int(nbytes), 1 if random_rows else 0, ctypes.byref(v)))
+    return v.value
 
 
 def bruteforce_nn(base, q):

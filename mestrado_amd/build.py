@@ -13,7 +13,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libmivf.so")
 
 SOURCES = ["dist.hip", "coarse.hip", "select.hip", "scan.hip", "fast.hip", "exact.hip", "synth.hip", "index.hip",
-           "batched.hip", "serve.hip", "io.hip", "adc.hip", "compat.hip", "compat_seq.hip", "compat_par.hip"]
+           "batched.hip", "serve.hip", "io.hip", "adc.hip", "calib.hip", "compat.hip", "compat_seq.hip", "compat_par.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

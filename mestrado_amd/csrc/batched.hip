@@ -228,7 +228,51 @@ int autotune(mivf_index *ix, XWs &x, const float *d_q, int nq, int wmax, int kma
     return kOk;
 }
 
+XWs &index_xws(mivf_index *ix);
+
 }  // namespace
+
+bool list_major_ok(const mivf_index *ix, const float *d_q, int nq) {
+    return ix->list_major && nq >= ix->list_major_min && (ix->m == 8 || ix->m == 16) && ix->nlist <= 8192 &&
+           cfg_ok(ix->m, XCfg{ix->batched_g, ix->batched_nt}) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0;
+}
+
+// mivf_search of a large batch: the batched list-major kernels with every
+// query's w and k equal (filled on the device), overflowed queries rerun by
+// the plain kernels.
+int search_list_major(mivf_index *ix, const float *d_q, int nq, int w, int k, int32_t *d_ids, float *d_dis,
+                      hipStream_t st) {
+    MIVF_TRY(upload_lists(ix));
+    XWs &x = index_xws(ix);
+    const XCfg cfg{ix->batched_g, ix->batched_nt};
+    MIVF_TRY(x.wq.ensure(sizeof(int32_t) * nq));
+    MIVF_TRY(x.kq.ensure(sizeof(int32_t) * nq));
+    MIVF_CHECK_HIP(launch_fill_i32(x.wq.as<int>(), nq, w, st));
+    MIVF_CHECK_HIP(launch_fill_i32(x.kq.as<int>(), nq, k, st));
+    Timer tm;
+    std::fill(ix->timings, ix->timings + 8, 0.0f);
+    MIVF_TRY(tm.start(ix->profile, st, ix->ev));
+    MIVF_TRY(batched_enqueue(ix, x, cfg, d_q, nq, w, k, d_ids, d_dis, st, tm.on ? &tm : nullptr));
+    if (tm.on) {
+        MIVF_CHECK_HIP(hipEventSynchronize(tm.ev[tm.n - 1]));
+        for (int i = 0; i + 1 < tm.n && i < 8; ++i) {
+            float ms = 0;
+            MIVF_CHECK_HIP(hipEventElapsedTime(&ms, tm.ev[i], tm.ev[i + 1]));
+            ix->timings[i] = ms;
+        }
+    }
+    // the overflow count is read on the host; the per-query w and k of a
+    // rerun are only needed when some query overflowed
+    float kept[8];
+    std::copy(ix->timings, ix->timings + 8, kept);
+    const std::vector<int32_t> hw(size_t(nq), w), hk(size_t(nq), k);
+    int nov = 0;
+    MIVF_TRY(batched_fixup(ix, x, d_q, nq, hw.data(), hk.data(), k, d_ids, d_dis, st, &nov));
+    std::copy(kept, kept + 8, ix->timings);
+    ix->batched_last_overflow = nov;
+    return kOk;
+}
+
 }  // namespace mivf
 
 using namespace mivf;
@@ -255,13 +299,14 @@ struct mivf_plan {
     }
 };
 
+namespace mivf {
 namespace {
 XWs &index_xws(mivf_index *ix) {
-    static_assert(sizeof(XWs) > 0, "");
     if (!ix->xws) ix->xws = std::make_shared<XWs>();
     return *std::static_pointer_cast<XWs>(ix->xws);
 }
 }  // namespace
+}  // namespace mivf
 
 int mivf_search_batched(mivf_index *ix, const float *d_q, int nq, const int32_t *w, const int32_t *k, int kmax,
                         int32_t *d_ids, float *d_dis, void *stream) try {

@@ -88,10 +88,21 @@ __device__ void xflush(const ExactArgs &a, unsigned char *smem, int64_t beg, int
 // The G exact tables of one item (reference arithmetic, as the plain scan's
 // table phase): r = fl(q - c_l); |r_s|^2 = sum_i fl(r_i^2) in order;
 // dot = sum_i fl(r_i p_i) in order; T = fl(fl(-2 dot) + fl(|p|^2_f64 + |r_s|^2))
-// (-2 dot is exact, so fma(-2, dot, c) rounds once, identically).
+// (-2 dot is exact, so fma(-2, dot, cc) rounds once, identically).
+// Each thread owns one subquantizer s and E codes of it (c0, c0 + CPS, ...),
+// so the residual slices r_s of GC queries are loaded into registers once and
+// reused for all E codes; consecutive lanes take consecutive codes (coalesced
+// centroid loads, conflict-free 16-byte table stores).  The dot starts from
+// the first product: an all-zero dot cannot change an entry, since the norm
+// term is never -0.
 template <int M, int G, int NT>
 __device__ void xtables(const ExactArgs &a, unsigned char *smem, int l) {
     constexpr int DS = D / M;
+    constexpr int E = M * 256 / NT;                    // codes per thread
+    constexpr int CPS = 256 / E;                       // threads per subquantizer
+    constexpr int GC = (G * DS <= 64) ? G : 64 / DS;   // queries per register-resident chunk
+    static_assert(E * NT == M * 256 && CPS * M == NT, "one subquantizer per thread");
+    static_assert(G % GC == 0 && GC % 4 == 0, "query chunks of whole float4s");
     using L = XLay<M, G>;
     float *R = reinterpret_cast<float *>(smem + L::R);
     float *rn = reinterpret_cast<float *>(smem + L::rn);
@@ -115,39 +126,66 @@ __device__ void xtables(const ExactArgs &a, unsigned char *smem, int l) {
         rn[g * M + s] = acc;
     }
     __syncthreads();
-    for (int idx = tid; idx < M * 256; idx += NT) {
-        const int s = idx >> 8, c = idx & 255;
-        float vals[G];
-        if (c < a.ks) {
-            float pr[DS];
-            const float *src = a.pq + (int64_t(s) * a.ks + c) * DS;
+    const int s = tid / CPS, c0 = tid % CPS;
 #pragma unroll
-            for (int i = 0; i < DS; ++i) pr[i] = src[i];
-            const double pnv = a.pq_norm[int64_t(s) * a.ks + c];
+    for (int g0 = 0; g0 < G; g0 += GC) {
+        float r[GC][DS];
+        double rn64[GC];
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                float dot = 0.0f;
+        for (int g = 0; g < GC; ++g) {
 #pragma unroll
-                for (int i = 0; i < DS; ++i) {
-                    const float prod = R[g * D + s * DS + i] * pr[i];
-                    dot = dot + prod;
-                }
-                const float cc = float(pnv + double(rn[g * M + s]));
-                vals[g] = __builtin_fmaf(-2.0f, dot, cc);
-            }
-        } else {
-#pragma unroll
-            for (int g = 0; g < G; ++g) vals[g] = 0.0f;  // codes are < ks: never indexed
+            for (int i = 0; i < DS; ++i) r[g][i] = R[(g0 + g) * D + s * DS + i];
+            rn64[g] = double(rn[(g0 + g) * M + s]);
         }
-        float4 *dst = reinterpret_cast<float4 *>(smem + L::tab + size_t(idx) * G * 4);
+#pragma unroll 1
+        for (int j = 0; j < E; ++j) {
+            const int c = c0 + CPS * j;
+            float vals[GC];
+            if (c < a.ks) {
+                float pr[DS];
+                const float4 *src = reinterpret_cast<const float4 *>(a.pq + (int64_t(s) * a.ks + c) * DS);
 #pragma unroll
-        for (int g4 = 0; g4 < G / 4; ++g4) dst[g4] = make_float4(vals[4 * g4], vals[4 * g4 + 1], vals[4 * g4 + 2], vals[4 * g4 + 3]);
+                for (int i = 0; i < DS / 4; ++i) {
+                    const float4 v = src[i];
+                    pr[4 * i] = v.x;
+                    pr[4 * i + 1] = v.y;
+                    pr[4 * i + 2] = v.z;
+                    pr[4 * i + 3] = v.w;
+                }
+                const double pnv = a.pq_norm[int64_t(s) * a.ks + c];
+#pragma unroll
+                for (int g = 0; g < GC; ++g) {
+                    float dot = r[g][0] * pr[0];
+#pragma unroll
+                    for (int i = 1; i < DS; ++i) {
+                        const float prod = r[g][i] * pr[i];
+                        dot = dot + prod;
+                    }
+                    const float cc = float(pnv + rn64[g]);
+                    vals[g] = __builtin_fmaf(-2.0f, dot, cc);
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < GC; ++g) vals[g] = 0.0f;  // codes are < ks: never indexed
+            }
+            float4 *dst = reinterpret_cast<float4 *>(smem + L::tab + (size_t(s * 256 + c) * G + g0) * 4);
+#pragma unroll
+            for (int g4 = 0; g4 < GC / 4; ++g4)
+                dst[g4] = make_float4(vals[4 * g4], vals[4 * g4 + 1], vals[4 * g4 + 2], vals[4 * g4 + 3]);
+        }
     }
     __syncthreads();  // tables ready; the pool may now overwrite R and rn
 }
 
+// Waves per SIMD the LDS allows: two workgroups per CU when two tables fit
+// in the 160 KiB, else one.
 template <int M, int G, int NT>
-__global__ __launch_bounds__(NT, NT / 128) void exact_scan_kernel(ExactArgs a) {  // two workgroups per CU
+constexpr int xwaves() {
+    return (2 * XLay<M, G>::total <= 160 * 1024 ? 2 : 1) * NT / 256;
+}
+
+template <int M, int G, int NT>
+__global__ __launch_bounds__(NT, (xwaves<M, G, NT>())) void exact_scan_kernel(ExactArgs a) {
     using L = XLay<M, G>;
     constexpr int RW = int(sizeof(FastItem) / 4);
     static_assert(G % 4 == 0 && G <= 16, "G must be 4, 8, 12 or 16");
@@ -224,8 +262,6 @@ __global__ __launch_bounds__(NT, NT / 128) void exact_scan_kernel(ExactArgs a) {
             if (base + NT < len) xload_codes<M>(lc, min(v + NT, len - 1), cn);
             float acc[G];
 #pragma unroll
-            for (int g = 0; g < G; ++g) acc[g] = 0.0f;
-#pragma unroll
             for (int s = 0; s < M; ++s) {
                 const uint32_t code = (cw[s >> 2] >> (8 * (s & 3))) & 255u;
                 const uint32_t row = uint32_t(L::tab) + (uint32_t(s) * 256u + code) * uint32_t(G * 4);
@@ -233,7 +269,8 @@ __global__ __launch_bounds__(NT, NT / 128) void exact_scan_kernel(ExactArgs a) {
                 for (int g4 = 0; g4 < G / 4; ++g4) {
                     const f32x4 x = *reinterpret_cast<lds_f32x4 *>(uintptr_t(row + 16u * g4));
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[4 * g4 + j] = acc[4 * g4 + j] + x[j];
+                    for (int j = 0; j < 4; ++j)
+                        acc[4 * g4 + j] = s == 0 ? x[j] : acc[4 * g4 + j] + x[j];  // s = 0f; s = fl(s + T)
                 }
             }
             uint32_t pm = 0u;

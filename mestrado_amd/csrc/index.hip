@@ -735,6 +735,8 @@ int mivf_search(mivf_index *ix, const float *d_q, int nq, int w, int k, int32_t 
     MIVF_REQUIRE(k >= 1 && k <= 1024, kInvalid, "search: k must be in 1..1024");
     if (nq == 0) return kOk;
     MIVF_REQUIRE(d_q && d_ids && d_dis, kInvalid, "search: null pointer");
+    if (list_major_ok(ix, d_q, nq))
+        return search_list_major(ix, d_q, nq, w, k, d_ids, d_dis, static_cast<hipStream_t>(stream));
     return search_entry(ix, ix->ws, d_q, nq, w, nullptr, nullptr, k, k, d_ids, d_dis, static_cast<hipStream_t>(stream));
 } MIVF_CATCH_ALL
 
@@ -1126,6 +1128,11 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     } else if (n == "batched_threads") {
         MIVF_REQUIRE(value == 256 || value == 512, kInvalid, "batched_threads must be 256 or 512");
         ix->batched_nt = int(value);
+    } else if (n == "list_major") {
+        ix->list_major = value != 0;
+    } else if (n == "list_major_min_queries") {
+        MIVF_REQUIRE(value >= 1, kInvalid, "list_major_min_queries must be >= 1");
+        ix->list_major_min = value;
     } else if (n == "fast_capacity") {
         MIVF_REQUIRE(value >= 1024 && value <= (1 << 20), kInvalid, "fast_capacity must be in 1024..2^20");
         ix->fast_cap = int(value);

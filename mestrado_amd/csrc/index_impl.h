@@ -141,7 +141,11 @@ struct mivf_index {
     int fast_seed_cap = 1024; // vectors entering the seed bound (0 = all of the seed lists)
     int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
     int batched_cap = 8192;   // candidate capacity per query of the batched exact form (>= 2 kmax)
-    int batched_g = 8, batched_nt = 512;  // its work-item shape (queries per item, threads)
+    int batched_g = 4, batched_nt = 512;  // its work-item shape (queries per item, threads)
+    // mivf_search on batches of at least list_major_min queries runs the
+    // batched list-major kernels (identical results)
+    bool list_major = true;
+    int64_t list_major_min = 2048;
     int batched_last_overflow = 0;
     int ncu = 256;            // compute units of the device
     std::shared_ptr<void> xws;  // workspace of the batched exact form (batched.hip)
@@ -170,6 +174,11 @@ namespace mivf {
 int search_entry(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, const int32_t *d_wq,
                  const int32_t *d_kq, int kuni, int kstride, int32_t *d_ids, float *d_dis, hipStream_t st);
 int upload_lists(mivf_index *ix);
+// Whether mivf_search sends this batch to the list-major kernels, and that
+// path (batched.hip): uniform w and k, results identical to search_entry.
+bool list_major_ok(const mivf_index *ix, const float *d_q, int nq);
+int search_list_major(mivf_index *ix, const float *d_q, int nq, int w, int k, int32_t *d_ids, float *d_dis,
+                      hipStream_t st);
 // Coarse top-w of nq queries into probes/dis [nq][w] (ws.dmat sized nq * nlist).
 int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_t *probes, float *dis, hipStream_t st,
                  Timer *tm);

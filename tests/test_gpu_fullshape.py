@@ -101,8 +101,14 @@ def test_c2_plain_full(orc, c2):
     t0 = time.time()
     rids, rdis = orc.search(index, query, c["w"], c["k"])
     _say(f"c2 oracle search {time.time() - t0:.1f}s")
-    assert_same_results(ids, dis, rids, rdis, "c2 w=16 k=100")
+    assert_same_results(ids, dis, rids, rdis, "c2 w=16 k=100 (list-major)")
     assert (ids >= 0).all()
+    ix.set_option("list_major", 0)
+    try:
+        pids, pdis = ix.search(query, c["w"], c["k"])
+    finally:
+        ix.set_option("list_major", 1)
+    assert_same_results(pids, pdis, rids, rdis, "c2 w=16 k=100 (per-query kernels)")
 
 
 def test_c3_encode_sample(orc, c3):
@@ -120,7 +126,13 @@ def test_c3_headline_full(orc, c3):
     t0 = time.time()
     rids, rdis = orc.search(index, query, w, k)
     _say(f"c3 oracle search {time.time() - t0:.1f}s")
-    assert_same_results(ids, dis, rids, rdis, "c3 plain w=32 k=100")
+    assert_same_results(ids, dis, rids, rdis, "c3 plain w=32 k=100 (list-major)")
+    ix.set_option("list_major", 0)
+    try:
+        pids, pdis = ix.search(query, w, k)
+    finally:
+        ix.set_option("list_major", 1)
+    assert_same_results(pids, pdis, rids, rdis, "c3 plain w=32 k=100 (per-query kernels)")
     nq = query.shape[0]
     wq = np.full(nq, w, np.int32)
     kq = np.full(nq, k, np.int32)

@@ -264,6 +264,7 @@ def test_search_query_order(orc, zipf16, w, k, ppg):
     thr = orc.zipf_thresholds(seed, NCOMP, 1.1)
     query = orc.gen_rows(seed, STREAM_QUERY, 1000, 2500, cent, thr, SIGMA)  # 2500 queries: ordering on
     ix.set_option("probes_per_group", ppg)
+    ix.set_option("list_major", 0)  # the per-query kernels
     try:
         ids, dis = ix.search(query, w, k)
         ix.set_option("query_order", 0)
@@ -271,9 +272,29 @@ def test_search_query_order(orc, zipf16, w, k, ppg):
     finally:
         ix.set_option("query_order", 1)
         ix.set_option("probes_per_group", 0)
+        ix.set_option("list_major", 1)
     assert np.array_equal(ids, uids) and np.array_equal(dis.view(np.uint32), udis.view(np.uint32))
     rids, rdis = orc.search(index, query, w, k)
     assert_same_results(ids, dis, rids, rdis, f"ordered scan w={w} k={k} ppg={ppg}")
+    # the default path of a batch this large: the list-major kernels
+    lids, ldis = ix.search(query, w, k)
+    assert_same_results(lids, ldis, rids, rdis, f"list-major search w={w} k={k}")
+
+
+@pytest.mark.parametrize("nq,w,k", [(2048, 8, 1), (3000, 64, 100), (2100, 200, 128)])
+def test_search_list_major_threshold(orc, zipf16, nq, w, k):
+    """mivf_search at and above the list-major threshold (2048 queries) and
+    the per-query kernels below it return the oracle's results."""
+    ix, index, _, _ = zipf16
+    from mestrado_amd.workloads import NCOMP
+    cent = orc.gen_centres(103, NCOMP, 128)
+    thr = orc.zipf_thresholds(103, NCOMP, 1.1)
+    query = orc.gen_rows(103, STREAM_QUERY, 5000, nq, cent, thr, SIGMA)
+    rids, rdis = orc.search(index, query, w, k)
+    ids, dis = ix.search(query, w, k)
+    assert_same_results(ids, dis, rids, rdis, f"list-major nq={nq} w={w} k={k}")
+    ids, dis = ix.search(query[:2047], w, k)
+    assert_same_results(ids, dis, rids[:2047], rdis[:2047], f"per-query nq=2047 w={w} k={k}")
 
 
 def test_search_m16_rerank_and_var(orc, zipf16, mivf):
