@@ -219,7 +219,10 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * results unchanged, default 1), "batched_group", "batched_threads",
  * "batched_capacity" (mivf_search_batched), "list_major" (1 = mivf_search
  * sends large batches to the list-major kernels, default 1),
- * "list_major_min_queries" (default 2048).                                  */
+ * "list_major_min_queries" (default 2048), "coarse_mfma" (1 = coarse top-w
+ * through BF16 matrix-core distances and the exact recheck, for d = 128 and
+ * nlist <= 32768; 0 = exact FP32 distance tiles for every shape; the same
+ * probes either way).                                                       */
 int mivf_set_option(mivf_index *ix, const char *name, int64_t value);
 /* Per-stage milliseconds of the last profiled search:
  * [0] coarse distances, [1] probe select, [2] scan, [3] merge (mivf_search on

@@ -423,7 +423,13 @@ __global__ __launch_bounds__(NTC) void coarse_select_wave_kernel(const float *__
 
 }  // namespace
 
-bool coarse_mfma_ok(int d, int nlist, int w) { return d == CD && nlist >= 1 && nlist <= 8192 && w >= 1 && w <= RCAP; }
+// The workgroup select holds a query's nlist approximate distances in LDS
+// (plus RCAP keys): up to 32768 lists, 136 KiB of the 160 KiB.
+constexpr int kMaxMfmaLists = 32768;
+
+bool coarse_mfma_ok(int d, int nlist, int w) {
+    return d == CD && nlist >= 1 && nlist <= kMaxMfmaLists && w >= 1 && w <= RCAP;
+}
 
 size_t coarse_select_lds(int nlist) { return ((size_t(nlist) * 4 + 15) & ~size_t(15)) + RCAP * 8; }
 
@@ -448,6 +454,15 @@ hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxc
                                 const double *qn, int nq, int w, const float *approx, int64_t lda, int32_t *probes,
                                 float *pdis, int32_t *redo, int *redo_cnt, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
+    if (nlist > kMaxMfmaLists) return hipErrorInvalidValue;
+    static size_t lds_set = 64 * 1024;  // dynamic LDS allowed so far (64 KiB needs no attribute)
+    const size_t lds = coarse_select_lds(nlist);
+    if (lds > lds_set) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&coarse_select_kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+        lds_set = lds;
+    }
     if (nlist >= 64 && nlist <= 64 * NPL && w <= 64 && redo && redo_cnt) {
         hipError_t e = launch_fill_i32(redo_cnt, 1, 0, s);  // a kernel, not a memset: graph-capturable
         if (e != hipSuccess) return e;

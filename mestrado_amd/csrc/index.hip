@@ -380,7 +380,7 @@ int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_
     const int d = ix->d, nlist = ix->nlist;
     MIVF_TRY(ws.qn.ensure(sizeof(double) * nq));
     MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ws.qn.as<double>(), st));
-    if (coarse_mfma_ok(d, nlist, w) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0) {
+    if (ix->coarse_mfma && coarse_mfma_ok(d, nlist, w) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0) {
         MIVF_CHECK_HIP(launch_coarse_approx(ix->coarse_hi.p, ix->coarse_lo.p, ix->coarse_n32.as<float>(), nlist, d_q,
                                             ws.qn.as<double>(), nq, ws.dmat.as<float>(), nlist, st));
         if (tm) MIVF_TRY(tm->mark());
@@ -1128,6 +1128,8 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     } else if (n == "batched_threads") {
         MIVF_REQUIRE(value == 256 || value == 512, kInvalid, "batched_threads must be 256 or 512");
         ix->batched_nt = int(value);
+    } else if (n == "coarse_mfma") {
+        ix->coarse_mfma = value != 0;
     } else if (n == "list_major") {
         ix->list_major = value != 0;
     } else if (n == "list_major_min_queries") {

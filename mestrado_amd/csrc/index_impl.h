@@ -145,6 +145,9 @@ struct mivf_index {
     // mivf_search on batches of at least list_major_min queries runs the
     // batched list-major kernels (identical results)
     bool list_major = true;
+    // coarse top-w through the BF16 matrix-core distances + exact recheck
+    // (0: the exact FP32 distance tiles + row selection for every shape)
+    bool coarse_mfma = true;
     int64_t list_major_min = 2048;
     int batched_last_overflow = 0;
     int ncu = 256;            // compute units of the device

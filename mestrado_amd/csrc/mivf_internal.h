@@ -117,7 +117,7 @@ hipError_t launch_row_topk(const float *D, int64_t rows, int n, int64_t ld, int 
 
 // Coarse top-w with the exact reference arithmetic: BF16-MFMA approximate
 // distances, then an exact recheck of every centroid within the proven error
-// margin of the w-th (d == 128, nlist <= 8192, w <= 1024, 16-byte aligned
+// margin of the w-th (d == 128, nlist <= 32768, w <= 1024, 16-byte aligned
 // queries; otherwise use launch_dist_store + launch_row_topk).
 bool coarse_mfma_ok(int d, int nlist, int w);
 size_t coarse_select_lds(int nlist);

@@ -28,4 +28,12 @@ WORKLOADS = {
     "ivfpq_10m_l4096_m16": dict(seed=103, ntrain=1_000_000, nbase=10_000_000, nquery=10_000,
                                 nlist=4096, m=16, w=32, k=100, zipf=1.1, niter_coarse=50,
                                 niter_pq=100),
+    # serving loop: 100M base streamed in 1M-row chunks (ids = global row
+    # numbers) and kept only as PQ16 codes; 16384 lists trained on 1M
+    # vectors; open-loop Poisson requests, batch sizes log-uniform 1..1024,
+    # w from {8, 32, 128}, k from {10, 100}
+    "ivfpq_100m_l16384_m16": dict(seed=105, ntrain=1_000_000, nbase=100_000_000, nquery=10_000,
+                                  nlist=16384, m=16, w=32, k=100, zipf=None, niter_coarse=50,
+                                  niter_pq=100, stream_chunk=1_000_000, serve_w=(8, 32, 128),
+                                  serve_k=(10, 100), serve_pool=100_000),
 }

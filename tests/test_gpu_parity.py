@@ -319,3 +319,39 @@ def test_search_other_m(mivf, orc, c1, m):
         ids, dis = ix.search(c1["query"], w, k)
         rids, rdis = orc.search(index, c1["query"], w, k)
         assert_same_results(ids, dis, rids, rdis, f"m={m} w={w} k={k}")
+
+
+@pytest.fixture(scope="module")
+def wide(mivf, orc):
+    """An index with more lists than the 8192 of the list-major kernels:
+    9000 lists (not a power of two), m = 16, 120k base vectors."""
+    seed = 107
+    cent = orc.gen_centres(seed, NCOMP, D)
+    train = orc.gen_rows(seed, STREAM_TRAIN, 0, 40_000, cent, None, SIGMA)
+    base = orc.gen_rows(seed, STREAM_BASE, 0, 120_000, cent, None, SIGMA)
+    query = orc.gen_rows(seed, STREAM_QUERY, 0, 150, cent, None, SIGMA)
+    coarse, pq = mivf.train(train, 9000, 16, 256, 3, 4, seed)
+    ix = mivf.Index(coarse, pq)
+    ix.add(base)
+    return ix, (coarse, pq) + tuple(ix.get_lists()), query
+
+
+@pytest.mark.parametrize("mfma", [1, 0])
+def test_many_lists_coarse_and_search(orc, wide, mfma):
+    """nlist = 9000: coarse top-w on the matrix-core path (workgroup select
+    with the distances in > 64 KiB of LDS) and on the forced exact FP32 tiles
+    give the oracle's probes; searches match the oracle bit for bit."""
+    ix, index, query = wide
+    coarse = index[0]
+    ix.set_option("coarse_mfma", mfma)
+    try:
+        for w in (1, 8, 64, 300):
+            idx, dis = ix.coarse(query, w)
+            ridx, rdis = orc.knn(coarse, query, w)
+            assert_same_results(idx, dis, ridx, rdis, f"coarse nlist=9000 w={w} mfma={mfma}")
+        for w, k in ((8, 100), (64, 10)):
+            ids, dis = ix.search(query, w, k)
+            rids, rdis = orc.search(index, query, w, k)
+            assert_same_results(ids, dis, rids, rdis, f"search nlist=9000 w={w} k={k} mfma={mfma}")
+    finally:
+        ix.set_option("coarse_mfma", 1)
