@@ -197,6 +197,12 @@ int mivf_fast_stats(mivf_index *ix, int64_t *candidates, int64_t *overflowed_que
 int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t *d_t8,
                      float *d_delta);
 
+/* Test hook of the FP32-built table types (option "fast_tables" 1 = FP16
+ * entries, 2 = 8-bit entries with a per-subquantizer offset and scale): the
+ * approximate ADC distance of every vector of list `list` for queries
+ * d_q[0..n-1] (n <= 16), d_out [len][16], computed by the scan's own loop.  */
+int mivf_fast_var_adc(mivf_index *ix, const float *d_q, int n, int list, int tables, float *d_out);
+
 /* The seed thresholds of mivf_search_fast: d_tau [nq] >= the kl-th smallest
  * ADC distance over each query's seed lists (+inf when they hold fewer than
  * kl vectors).  For tests.                                                  */
@@ -219,7 +225,11 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * results unchanged, default 1), "batched_group", "batched_threads",
  * "batched_capacity" (mivf_search_batched), "list_major" (1 = mivf_search
  * sends large batches to the list-major kernels, default 1),
- * "list_major_min_queries" (default 2048), "coarse_mfma" (1 = coarse top-w
+ * "fast_tables" (table type of mivf_search_fast: 0 = 8-bit entries built on
+ * the FP16 matrix cores with one scale per (query, list) and int8
+ * matrix-core sums, the default; 1 = FP16 entries; 2 = 8-bit entries with a
+ * per-subquantizer offset and scale; 1 and 2 are built from FP32 tables and
+ * summed in FP32), "list_major_min_queries" (default 2048), "coarse_mfma" (1 = coarse top-w
  * through BF16 matrix-core distances and the exact recheck, for d = 128 and
  * nlist <= 32768; 0 = exact FP32 distance tiles for every shape; the same
  * probes either way).                                                       */

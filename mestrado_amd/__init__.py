@@ -62,6 +62,7 @@ _SIGS = {
     "mivf_fast_stats": [_vp, _i64p, _i64p],
     "mivf_index_attach_base": [_vp, _vp, _i64],
     "mivf_fast_tables": [_vp, _vp, _i, _i, _vp, _vp],
+    "mivf_fast_var_adc": [_vp, _vp, _i, _i, _i, _vp],
     "mivf_fast_seed_tau": [_vp, _vp, _i, _i, _i, _vp],
     "mivf_fast_select": [_vp, _vp, _i, _i, _i, _vp],
     "mivf_search_batched": [_vp, _vp, _i, _vp, _vp, _i, _vp, _vp, _vp],
@@ -468,6 +469,15 @@ class Index:
         dl = DeviceArray((16,), np.float32)
         _check(lib().mivf_fast_tables(self.h, qd.ptr, qd.shape[0], int(list_no), t8.ptr, dl.ptr))
         return t8.numpy(), dl.numpy()
+
+    def fast_var_adc(self, q, list_no, list_len, tables):
+        """Approximate ADC [len][16] of list `list_no` for queries q[:n]
+        (n <= 16) with the FP32-built table type `tables` (1 FP16, 2 8-bit
+        per-subquantizer offset/scale)."""
+        qd, _ = _dev(q, np.float32)
+        out = DeviceArray((max(list_len, 1), 16), np.float32)
+        _check(lib().mivf_fast_var_adc(self.h, qd.ptr, qd.shape[0], int(list_no), int(tables), out.ptr))
+        return out.numpy()[:list_len]
 
     def fast_seed_tau(self, q, w, kl):
         """Seed thresholds [nq] of the reduced-precision form."""

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libmivf.so")
 
-SOURCES = ["dist.hip", "coarse.hip", "select.hip", "scan.hip", "fast.hip", "exact.hip", "synth.hip", "index.hip",
+SOURCES = ["dist.hip", "coarse.hip", "select.hip", "scan.hip", "fast.hip", "fastvar.hip", "exact.hip", "synth.hip", "index.hip",
            "batched.hip", "serve.hip", "io.hip", "adc.hip", "calib.hip", "compat.hip", "compat_seq.hip", "compat_par.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -976,10 +976,18 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         a.runthr = ix->fast_running ? ix->f_rthr.as<float>() : nullptr;
         a.rhist = ix->fast_running ? ix->f_rhist.as<int>() : nullptr;
         a.kl = kl;
+        a.pq = ix->pq.as<float>();
+        a.pq_norm = ix->pq_n64.as<double>();
         int dev = 0, ncu = 256;
         MIVF_CHECK_HIP(hipGetDevice(&dev));
         MIVF_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        MIVF_CHECK_HIP(launch_fast_scan(a, ix->m, 2 * ncu, st));
+        if (ix->fast_tables == 0) {
+            MIVF_CHECK_HIP(launch_fast_scan(a, ix->m, 2 * ncu, st));
+        } else {
+            a.runthr = nullptr;  // the FP32-built table types filter by the seed bound alone
+            a.rhist = nullptr;
+            MIVF_CHECK_HIP(launch_fast_var_scan(a, ix->m, ix->fast_tables, 2 * ncu, st));
+        }
     }
     MIVF_TRY(tm.mark());
     // 5. shortlist and exact rerank
@@ -1043,6 +1051,29 @@ int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t 
     a.pmax = ix->pmax.as<float>();
     a.ks = ix->ks;
     MIVF_CHECK_HIP(launch_fast_tables(a, ix->m, list, n, d_t8, d_delta, nullptr));
+    MIVF_CHECK_HIP(hipDeviceSynchronize());
+    return kOk;
+} MIVF_CATCH_ALL
+
+int mivf_fast_var_adc(mivf_index *ix, const float *d_q, int n, int list, int tables, float *d_out) try {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(d_q && d_out && n >= 1 && n <= 16 && list >= 0 && list < ix->nlist, kInvalid,
+                 "fast_var_adc: bad arguments");
+    MIVF_REQUIRE(tables == 1 || tables == 2, kInvalid, "fast_var_adc: tables must be 1 or 2");
+    MIVF_REQUIRE(ix->m == 8 || ix->m == 16, kUnsupported, "fast_var_adc: compiled for m in {8, 16}");
+    MIVF_TRY(upload_lists(ix));
+    FastArgs a{};
+    a.queries = d_q;
+    a.coarse = ix->coarse.as<float>();
+    a.pmax = ix->pmax.as<float>();
+    a.codes = ix->codes.as<uint8_t>();
+    a.ks = ix->ks;
+    a.err = ix->errflag.as<int>();
+    a.pq = ix->pq.as<float>();
+    a.pq_norm = ix->pq_n64.as<double>();
+    const int64_t beg = ix->h_off[list];
+    const int len = int(ix->h_off[list + 1] - beg);
+    MIVF_CHECK_HIP(launch_fast_var_adc(a, ix->m, tables, list, n, beg, len, d_out, nullptr));
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
 } MIVF_CATCH_ALL
@@ -1128,6 +1159,9 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     } else if (n == "batched_threads") {
         MIVF_REQUIRE(value == 256 || value == 512, kInvalid, "batched_threads must be 256 or 512");
         ix->batched_nt = int(value);
+    } else if (n == "fast_tables") {
+        MIVF_REQUIRE(value >= 0 && value <= 2, kInvalid, "fast_tables must be 0, 1 or 2");
+        ix->fast_tables = int(value);
     } else if (n == "coarse_mfma") {
         ix->coarse_mfma = value != 0;
     } else if (n == "list_major") {

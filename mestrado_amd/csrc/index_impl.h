@@ -148,6 +148,11 @@ struct mivf_index {
     // coarse top-w through the BF16 matrix-core distances + exact recheck
     // (0: the exact FP32 distance tiles + row selection for every shape)
     bool coarse_mfma = true;
+    // table type of mivf_search_fast: 0 = 8-bit entries built on the FP16
+    // matrix cores, one scale per (query, list), int8 matrix-core sums;
+    // 1 = FP16 entries, 2 = 8-bit entries with a per-subquantizer offset and
+    // scale (both built from FP32 tables, FP32 sums)
+    int fast_tables = 0;
     int64_t list_major_min = 2048;
     int batched_last_overflow = 0;
     int ncu = 256;            // compute units of the device

@@ -206,6 +206,8 @@ struct FastArgs {
     float *runthr;              // [nq] running bound on the kl-th candidate distance (+inf), or nullptr
     int *rhist;                 // [nq][fast_hist_bins()] candidate-distance histograms (zeroed), with runthr
     int kl;
+    const float *pq;            // [m][ks][ds] (the FP32-built table types of fastvar.hip)
+    const double *pq_norm;      // [m][ks] f64 norms
 };
 int fast_group_size();
 int fast_hist_bins();
@@ -266,6 +268,14 @@ struct SeedArgs {
 // they hold fewer than kl vectors).
 hipError_t launch_seed_tau(const SeedArgs &a, int m, hipStream_t s);
 hipError_t launch_fast_scan(const FastArgs &a, int m, int grid, hipStream_t s);
+// The scan with tables built in FP32 and stored as FP16 (tables = 1) or as
+// 8-bit entries with a per-subquantizer offset and scale (tables = 2)
+// (fastvar.hip); same items, candidates and keys as launch_fast_scan.
+hipError_t launch_fast_var_scan(const FastArgs &a, int m, int tables, int grid, hipStream_t s);
+// Test hook: approximate ADC distances out [len][16] of queries 0..n-1
+// (n <= 16) over list l (positions beg .. beg + len) with table type tables.
+hipError_t launch_fast_var_adc(const FastArgs &a, int m, int tables, int l, int n, int64_t beg, int len, float *out,
+                               hipStream_t s);
 // The 8-bit tables and scales of one work item (queries 0..n-1 of a.queries,
 // n <= 16, list l): t8 [m][256][16] uint8, delta [16].  Test hook.
 hipError_t launch_fast_tables(const FastArgs &a, int m, int l, int n, uint8_t *t8, float *delta, hipStream_t s);

@@ -306,3 +306,115 @@ def test_fast_running_bound_changes_nothing(mivf, zipf16):
         ix.set_option("fast_capacity", 16384)
         ix.set_option("profile", 0)
     assert max(cands) <= ref_c
+
+
+def var_adc_bounds(orc, pq, res, codes, tables):
+    """Exact ADC of codes for residual res (oracle tables + sumidxtab2) and
+    the error bound of table type `tables` for each vector."""
+    tab = orc.tables(res, pq)  # [m][ks]
+    m, ks = tab.shape
+    exact = orc.adc(tab, codes)
+    ent = tab[np.arange(m)[None, :], codes.astype(np.int64)]  # [n][m] entries of each vector
+    if tables == 1:
+        # each FP16 entry within 2^-11 of itself; FP32 sums
+        bound = 2.0 ** -11 * np.abs(ent).sum(1) * 1.01 + 1e-6 * np.abs(exact)
+    else:
+        rng = tab.max(1) - tab.min(1)  # per subquantizer over the codes < ks
+        bound = np.full(len(codes), 0.5002 * float((rng / 255.0).sum())) + 1e-6 * (np.abs(exact) + tab.max(1).sum())
+    return exact, bound
+
+
+@pytest.mark.parametrize("tables", [1, 2])
+@pytest.mark.parametrize("shape", ["m8", "m16"])
+def test_var_tables_adc_within_bound(mivf, orc, c1, zipf16, tables, shape):
+    """The FP16 and the per-subquantizer offset/scale 8-bit tables: every
+    approximate ADC the scan loop computes is within its table type's proven
+    error bound of the oracle's exact ADC (the bound the candidate filter
+    uses), for 16 queries against several lists."""
+    if shape == "m8":
+        g = c1["gold_q"]
+        coarse, pq = g["coarse"], g["pq"]
+        ix = mivf.Index(coarse, pq)
+        ix.add(c1["base"])
+        off, ids, codes = ix.get_lists()
+        query = c1["query"][:16]
+    else:
+        ix, index, query, _ = zipf16
+        coarse, pq, off, ids, codes = index
+        query = query[:16]
+    lens = np.diff(off)
+    lists = [int(l) for l in np.argsort(-lens)[:3]] + [int(np.nonzero(lens > 0)[0][0])]
+    worst = 0.0
+    for l in lists:
+        n = int(lens[l])
+        approx = ix.fast_var_adc(query, l, n, tables)
+        cl = codes[off[l]:off[l + 1]]
+        for gq in range(query.shape[0]):
+            res = (query[gq] - coarse[l]).astype(np.float32)
+            exact, bound = var_adc_bounds(orc, pq, res, cl, tables)
+            err = np.abs(approx[:, gq].astype(np.float64) - exact.astype(np.float64))
+            assert (err <= bound).all(), f"list {l} query {gq}: max err {err.max()} over bound {bound[err.argmax()]}"
+            worst = max(worst, float((err / np.maximum(bound, 1e-30)).max()))
+    assert worst > 0.0  # the entries are rounded: some error is there
+
+
+@pytest.mark.parametrize("tables", [1, 2])
+@pytest.mark.parametrize("w,kl,k", [(32, 256, 100), (8, 1024, 10), (17, 192, 100)])
+def test_fast_var_tables_exact_rows(mivf, orc, zipf16, zipf16_gt, tables, w, kl, k):
+    """End to end with the FP16 and the 8-bit per-subquantizer tables: exact
+    rerank distances, (distance, id) order, ids from the probed lists, and
+    recall within one query of the exact RERANK path."""
+    ix, index, query, base = zipf16
+    ix.set_option("fast_tables", tables)
+    ix.set_option("fast_capacity", 1 << 16)
+    ix.set_option("profile", 1)
+    try:
+        ids, dis = ix.search_fast(query, w, kl, k, base)
+        over = ix.fast_stats()[1]
+        ids2, dis2 = ix.search_fast(query, w, kl, k, base)
+    finally:
+        ix.set_option("fast_tables", 0)
+        ix.set_option("fast_capacity", 16384)
+        ix.set_option("profile", 0)
+    assert over == 0
+    assert np.array_equal(ids, ids2) and np.array_equal(dis.view(np.uint32), dis2.view(np.uint32)), "not deterministic"
+    coarse_ids, _ = ix.coarse(query, w)
+    lo = list_of_ids(index[2], index[3], base.shape[0])
+    check_exact_rows(orc, ids, dis, base, query, coarse_ids, lo)
+    rids, _ = ix.search_rerank(query, w, kl, k, base)
+    R = min(k, 100)
+    rf = orc.recall_at(ids, zipf16_gt[:, None], R)
+    rr = orc.recall_at(rids, zipf16_gt[:, None], R)
+    assert rf >= rr - 100.0 / query.shape[0] - 1e-9, f"tables={tables}: recall@{R} {rf} vs rerank {rr}"
+    agree = np.mean([len(set(a[a >= 0]) & set(b[b >= 0])) / max(1, (b >= 0).sum()) for a, b in zip(ids, rids)])
+    assert kl < 2 * k or agree >= 0.97, f"tables={tables}: only {agree:.3f} of the exact RERANK ids returned"
+
+
+@pytest.mark.parametrize("tables", [1, 2])
+def test_fast_var_tables_m8_empty_lists(mivf, orc, c1, tables):
+    """m = 8 with empty lists and k larger than the available results."""
+    q = c1["gold_q"]
+    nlist = q["coarse"].shape[0]
+    base = c1["base"]
+    ix = mivf.Index(q["coarse"], q["pq"])
+    ix.add(base)
+    off, ids_all, codes = ix.get_lists()
+    keep = np.ones(len(ids_all), bool)
+    for l in range(0, nlist, 3):
+        keep[off[l]:off[l + 1]] = False
+    sizes = np.diff(off)
+    sizes[::3] = 0
+    off2 = np.zeros(nlist + 1, np.int64)
+    off2[1:] = np.cumsum(sizes)
+    ix2 = mivf.Index(q["coarse"], q["pq"])
+    ix2.set_lists(off2, ids_all[keep], codes[keep])
+    ix2.set_option("fast_tables", tables)
+    lo = list_of_ids(off2, ids_all[keep], base.shape[0])
+    query = c1["query"]
+    for w, kl, k in ((8, 256, 100), (1, 1024, 1000), (32, 1024, 100)):
+        ids, dis = ix2.search_fast(query, w, kl, k, base)
+        coarse_ids, _ = ix2.coarse(query, w)
+        check_exact_rows(orc, ids, dis, base, query, coarse_ids, lo)
+        rids, _ = ix2.search_rerank(query, w, kl, k, base)
+        agree = np.mean([len(set(a[a >= 0]) & set(b[b >= 0])) / max(1, (b >= 0).sum()) for a, b in zip(ids, rids)])
+        assert kl < 2 * k or agree >= 0.97, f"tables={tables} w={w} kl={kl}: agreement {agree:.3f}"
