@@ -60,6 +60,9 @@ def main():
                                                                    "(0 = library default)")
     ap.add_argument("--serve-fracs", default="0.25,0.5,0.75", help="offered load as fractions of the batch QPS")
     ap.add_argument("--serve-reqs", type=int, default=12000)
+    ap.add_argument("--precision-study", action="store_true",
+                    help="the reduced-precision table types on this workload's index (w=32, shortlist 1024, "
+                         "k=100): recall and distance error against FP64 per table type")
     ap.add_argument("--profile-only", action="store_true",
                     help="run the timed steps of the search forms and a 1 GiB device copy "
                          "(counter calibration), then exit")
@@ -76,6 +79,8 @@ def main():
     cfg = dict(WORKLOADS[args.workload])
     if cfg.get("stream_chunk"):
         return streamed_serving(args, cfg)
+    if args.precision_study:
+        return precision_study(args, cfg)
     if args.nbase:
         cfg["nbase"] = args.nbase
         cfg["ntrain"] = min(cfg["ntrain"], max(cfg["nbase"] // 10, cfg["nlist"] * 8))
@@ -446,6 +451,137 @@ def main():
         "fast_form": fast,
         "batched_uneven": uneven,
         "serving": serving,
+    }
+    print(json.dumps(res), flush=True)
+
+
+def precision_study(args, cfg):
+    """The reduced-precision form on a bit-exact workload's index: BF16
+    matrix-core coarse assignment with the FP32 recheck, then each table type
+    (8-bit built on the FP16 matrix cores with one scale per (query, list);
+    FP16 entries; 8-bit entries with a per-subquantizer offset and scale), a
+    shortlist of 1024 per query and the exact FP32 rerank against the raw
+    float32 base, at w = 32 and k = 100.  Recall@1/10/100 and the distance
+    error are measured against the FP64 brute-force nearest neighbour of 1,000
+    queries; QPS over the workload's whole query batch."""
+    import mestrado_amd as M
+    from mestrado_amd.workloads import D, NCOMP, SIGMA, STREAM_TRAIN, STREAM_BASE, STREAM_QUERY
+    from oracle import pyoracle as O
+
+    seed, nq = cfg["seed"], cfg["nquery"]
+    w, kl, k = 32, 1024, 100
+    t0 = time.time()
+    cent = M.synth_centres(seed, NCOMP, D)
+    thr = M.zipf_thresholds(seed, NCOMP, cfg["zipf"]) if cfg["zipf"] else None
+    train = M.synth_rows(seed, STREAM_TRAIN, 0, cfg["ntrain"], cent, thr, SIGMA)
+    coarse, pq = M.train(train, cfg["nlist"], cfg["m"], 256, cfg["niter_coarse"], cfg["niter_pq"], seed)
+    train.free()
+    base = M.synth_rows(seed, STREAM_BASE, 0, cfg["nbase"], cent, thr, SIGMA)
+    query = M.synth_rows(seed, STREAM_QUERY, 0, nq, cent, thr, SIGMA)
+    ix = M.Index(coarse, pq)
+    ix.add(base)
+    off, ids_l, codes_l = ix.get_lists()
+    log(f"index built in {time.time() - t0:.1f}s")
+    ng = min(args.gt_queries, 1000)
+    qh = query.numpy()
+    qg = M.DeviceArray.from_numpy(qh[:ng])
+    gt, gt_d = M.bruteforce_nn(base, qg)
+    base_h = base.numpy()
+    q64 = qh[:ng].astype(np.float64)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        M.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        M.synchronize()
+        return (time.perf_counter() - t) / args.steps
+
+    def quality(ids_h, dis_h):
+        r = {f"recall@{R}": round(100.0 * float((ids_h[:ng, :R] == gt[:, None]).any(1).mean()), 3)
+             for R in (1, 10, 100)}
+        ids_s, dis_s = ids_h[:ng], dis_h[:ng].astype(np.float64)
+        ok = ids_s >= 0
+        rows = base_h[np.where(ok, ids_s, 0)].astype(np.float64)
+        d64 = ((rows - q64[:, None, :]) ** 2).sum(axis=2)
+        rel = np.abs(dis_s - d64)[ok] / np.maximum(d64[ok], 1e-30)
+        gap = (d64[:, 0][ok[:, 0]] - gt_d[ok[:, 0]]) / np.maximum(gt_d[ok[:, 0]], 1e-30)
+        r["returned_dist_rel_err_vs_fp64"] = {"mean": float(f"{rel.mean():.3g}"), "max": float(f"{rel.max():.3g}")}
+        r["top1_dist_gap_vs_fp64_nn"] = {"mean": float(f"{gap.mean():.3g}"), "max": float(f"{gap.max():.3g}")}
+        return r
+
+    oi = M.DeviceArray((nq, k), np.int32)
+    od = M.DeviceArray((nq, k), np.float32)
+    forms = {}
+    # the plain exact form (bit-exact to the reference arithmetic) at w = 32
+    t = timed(lambda: ix.search(query, w, k, out=(oi, od)))
+    forms["plain_exact"] = {"qps": round(nq / t, 1), **quality(oi.numpy(), od.numpy())}
+    # the reference's RERANK path: exact ADC shortlist of 1024, exact FP32 rerank
+    rq = M.DeviceArray.from_numpy(qh[:ng])
+    t = timed(lambda: ix.search_rerank(rq, w, kl, k, base))
+    r_ids, r_dis = ix.search_rerank(rq, w, kl, k, base)
+    forms["exact_adc_rerank"] = {"qps_1000_queries": round(ng / t, 1), **quality(r_ids, r_dis)}
+    log(f"plain {forms['plain_exact']}, exact-ADC rerank {forms['exact_adc_rerank']}")
+    ix.attach_base(base)
+    ix.set_option("fast_capacity", 1 << 16)
+    names = {0: "int8_tables_per_list_scale_mfma", 1: "fp16_tables", 2: "int8_tables_per_subquantizer_offset_scale"}
+    # the approximate ADC of each table type against the exact ADC, over 16
+    # queries x the 4 longest lists
+    lens = np.diff(off)
+    sample_lists = [int(l) for l in np.argsort(-lens)[:4]]
+    q16 = qh[:16]
+    for tt, name in names.items():
+        ix.set_option("fast_tables", tt)
+        t = timed(lambda: ix.search_fast(query, w, kl, k, out=(oi, od)))
+        ix.set_option("profile", 1)
+        ix.search_fast(query, w, kl, k, out=(oi, od))
+        cands, over = ix.fast_stats()
+        st = ix.timings()
+        ix.set_option("profile", 0)
+        f = {"qps": round(nq / t, 1), "candidates_per_query": round(cands / nq, 1), "overflowed_queries": over,
+             "stages_ms": {n: round(float(v), 4) for n, v in
+                           zip(("coarse", "seed", "pairs", "scan", "select", "rerank"), st)}}
+        f.update(quality(oi.numpy(), od.numpy()))
+        rels = []
+        for l in sample_lists:
+            cl = codes_l[off[l]:off[l + 1]]
+            if tt == 0:
+                t8, dl = ix.fast_tables(q16, l)
+            else:
+                approx = ix.fast_var_adc(q16, l, len(cl), tt)
+            for g in range(16):
+                res = (q16[g] - coarse[l]).astype(np.float32)
+                exact = O.adc(O.tables(res, pq), cl).astype(np.float64)
+                if tt == 0:
+                    ap_ = dl[g] * t8[np.arange(cfg["m"])[None, :], cl.astype(np.int64), g].astype(np.float64).sum(1)
+                else:
+                    ap_ = approx[:, g].astype(np.float64)
+                rels.append(np.abs(ap_ - exact) / np.maximum(exact, 1e-30))
+        rels = np.concatenate(rels)
+        f["approx_adc_rel_err"] = {"mean": float(f"{rels.mean():.3g}"), "p99": float(f"{np.quantile(rels, 0.99):.3g}"),
+                                   "max": float(f"{rels.max():.3g}"), "pairs": int(rels.size)}
+        forms[name] = f
+        log(f"{name}: {f}")
+    ix.set_option("fast_tables", 0)
+    # coarse assignment: BF16 matrix-core distances + FP32 recheck vs the
+    # exact FP32 distance tiles
+    pm, pmd = ix.coarse(qg, w)
+    ix.set_option("coarse_mfma", 0)
+    pe, ped = ix.coarse(qg, w)
+    ix.set_option("coarse_mfma", 1)
+    coarse_same = bool(np.array_equal(pm, pe) and np.array_equal(pmd.view(np.uint32), ped.view(np.uint32)))
+    res = {
+        "metric": "QPS at fixed recall@100 (10M x128 IVF4096,PQ16); ADC-scan HBM GB/s; p99 ms",
+        "value": forms["plain_exact"]["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(nq / forms["plain_exact"]["qps"] * 1000, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Gaussian mixture, integer-valued float32)",
+        "config": {"workload": args.workload, "study": "reduced-precision table types", "nbase": cfg["nbase"],
+                   "nlist": cfg["nlist"], "m": cfg["m"], "nquery": nq, "w": w, "kl": kl, "k": k,
+                   "quality_queries": ng, "coarse_bf16_recheck_equals_exact_fp32": coarse_same},
+        "forms": forms,
     }
     print(json.dumps(res), flush=True)
 
