@@ -20,18 +20,26 @@
 // A vector is kept for query g when its approximate ADC is <= tau_g plus that
 // bound (and a relative 2^-20 for the FP32 sums), so every vector whose exact
 // ADC is <= tau_g (the seed bound of the query's kl-th distance) is kept, as
-// in the 8-bit matrix-core scan; the same shortlist select and exact rerank
-// follow.  Each thread scans one vector per step for all 16 queries: one or
+// in the 8-bit matrix-core scan, and when it is <= the query's running bound
+// R_q (at least kl candidates already have an approximate distance <= R_q, so
+// the shortlist of the kl smallest is the same with or without it); the same
+// shortlist select and exact rerank follow.  Each thread scans one vector per step for all 16 queries: one or
 // two 16-byte LDS reads per subquantizer, FP32 sums on the VALU.
 #include "select_dev.h"
 
 namespace mivf {
 namespace {
 
+// row stride of the running-bound histograms (FastArgs::rhist), shared with
+// the 8-bit matrix-core scan
+constexpr int kHistStride = 64;
+__device__ inline int fast_hist_bins_dev() { return kHistStride; }
+
 constexpr int D = 128;
 constexpr int VG = 16;      // queries per work item
 constexpr int VNT = 512;    // threads per workgroup
-constexpr int VPOOL = 1280; // staged candidates
+constexpr int VPOOL = 1024; // staged candidates
+constexpr int VNBH = 32;    // bins of the running-bound histograms (the first VNBH of each row)
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -44,10 +52,12 @@ struct VLay {
     static constexpr size_t tab = 0;                                    // [M][256][VG] entries
     static constexpr size_t pool = tab + size_t(M) * 256 * VG * EB;     // uint2 [VPOOL] (scan phase)
     static constexpr size_t R = pool;                                   // float [VG][D] (table phase)
-    static constexpr size_t rn = R + size_t(VG) * D * 4;                // float [VG][M] (table phase)
-    static_assert(rn + size_t(VG) * M * 4 <= pool + size_t(VPOOL) * 8, "table-phase data must fit in the pool");
+    static_assert(size_t(VG) * D * 4 <= size_t(VPOOL) * 8, "the residuals must fit in the pool");
     static constexpr size_t prank = pool + size_t(VPOOL) * 8;           // uint16 [VPOOL]
-    static constexpr size_t isc = prank + size_t(VPOOL) * 2;            // float [M][VG] (hi - lo) / 255
+    static constexpr size_t rn = prank + size_t(VPOOL) * 2;             // float [VG][M]
+    static constexpr size_t lhist = rn + size_t(VG) * M * 4;            // int [VG][VNBH] flush histogram
+    static constexpr size_t hsc = lhist + size_t(VG) * VNBH * 4;        // float [VG] histogram scale VNBH / tau (0: none)
+    static constexpr size_t isc = hsc + VG * 4;                         // float [M][VG] (hi - lo) / 255
     static constexpr size_t lov = isc + size_t(M) * VG * 4;             // float [M][VG] lo_s
     static constexpr size_t mul = lov + size_t(M) * VG * 4;             // float [VG] 2^E_g (FP16) / 1
     static constexpr size_t base = mul + VG * 4;                        // float [VG] sum_s lo_s (8-bit) / 0
@@ -268,11 +278,20 @@ __device__ void vtables(const FastArgs &a, unsigned char *smem, int l, const flo
             basev[g] = 0.0f;
         }
         const float rel = TT == 1 ? 0x1.0p-10f : 0.0f;
+        const bool seeded = tau < 3.0e38f;
         float lm;
-        if (qid[g] < 0) lm = -__builtin_huge_valf();          // empty slot: nothing passes
-        else if (!(tau < 3.0e38f)) lm = __builtin_huge_valf(); // no seed bound: keep everything
+        if (qid[g] < 0) lm = -__builtin_huge_valf();  // empty slot: nothing passes
+        else if (!seeded) lm = __builtin_huge_valf();  // no seed bound: keep everything
         else lm = tau * (1.0f + rel) + slack + 0x1.0p-20f * (fabsf(tau) + bnd);
+        if (qid[g] >= 0 && a.runthr) {
+            // running bound R: >= kl candidates of the query already have an
+            // approximate distance <= R, so a vector above R cannot enter its
+            // shortlist of the kl smallest
+            lm = fminf(lm, a.runthr[qid[g]]);
+        }
         lim[g] = lm;
+        float *hsc = reinterpret_cast<float *>(smem + L::hsc);
+        hsc[g] = (qid[g] >= 0 && seeded && tau > 0.0f && a.runthr) ? float(VNBH) / tau : 0.0f;
     }
     __syncthreads();  // tables and constants ready; the pool may now overwrite R and rn
 }
@@ -335,10 +354,13 @@ __device__ void vflush(const FastArgs &a, unsigned char *smem, int64_t beg, int 
     uint16_t *prank = reinterpret_cast<uint16_t *>(smem + L::prank);
     int *gcnt = reinterpret_cast<int *>(smem + L::gcnt);
     int *gbase = reinterpret_cast<int *>(smem + L::gbase);
+    int *lhist = reinterpret_cast<int *>(smem + L::lhist);
+    const float *hsc = reinterpret_cast<const float *>(smem + L::hsc);
     const int *qid = reinterpret_cast<const int *>(smem + L::qid);
     const int tid = threadIdx.x;
     const int n = min(filled, VPOOL);
     if (tid < VG) gcnt[tid] = 0;
+    for (int i = tid; i < VG * VNBH; i += VNT) lhist[i] = 0;
     lds_barrier();
     for (int i = tid; i < n; i += VNT) prank[i] = uint16_t(atomicAdd(gcnt + (pool[i].x >> 28), 1));
     lds_barrier();
@@ -348,11 +370,43 @@ __device__ void vflush(const FastArgs &a, unsigned char *smem, int64_t beg, int 
         const uint2 e = pool[i];
         const int g = int(e.x >> 28);
         const int pos = gbase[g] + int(prank[i]);
-        if (pos < a.capq)
-            a.cand[int64_t(qid[g]) * a.capq + pos] =
-                ckey(__builtin_bit_cast(float, e.y), int32_t(beg + (e.x & 0x0FFFFFFFu)));
+        const float ad = __builtin_bit_cast(float, e.y);
+        if (pos < a.capq) {
+            a.cand[int64_t(qid[g]) * a.capq + pos] = ckey(ad, int32_t(beg + (e.x & 0x0FFFFFFFu)));
+            if (hsc[g] > 0.0f) atomicAdd(lhist + g * VNBH + min(VNBH - 1, max(0, int(ad * hsc[g]))), 1);
+        }
     }
     lds_barrier();  // the pool is free again
+    if (a.runthr) {
+        // this flush's counts into the query's global histogram of candidate
+        // distances (bin b: [b, b + 1) tau / VNBH); the first bin whose
+        // cumulative count reaches kl bounds the kl-th smallest candidate
+        // distance so far (counts read back may miss concurrent additions,
+        // which only gives a larger, still valid, bound)
+        for (int i = tid; i < VG * VNBH; i += VNT) {
+            const int c = lhist[i];
+            if (c > 0) atomicAdd(a.rhist + int64_t(qid[i / VNBH]) * fast_hist_bins_dev() + (i % VNBH), c);
+        }
+        const int lane = tid & 63;
+        for (int g = tid >> 6; g < VG; g += VNT / 64) {  // one wavefront per slot
+            if (!(hsc[g] > 0.0f) || gcnt[g] == 0) continue;  // wave-uniform
+            const int q = qid[g];
+            int c = lane < VNBH - 1 ? __hip_atomic_load(a.rhist + int64_t(q) * fast_hist_bins_dev() + lane,
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : 0;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int o = __shfl_up(c, off, 64);
+                if (lane >= off) c += o;
+            }
+            const uint64_t reach = __ballot(lane < VNBH - 1 && c >= a.kl);
+            if (reach != 0 && lane == 0) {
+                const int b = __ffsll((unsigned long long)reach) - 1;
+                const float R = (float(b + 1) + 0.01f) / hsc[g];
+                atomicMin(reinterpret_cast<int *>(a.runthr) + q, __float_as_int(R));
+            }
+        }
+    }
 }
 
 template <int M, int TT>

@@ -984,8 +984,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         if (ix->fast_tables == 0) {
             MIVF_CHECK_HIP(launch_fast_scan(a, ix->m, 2 * ncu, st));
         } else {
-            a.runthr = nullptr;  // the FP32-built table types filter by the seed bound alone
-            a.rhist = nullptr;
+            MIVF_REQUIRE(fast_hist_bins() == 64, kHip, "search_fast: histogram stride differs between the scans");
             MIVF_CHECK_HIP(launch_fast_var_scan(a, ix->m, ix->fast_tables, 2 * ncu, st));
         }
     }
