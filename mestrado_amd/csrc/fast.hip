@@ -1205,7 +1205,8 @@ hipError_t launch_seed_m(const SeedArgs &a, hipStream_t s) {
 hipError_t launch_seed_tau(const SeedArgs &a, int m, hipStream_t s) {
     if (a.nq <= 0) return hipSuccess;
     if (a.kl < 1) return hipErrorInvalidValue;
-    if (a.cap > 0 && (a.klq || a.cap < a.kl || a.cap > SCAP)) return hipErrorInvalidValue;
+    // a cap must hold the kl-th key: cap >= kl (per-query klq[q] are <= kl)
+    if (a.cap > 0 && (a.cap < a.kl || a.cap > SCAP)) return hipErrorInvalidValue;
     const bool small = a.cap > 0 && a.cap <= 1024;
     switch (m) {
         case 8: return small ? launch_seed_m<8, 1024>(a, s) : launch_seed_m<8, SCAP>(a, s);
