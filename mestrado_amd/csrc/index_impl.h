@@ -140,7 +140,7 @@ struct mivf_index {
     int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
     int fast_seed_cap = 1024; // vectors entering the seed bound (0 = all of the seed lists)
     int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
-    int batched_cap = 8192;   // candidate capacity per query of the batched exact form (>= 2 kmax)
+    int batched_cap = 16384;  // candidate capacity per query of the batched exact form (>= 2 kmax)
     int batched_g = 4, batched_nt = 512;  // its work-item shape (queries per item, threads)
     // mivf_search on batches of at least list_major_min queries runs the
     // batched list-major kernels (identical results)

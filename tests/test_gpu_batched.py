@@ -45,7 +45,7 @@ def test_batched_overflow_fallback(mivf, zipf16):
         ids, dis = ix.search_batched(query, w, k, 100)
         over = ix.batched_stats()
     finally:
-        ix.set_option("batched_capacity", 8192)  # the library default
+        ix.set_option("batched_capacity", 16384)  # the library default
     assert over > 0, "the tiny buffer should overflow"
     assert_same(ids, dis, rids, rdis, "batched with overflow")
     ids, dis = ix.search_batched(query, w, k, 100)
