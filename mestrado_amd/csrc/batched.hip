@@ -93,9 +93,8 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     sa.nlist = nlist;
     sa.ks = ix->ks;
     sa.kl = kmax;
-    // the first 1024 vectors of the seed lists (a sample: the k-th smallest
-    // of a subset of the probed vectors still bounds the k-th of all)
-    sa.cap = std::max(1024, kmax);
+    // the whole seed lists: a tighter bound than a sample of them keeps the
+    // candidates, and so the select, small
     sa.tau = x.tau.as<float>();
     MIVF_CHECK_HIP(launch_seed_tau(sa, ix->m, st));
     MIVF_CHECK_HIP(launch_pairs(probes, nq, wmax, nullptr, ix->off.as<int64_t>(), x.tau.as<float>(), nlist, cfg.g,
