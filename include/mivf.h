@@ -354,6 +354,29 @@ int mivf_write_ivf_file(const char *path, int coarsek, int m, const int64_t *off
 int mivf_read_ivf_file(const char *path, int coarsek, int m, int64_t *offsets, int32_t *ids,
                        uint8_t *codes, int64_t cap);
 
+/* Vector files: fvecs_read / b2fvecs_read (parallel/yael_needs/vector.c:296-350),
+ * ivecs_read and my_bvecs_read (parallel/pq-utils/pq_test_load_vectors.cpp:236-325).
+ * Each record is int32 d, then d float32 (fvecs), int32 (ivecs) or uint8
+ * (bvecs) values.  vecs_info reads the dimension of the first record and the
+ * record count (elem_bytes 4 for fvecs/ivecs, 1 for bvecs; an empty file
+ * gives d = n = 0).  The readers copy records [row0, row0 + n) into out
+ * [n][d] (bvecs values widened to float32, as my_bvecs_read does); a record
+ * of another dimension, or a file shorter than row0 + n records, gives
+ * MIVF_EIO.                                                                  */
+int mivf_vecs_info(const char *path, int elem_bytes, int *d, int64_t *n);
+int mivf_read_fvecs(const char *path, int64_t row0, int64_t n, int d, float *out);
+int mivf_read_bvecs(const char *path, int64_t row0, int64_t n, int d, float *out);
+int mivf_read_ivecs(const char *path, int64_t row0, int64_t n, int d, int32_t *out);
+/* Replaces the load-then-ivfpq_assign loop of the reference's index builders
+ * (parallel2/genivf.cpp:33-60, parallel/ivf_pq/ivf_training.cpp:165-224):
+ * records [row0, row0 + n) of an fvecs (elem_bytes 4) or bvecs (elem_bytes
+ * 1) file of the index's dimension are read `chunk` records at a time
+ * (chunk <= 0: 2^20) and added as mivf_index_add would, ids id0, id0+1, ...
+ * Host memory holds one chunk; the lists end exactly as one mivf_index_add
+ * of all n rows would leave them.                                           */
+int mivf_index_add_file(mivf_index *ix, const char *path, int elem_bytes, int64_t row0, int64_t n,
+                        int32_t id0, int64_t chunk);
+
 #ifdef __cplusplus
 }
 #endif

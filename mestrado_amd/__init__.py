@@ -96,6 +96,11 @@ _SIGS = {
     "mivf_read_ivf_dir": [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _i64],
     "mivf_write_ivf_file": [ctypes.c_char_p, _i, _i, _vp, _vp, _vp],
     "mivf_read_ivf_file": [ctypes.c_char_p, _i, _i, _vp, _vp, _vp, _i64],
+    "mivf_vecs_info": [ctypes.c_char_p, _i, _i32p, _i64p],
+    "mivf_read_fvecs": [ctypes.c_char_p, _i64, _i64, _i, _vp],
+    "mivf_read_bvecs": [ctypes.c_char_p, _i64, _i64, _i, _vp],
+    "mivf_read_ivecs": [ctypes.c_char_p, _i64, _i64, _i, _vp],
+    "mivf_index_add_file": [_vp, ctypes.c_char_p, _i, _i64, _i64, ctypes.c_int32, _i64],
 }
 
 
@@ -255,6 +260,30 @@ def lds_read_rate(nbytes, random_rows):
     return v.value
 
 
+_VECS_ELEM = {"fvecs": 4, "ivecs": 4, "bvecs": 1}
+
+
+def vecs_info(path, kind):
+    """(d, n) of an fvecs / ivecs / bvecs file."""
+    d = ctypes.c_int32(0)
+    n = ctypes.c_int64(0)
+    _check(lib().mivf_vecs_info(str(path).encode(), _VECS_ELEM[kind], ctypes.byref(d), ctypes.byref(n)))
+    return d.value, n.value
+
+
+def read_vecs(path, kind, row0=0, n=None):
+    """Records [row0, row0 + n) of a vector file as an [n, d] array: float32
+    for fvecs and bvecs (bytes widened), int32 for ivecs."""
+    d, total = vecs_info(path, kind)
+    if n is None:
+        n = max(total - row0, 0)
+    out = np.empty((n, d), np.int32 if kind == "ivecs" else np.float32)
+    fn = {"fvecs": lib().mivf_read_fvecs, "bvecs": lib().mivf_read_bvecs, "ivecs": lib().mivf_read_ivecs}[kind]
+    if n:
+        _check(fn(str(path).encode(), row0, n, d, _ptr(out)))
+    return out
+
+
 def bruteforce_nn(base, q):
     """Exact FP64 nearest neighbour: returns (ids [nq] int32, dist [nq] float64)."""
     bd, _ = _dev(base, np.float32)
@@ -326,6 +355,15 @@ class Index:
         else:
             x = np.ascontiguousarray(x, dtype=np.float32)
             _check(lib().mivf_index_add(self.h, _ptr(x), x.shape[0], id0))
+
+    def add_file(self, path, kind="fvecs", row0=0, n=None, id0=0, chunk=0):
+        """Add records [row0, row0 + n) of an fvecs or bvecs file, `chunk`
+        records at a time (0: 2^20)."""
+        if kind not in ("fvecs", "bvecs"):
+            raise ValueError("add_file reads fvecs or bvecs")
+        if n is None:
+            n = max(vecs_info(path, kind)[1] - row0, 0)
+        _check(lib().mivf_index_add_file(self.h, str(path).encode(), _VECS_ELEM[kind], row0, n, id0, chunk))
 
     def encode(self, x):
         xd, _ = _dev(x, np.float32)

@@ -711,6 +711,26 @@ int mivf_index_add(mivf_index *ix, const float *x, int64_t n, int32_t id0) try {
     return kOk;
 } MIVF_CATCH_ALL
 
+int mivf_index_add_file(mivf_index *ix, const char *path, int elem_bytes, int64_t row0, int64_t n, int32_t id0,
+                        int64_t chunk) try {
+    MIVF_TRY(check_index(ix));
+    MIVF_REQUIRE(path && (elem_bytes == 1 || elem_bytes == 4) && row0 >= 0 && n >= 0, kInvalid,
+                 "index_add_file: bad arguments");
+    MIVF_REQUIRE(int64_t(id0) + n <= int64_t(INT32_MAX) + 1, kInvalid, "index_add_file: ids past 2^31 - 1");
+    if (n == 0) return kOk;
+    const int64_t ch = std::max<int64_t>(1, std::min<int64_t>(chunk > 0 ? chunk : (int64_t(1) << 20), n));
+    std::vector<float> h(size_t(ch) * ix->d);
+    DevBuf dx;
+    MIVF_TRY(dx.ensure(sizeof(float) * h.size()));
+    for (int64_t r = 0; r < n; r += ch) {
+        const int64_t nn = std::min(ch, n - r);
+        MIVF_TRY(read_vecs_rows<float>(path, elem_bytes, row0 + r, nn, ix->d, h.data()));
+        MIVF_CHECK_HIP(hipMemcpy(dx.p, h.data(), sizeof(float) * size_t(nn) * ix->d, hipMemcpyHostToDevice));
+        MIVF_TRY(add_impl(ix, dx.as<float>(), nn, id0 + int32_t(r)));
+    }
+    return kOk;
+} MIVF_CATCH_ALL
+
 int mivf_index_add_device(mivf_index *ix, const float *d_x, int64_t n, int32_t id0) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(n >= 0 && (n == 0 || d_x), kInvalid, "add: bad arguments");

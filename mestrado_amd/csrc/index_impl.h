@@ -182,6 +182,10 @@ namespace mivf {
 int search_entry(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, const int32_t *d_wq,
                  const int32_t *d_kq, int kuni, int kstride, int32_t *d_ids, float *d_dis, hipStream_t st);
 int upload_lists(mivf_index *ix);
+// Records [row0, row0 + n) of an fvecs / ivecs (elem 4) or bvecs (elem 1)
+// file of dimension d, converted to T (io.hip).
+template <typename T>
+int read_vecs_rows(const char *path, int elem, int64_t row0, int64_t n, int d, T *out);
 // Whether mivf_search sends this batch to the list-major kernels, and that
 // path (batched.hip): uniform w and k, results identical to search_entry.
 bool list_major_ok(const mivf_index *ix, const float *d_q, int nq);

@@ -9,8 +9,14 @@
 //               parallel2/ivf_pq/ivf_search.cpp:293-321)
 //  single file  int32 coarsek, then per list: int32 n, int32 ids[n], int32 n,
 //               int32 m, int32 codes[n][m] (parallel/ivf_pq/ivf_search.cpp:396-441)
+//  vector files fvecs / ivecs / bvecs: records of int32 d followed by d float32
+//               / int32 / uint8 values (parallel/yael_needs/vector.c:296-350,
+//               parallel/pq-utils/pq_test_load_vectors.cpp:236-325); bvecs values
+//               widen to float32, and a read may start at any record (as
+//               my_bvecs_read's offset does)
 // Every read is size-checked: a truncated or inconsistent file gives MIVF_EIO.
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <string>
 #include <vector>
@@ -122,7 +128,79 @@ int finish_lists(int nlist, int m, std::vector<std::vector<int32_t>> &lid,
 
 }  // namespace
 
+namespace mivf {
+
+// Records [row0, row0 + n) of a vector file with elem-byte values of
+// dimension d, converted to T (float from float32 or uint8, int32 from int32).
+template <typename T>
+int read_vecs_rows(const char *path, int elem, int64_t row0, int64_t n, int d, T *out) {
+    MIVF_REQUIRE(path && d > 0 && row0 >= 0 && n >= 0 && (n == 0 || out), kInvalid, "read_vecs: bad arguments");
+    MIVF_REQUIRE(elem == 1 || elem == 4, kInvalid, "read_vecs: elem must be 1 (bvecs) or 4 (fvecs, ivecs)");
+    File fh(path, "rb");
+    MIVF_TRY(open_or_fail(fh, path));
+    const int64_t rec = 4 + int64_t(d) * elem;
+    const int64_t left = bytes_left(fh.f);
+    MIVF_REQUIRE(left >= 0 && (row0 + n) * rec <= left, kIo,
+                 std::string("vector file shorter than the requested records: ") + path);
+    MIVF_REQUIRE(std::fseek(fh.f, long(row0 * rec), SEEK_SET) == 0, kIo, std::string("seek failed in ") + path);
+    const int64_t CH = 4096;  // records per read
+    std::vector<unsigned char> buf(size_t(std::min(n, CH) * rec));
+    for (int64_t r0 = 0; r0 < n; r0 += CH) {
+        const int64_t nn = std::min(CH, n - r0);
+        MIVF_TRY(read_exact(fh.f, buf.data(), size_t(rec), size_t(nn), path));
+        for (int64_t i = 0; i < nn; ++i) {
+            const unsigned char *p = buf.data() + i * rec;
+            int32_t dd;
+            std::memcpy(&dd, p, 4);
+            MIVF_REQUIRE(dd == d, kIo, std::string("unexpected vector dimension in ") + path);
+            T *o = out + (r0 + i) * d;
+            if (elem == 1) {
+                for (int t = 0; t < d; ++t) o[t] = T(p[4 + t]);
+            } else {
+                std::memcpy(o, p + 4, size_t(d) * 4);
+            }
+        }
+    }
+    return kOk;
+}
+template int read_vecs_rows<float>(const char *, int, int64_t, int64_t, int, float *);
+
+}  // namespace mivf
+
 extern "C" {
+
+int mivf_vecs_info(const char *path, int elem_bytes, int *d, int64_t *n) try {
+    MIVF_REQUIRE(path && (elem_bytes == 1 || elem_bytes == 4), kInvalid,
+                 "vecs_info: elem_bytes must be 1 (bvecs) or 4 (fvecs, ivecs)");
+    File fh(path, "rb");
+    MIVF_TRY(open_or_fail(fh, path));
+    const int64_t size = bytes_left(fh.f);
+    int32_t dd = 0;
+    if (size == 0) {
+        if (d) *d = 0;
+        if (n) *n = 0;
+        return kOk;
+    }
+    MIVF_TRY(read_exact(fh.f, &dd, 4, 1, path));
+    MIVF_REQUIRE(dd > 0, kIo, std::string("non-positive vector dimension in ") + path);
+    const int64_t rec = 4 + int64_t(dd) * elem_bytes;
+    MIVF_REQUIRE(size % rec == 0, kIo, std::string("vector file size is not a whole number of records: ") + path);
+    if (d) *d = dd;
+    if (n) *n = size / rec;
+    return kOk;
+} MIVF_CATCH_ALL
+
+int mivf_read_fvecs(const char *path, int64_t row0, int64_t n, int d, float *out) try {
+    return read_vecs_rows<float>(path, 4, row0, n, d, out);
+} MIVF_CATCH_ALL
+
+int mivf_read_bvecs(const char *path, int64_t row0, int64_t n, int d, float *out) try {
+    return read_vecs_rows<float>(path, 1, row0, n, d, out);
+} MIVF_CATCH_ALL
+
+int mivf_read_ivecs(const char *path, int64_t row0, int64_t n, int d, int32_t *out) try {
+    return read_vecs_rows<int32_t>(path, 4, row0, n, d, out);
+} MIVF_CATCH_ALL
 
 int mivf_write_cent(const char *header, const char *cent, const char *coa, int nsq, int ks, int ds,
                     int coarsek, int d, const float *pq, const float *coarse) try {

@@ -184,6 +184,80 @@ def test_ivf_dir_layout_matches_reference(L, tmp_path):
     assert L.mivf_read_ivf_dir(d.encode(), 4, m, off2.ctypes.data, None, None, ctypes.c_int64(0)) == 4
 
 
+def _write_vecs(path, a, dtype):
+    """Vector-file writer of this test: int32 d then d values per record."""
+    a = np.ascontiguousarray(a)
+    n, d = a.shape
+    with open(path, "wb") as f:
+        for r in a:
+            f.write(np.int32(d).tobytes())
+            f.write(r.astype(dtype).tobytes())
+
+
+def test_read_fvecs_reference_queries():
+    """The reference's own siftsmall query file: 100 records of 128 floats
+    (the file the reference's tests load with fvecs_read)."""
+    import mestrado_amd as M
+    sys_path = os.path.join(GOLD, "ref", "siftsmall_query.fvecs")
+    assert M.vecs_info(sys_path, "fvecs") == (128, 100)
+    raw = np.fromfile(sys_path, np.int32).reshape(100, 129)
+    assert (raw[:, 0] == 128).all()
+    want = raw[:, 1:].copy().view(np.float32)
+    assert np.array_equal(M.read_vecs(sys_path, "fvecs"), want)
+    assert np.array_equal(M.read_vecs(sys_path, "fvecs", 37, 11), want[37:48])
+    assert M.read_vecs(sys_path, "fvecs", 100).shape == (0, 128)
+
+
+def test_read_bvecs_ivecs_roundtrip(tmp_path):
+    import mestrado_amd as M
+    rng = np.random.default_rng(5)
+    b = rng.integers(0, 256, (1000, 128)).astype(np.uint8)
+    iv = rng.integers(-2 ** 31, 2 ** 31 - 1, (300, 100)).astype(np.int32)
+    _write_vecs(tmp_path / "x.bvecs", b, np.uint8)
+    _write_vecs(tmp_path / "gt.ivecs", iv, np.int32)
+    assert M.vecs_info(tmp_path / "x.bvecs", "bvecs") == (128, 1000)
+    got = M.read_vecs(tmp_path / "x.bvecs", "bvecs", 123, 500)
+    assert got.dtype == np.float32 and np.array_equal(got, b[123:623].astype(np.float32))
+    # the record boundary of a 4097-row read (two internal chunks)
+    big = rng.integers(0, 256, (5000, 16)).astype(np.uint8)
+    _write_vecs(tmp_path / "big.bvecs", big, np.uint8)
+    assert np.array_equal(M.read_vecs(tmp_path / "big.bvecs", "bvecs", 3, 4097), big[3:4100].astype(np.float32))
+    assert np.array_equal(M.read_vecs(tmp_path / "gt.ivecs", "ivecs"), iv)
+    (tmp_path / "empty.fvecs").write_bytes(b"")
+    assert M.vecs_info(tmp_path / "empty.fvecs", "fvecs") == (0, 0)
+
+
+def test_read_vecs_errors(L, tmp_path):
+    """Past the end, a dimension change inside the file, a ragged tail and a
+    missing file are MIVF_EIO; bad arguments are MIVF_EINVAL."""
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((10, 8)).astype(np.float32)
+    p = tmp_path / "x.fvecs"
+    _write_vecs(p, x, np.float32)
+    out = np.empty((11, 8), np.float32)
+    assert L.mivf_read_fvecs(str(p).encode(), 0, 11, 8, out.ctypes.data) == 4
+    assert L.mivf_read_fvecs(str(p).encode(), 5, 6, 8, out.ctypes.data) == 4
+    assert L.mivf_read_fvecs(str(p).encode(), 5, 5, 8, out.ctypes.data) == 0
+    assert np.array_equal(out[:5], x[5:])
+    # dimension 7 asked of an 8-dim file: records misalign, the d field differs
+    assert L.mivf_read_fvecs(str(p).encode(), 0, 3, 7, out.ctypes.data) == 4
+    raw = bytearray(p.read_bytes())
+    raw[36 * 4:36 * 4 + 4] = np.int32(9).tobytes()  # record 4's d field
+    q = tmp_path / "bad.fvecs"
+    q.write_bytes(bytes(raw))
+    assert L.mivf_read_fvecs(str(q).encode(), 0, 4, 8, out.ctypes.data) == 0
+    assert L.mivf_read_fvecs(str(q).encode(), 0, 5, 8, out.ctypes.data) == 4
+    assert b"dimension" in L.mivf_last_error()
+    (tmp_path / "ragged.fvecs").write_bytes(p.read_bytes()[:-3])
+    d = ctypes.c_int32(0)
+    n = ctypes.c_int64(0)
+    assert L.mivf_vecs_info(str(tmp_path / "ragged.fvecs").encode(), 4, ctypes.byref(d), ctypes.byref(n)) == 4
+    assert L.mivf_vecs_info(b"/nonexistent/x.fvecs", 4, ctypes.byref(d), ctypes.byref(n)) == 4
+    assert L.mivf_vecs_info(str(p).encode(), 2, ctypes.byref(d), ctypes.byref(n)) == 1
+    assert L.mivf_read_fvecs(str(p).encode(), -1, 1, 8, out.ctypes.data) == 1
+    assert L.mivf_read_fvecs(str(p).encode(), 0, 1, 0, out.ctypes.data) == 1
+
+
 def test_argument_validation_without_gpu(L):
     """Bad shapes are rejected before any device work."""
     co = np.zeros((16, 128), np.float32)

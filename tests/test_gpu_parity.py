@@ -71,6 +71,44 @@ def test_lists_match_golden_digests(c1_index):
         assert hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest() == man["sha256"][name], name
 
 
+def _write_vecs(path, a, dtype):
+    with open(path, "wb") as f:
+        for r in np.ascontiguousarray(a):
+            f.write(np.int32(r.shape[0]).tobytes())
+            f.write(r.astype(dtype).tobytes())
+
+
+def test_add_from_vector_files(mivf, orc, c1, c1_index, tmp_path):
+    """Adding straight from an fvecs file in uneven chunks leaves the lists
+    that one add of the same rows leaves; a bvecs file (bytes widened to
+    float) from row 1234 with ids from 50000 matches the oracle's encoding."""
+    q = c1["gold_q"]
+    base = c1["base"]
+    _write_vecs(tmp_path / "base.fvecs", base, np.float32)
+    ix = mivf.Index(q["coarse"], q["pq"])
+    try:
+        ix.add_file(tmp_path / "base.fvecs", "fvecs", chunk=3001)
+        for a, b in zip(ix.get_lists(), c1_index.get_lists()):
+            assert np.array_equal(a, b)
+    finally:
+        ix.close()
+    b8 = np.clip(np.rint(base), 0, 255).astype(np.uint8)
+    _write_vecs(tmp_path / "base.bvecs", b8, np.uint8)
+    ix = mivf.Index(q["coarse"], q["pq"])
+    try:
+        n = b8.shape[0] - 1234
+        ix.add_file(tmp_path / "base.bvecs", "bvecs", row0=1234, n=n, id0=50_000, chunk=4096)
+        off, ids, codes = ix.get_lists()
+        ra, rc = orc.encode(q["coarse"], q["pq"], b8[1234:].astype(np.float32))
+        pos = ids - 50_000
+        assert np.array_equal(np.sort(pos), np.arange(n))
+        lst = np.searchsorted(off, np.arange(n), side="right") - 1
+        assert np.array_equal(ra[pos], lst)
+        assert np.array_equal(rc[pos], codes)
+    finally:
+        ix.close()
+
+
 def test_kmeans_bitexact(mivf, orc, c1):
     x = c1["train"][:6000]
     g = mivf.kmeans(x, 64, 6, 99)
