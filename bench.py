@@ -60,6 +60,8 @@ def main():
                                                                    "(0 = library default)")
     ap.add_argument("--serve-fracs", default="0.25,0.5,0.75", help="offered load as fractions of the batch QPS")
     ap.add_argument("--serve-reqs", type=int, default=12000)
+    ap.add_argument("--serve-max-batch", type=int, default=4096, help="queries per serving micro-batch at most")
+    ap.add_argument("--serve-slots", type=int, default=3, help="serving micro-batches in flight")
     ap.add_argument("--precision-study", action="store_true",
                     help="the reduced-precision table types on this workload's index (w=32, shortlist 1024, "
                          "k=100): recall and distance error against FP64 per table type")
@@ -330,17 +332,25 @@ def main():
     if not args.no_serve and not args.profile_only:
         qps_peak = nq / t_plain
         mean_nq = 147.6  # mean of log-uniform 1..1024
+        common = dict(bmax=1024, w_choices=(w,), k_choices=(k,), slots=args.serve_slots,
+                      max_batch=args.serve_max_batch)
+        sat = M.serve_bench(ix, query, rate=1e9, nreq=min(args.serve_reqs, 4000), seed=21, **common)
+        log(f"serving saturated: {sat['qps']:.0f} q/s")
         pts = []
         for frac in [float(x) for x in args.serve_fracs.split(",")]:
-            st = M.serve_bench(ix, query, rate=frac * qps_peak / mean_nq, nreq=args.serve_reqs, seed=11,
-                               bmax=1024, w_choices=(w,), k_choices=(k,), slots=3, max_batch=1024)
+            st = M.serve_bench(ix, query, rate=frac * qps_peak / mean_nq, nreq=args.serve_reqs, seed=11, **common)
             pts.append({"load": frac, "offered_qps": round(st["offered_qps"], 1), "qps": round(st["qps"], 1),
                         "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
-                        "mean_ms": round(st["mean_ms"], 3), "batches": int(st["batches"])})
+                        "mean_ms": round(st["mean_ms"], 3), "batches": int(st["batches"]),
+                        "list_major_batches": int(st["list_major_batches"]),
+                        "rerun_queries": int(st["rerun_queries"])})
             log(f"serving at {frac:.2f} of the batch QPS: {pts[-1]}")
-        serving = {"form": "plain exact FP32 (per-request w, k)", "w": w, "k": k, "slots": 3, "max_batch": 1024,
-                   "requests_per_point": args.serve_reqs, "batch_sizes": "log-uniform 1..1024",
-                   "arrivals": "Poisson (open loop)", "load_reference_qps": round(qps_peak, 1), "points": pts}
+        serving = {"form": "plain exact FP32 (per-request w, k)", "w": w, "k": k, "slots": args.serve_slots,
+                   "max_batch": args.serve_max_batch, "requests_per_point": args.serve_reqs,
+                   "batch_sizes": "log-uniform 1..1024", "arrivals": "Poisson (open loop)",
+                   "load_reference_qps": round(qps_peak, 1),
+                   "load_reference": "the plain headline batch QPS (value)",
+                   "saturated_qps": round(sat["qps"], 1), "points": pts}
 
     if args.profile_only:
         # a plain device-to-device copy of known size: the reference point
@@ -669,8 +679,8 @@ def streamed_serving(args, cfg):
 
     # the server's saturated throughput: all requests arrive at once
     mean_nq = 147.6
-    common = dict(bmax=1024, w_choices=tuple(cfg["serve_w"]), k_choices=tuple(cfg["serve_k"]), slots=3,
-                  max_batch=1024)
+    common = dict(bmax=1024, w_choices=tuple(cfg["serve_w"]), k_choices=tuple(cfg["serve_k"]),
+                  slots=args.serve_slots, max_batch=args.serve_max_batch)
     sat = M.serve_bench(ix, pool, rate=1e9, nreq=min(args.serve_reqs, 4000), seed=21, **common)
     peak = sat["qps"]
     log(f"serving saturated: {peak:.0f} q/s")
@@ -679,7 +689,8 @@ def streamed_serving(args, cfg):
         sb = M.serve_bench(ix, pool, rate=frac * peak / mean_nq, nreq=args.serve_reqs, seed=11, **common)
         pts.append({"load": frac, "offered_qps": round(sb["offered_qps"], 1), "qps": round(sb["qps"], 1),
                     "p50_ms": round(sb["p50_ms"], 3), "p99_ms": round(sb["p99_ms"], 3),
-                    "mean_ms": round(sb["mean_ms"], 3), "batches": int(sb["batches"])})
+                    "mean_ms": round(sb["mean_ms"], 3), "batches": int(sb["batches"]),
+                    "list_major_batches": int(sb["list_major_batches"])})
         log(f"serving at {frac:.2f} of saturation: {pts[-1]}")
     res = {
         "metric": "QPS at fixed recall@100 (10M x128 IVF4096,PQ16); ADC-scan HBM GB/s; p99 ms",
@@ -694,7 +705,8 @@ def streamed_serving(args, cfg):
                    "recall_queries": ng, **rec},
         "stages_ms": {n: round(float(v), 4) for n, v in zip(("coarse", "select", "scan", "merge"), st)},
         "serving": {"form": "plain exact FP32 (per-request w, k)", "w_choices": list(cfg["serve_w"]),
-                    "k_choices": list(cfg["serve_k"]), "slots": 3, "max_batch": 1024,
+                    "k_choices": list(cfg["serve_k"]), "slots": args.serve_slots,
+                    "max_batch": args.serve_max_batch,
                     "requests_per_point": args.serve_reqs, "batch_sizes": "log-uniform 1..1024",
                     "arrivals": "Poisson (open loop)", "query_pool": pool_n, "saturated_qps": round(peak, 1),
                     "points": pts},

@@ -229,10 +229,14 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * the FP16 matrix cores with one scale per (query, list) and int8
  * matrix-core sums, the default; 1 = FP16 entries; 2 = 8-bit entries with a
  * per-subquantizer offset and scale; 1 and 2 are built from FP32 tables and
- * summed in FP32), "list_major_min_queries" (default 2048), "coarse_mfma" (1 = coarse top-w
- * through BF16 matrix-core distances and the exact recheck, for d = 128 and
- * nlist <= 32768; 0 = exact FP32 distance tiles for every shape; the same
- * probes either way).                                                       */
+ * summed in FP32), "list_major_min_queries" (default 2048), "coarse_mfma"
+ * (1 = coarse top-w through BF16 matrix-core distances and the exact
+ * recheck, for d = 128 and nlist <= 32768; 0 = exact FP32 distance tiles for
+ * every shape; the same probes either way), "serve_min_fill" and
+ * "serve_max_wait_us" (servers over this index: while a micro-batch is in
+ * flight, a free slot launches only once serve_min_fill queries wait or the
+ * oldest waiting request has waited serve_max_wait_us; defaults 1024 and
+ * 300; serve_min_fill 0 launches as soon as a slot is free).               */
 int mivf_set_option(mivf_index *ix, const char *name, int64_t value);
 /* Per-stage milliseconds of the last profiled search:
  * [0] coarse distances, [1] probe select, [2] scan, [3] merge (mivf_search on
@@ -287,8 +291,11 @@ int mivf_plan_free(mivf_plan *p);
  * [nq][k]) wait in arrival order; up to `slots` micro-batches run at once,
  * each on its own stream with its own workspace, packed from the waiting
  * requests up to max_batch queries and within mem_budget bytes of per-request
- * device state (0 = 4 GiB).  Searches are exact (mivf_search_var kernels), so
- * a request's results are the same alone and in any mix.  The server is driven
+ * device state (0 = 4 GiB).  Searches are exact: a micro-batch of at least
+ * the index's list_major_min_queries queries runs the list-major kernels of
+ * mivf_search_batched (m in {8, 16}, nlist <= 8192), a smaller one the
+ * per-query kernels of mivf_search_var; both give the same rows, so a
+ * request's results are the same alone and in any mix.  The server is driven
  * by the calling thread: submit, then progress / wait / drain.                */
 typedef struct mivf_server mivf_server;
 int mivf_server_create(mivf_index *ix, int slots, int max_batch, int64_t mem_budget, mivf_server **out);
@@ -300,11 +307,26 @@ int mivf_server_wait(mivf_server *sv, int64_t ticket);
 int mivf_server_drain(mivf_server *sv);
 /* Submission-to-completion time of a finished request, milliseconds.       */
 int mivf_server_latency(mivf_server *sv, int64_t ticket, double *ms);
+/* Grow every slot's workspaces before serving, so that no later micro-batch
+ * allocates device memory: each slot runs, untimed, micro-batches of w and k
+ * (the largest the server will see) at max_batch queries and, when the
+ * list-major kernels apply, at list_major_min_queries - 1, reading query rows
+ * from d_rows [n_rows][d] (fewer if n_rows is smaller).  Only while no request
+ * waits or runs; tickets issued earlier become invalid; counters restart.    */
+int mivf_server_warm(mivf_server *sv, const float *d_rows, int64_t n_rows, int w, int k);
+/* Counters since creation (or the last warm): micro-batches launched, those that ran the
+ * list-major kernels, and queries of those rerun by the per-query kernels
+ * because their candidates overflowed (any pointer may be NULL).           */
+int mivf_server_stats(mivf_server *sv, int64_t *batches, int64_t *list_major_batches,
+                      int64_t *rerun_queries);
 /* Open-loop benchmark: nreq requests, Poisson arrivals at `rate` requests/s,
  * nq log-uniform in 1..bmax, w and k drawn from the given choices, queries at
- * random rows of d_pool [pool_n][d].  stats[8] = achieved queries/s, p50 ms,
+ * random rows of d_pool [pool_n][d].  stats[10] = achieved queries/s, p50 ms,
  * p99 ms, mean ms, queries served, makespan ms, micro-batches, offered
- * queries/s (latency counted from each request's scheduled arrival).       */
+ * queries/s, list-major micro-batches, queries rerun after a candidate
+ * overflow (latency counted from each request's scheduled arrival).  Before
+ * the clock starts, mivf_server_warm grows the slots' workspaces for the
+ * largest w and k.                                                          */
 int mivf_serve_bench(mivf_index *ix, int slots, int max_batch, int64_t mem_budget, const float *d_pool,
                      int64_t pool_n, double rate, int nreq, uint64_t seed, int bmax,
                      const int32_t *w_choices, int nw, const int32_t *k_choices, int nk, double *stats);

@@ -80,6 +80,8 @@ _SIGS = {
     "mivf_server_wait": [_vp, _i64],
     "mivf_server_drain": [_vp],
     "mivf_server_latency": [_vp, _i64, ctypes.POINTER(ctypes.c_double)],
+    "mivf_server_stats": [_vp, _i64p, _i64p, _i64p],
+    "mivf_server_warm": [_vp, _vp, _i64, _i, _i],
     "mivf_serve_bench": [_vp, _i, _i, _i64, _vp, _i64, ctypes.c_double, _i, _u64, _i, _vp, _i, _vp, _i,
                          ctypes.POINTER(ctypes.c_double)],
     "mivf_set_option": [_vp, ctypes.c_char_p, _i64],
@@ -578,6 +580,20 @@ class Server:
         _check(lib().mivf_server_latency(self.h, ticket, ctypes.byref(v)))
         return v.value
 
+    def warm(self, rows, w, k):
+        """Grow every slot's workspaces for micro-batches of w and k (untimed
+        searches of `rows`); only while nothing waits or runs."""
+        rd, _ = _dev(rows, np.float32)
+        self._warm_rows = rd
+        _check(lib().mivf_server_warm(self.h, rd.ptr, rd.shape[0], w, k))
+        self._keep = {}
+
+    def stats(self):
+        """{batches, list_major_batches, rerun_queries} since creation or warm()."""
+        v = [ctypes.c_int64(0) for _ in range(3)]
+        _check(lib().mivf_server_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("batches", "list_major_batches", "rerun_queries"), (x.value for x in v)))
+
     def close(self):
         if self.h is not None:
             lib().mivf_server_free(self.h)
@@ -596,10 +612,11 @@ def serve_bench(index, pool, rate, nreq, seed=1, bmax=1024, w_choices=(32,), k_c
     pd, _ = _dev(pool, np.float32)
     wc = np.ascontiguousarray(w_choices, dtype=np.int32)
     kc = np.ascontiguousarray(k_choices, dtype=np.int32)
-    st = (ctypes.c_double * 8)()
+    st = (ctypes.c_double * 10)()
     _check(lib().mivf_serve_bench(index.h, slots, max_batch, mem_budget, pd.ptr, pd.shape[0], float(rate), nreq,
                                   seed, bmax, _ptr(wc), len(wc), _ptr(kc), len(kc), st))
-    keys = ("qps", "p50_ms", "p99_ms", "mean_ms", "queries", "makespan_ms", "batches", "offered_qps")
+    keys = ("qps", "p50_ms", "p99_ms", "mean_ms", "queries", "makespan_ms", "batches", "offered_qps",
+            "list_major_batches", "rerun_queries")
     return {k: float(v) for k, v in zip(keys, st)}
 
 

@@ -234,9 +234,13 @@ XWs &index_xws(mivf_index *ix);
 
 }  // namespace
 
-bool list_major_ok(const mivf_index *ix, const float *d_q, int nq) {
+bool list_major_fits(const mivf_index *ix, int nq) {
     return ix->list_major && nq >= ix->list_major_min && (ix->m == 8 || ix->m == 16) && ix->nlist <= 8192 &&
-           cfg_ok(ix->m, XCfg{ix->batched_g, ix->batched_nt}) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0;
+           cfg_ok(ix->m, XCfg{ix->batched_g, ix->batched_nt});
+}
+
+bool list_major_ok(const mivf_index *ix, const float *d_q, int nq) {
+    return list_major_fits(ix, nq) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0;
 }
 
 // mivf_search of a large batch: the batched list-major kernels with every
@@ -273,6 +277,43 @@ int search_list_major(mivf_index *ix, const float *d_q, int nq, int w, int k, in
     std::copy(kept, kept + 8, ix->timings);
     ix->batched_last_overflow = nov;
     return kOk;
+}
+
+std::shared_ptr<void> new_list_major_ws(hipStream_t st) {
+    auto x = std::make_shared<XWs>();
+    x->ws.order_on(st);
+    for (DevBuf *b : {&x->wq, &x->kq, &x->ns, &x->tau, &x->cnt, &x->poff, &x->ioff, &x->fill, &x->pairq, &x->items,
+                      &x->ctrl, &x->cand, &x->ccnt, &x->ovf})
+        b->ord = st;
+    return x;
+}
+
+// Device bytes per query the list-major kernels hold beyond the plain path's:
+// the candidate buffer of capacity(kmax) (distance, id) pairs.
+int64_t list_major_query_bytes(const mivf_index *ix, int kmax) { return int64_t(capacity(ix, kmax)) * 8; }
+
+int list_major_enqueue(mivf_index *ix, void *ws, const float *d_q, int nq, int wmax, const int32_t *d_w,
+                       const int32_t *d_k, int kmax, int32_t *d_ids, float *d_dis, int *h_flags, hipStream_t st) {
+    XWs &x = *static_cast<XWs *>(ws);
+    MIVF_REQUIRE((reinterpret_cast<uintptr_t>(d_q) & 15u) == 0, kInvalid, "list-major batch: queries not 16-byte aligned");
+    MIVF_REQUIRE(kmax >= 1 && kmax <= 1024 && wmax >= 1 && wmax <= std::min(ix->nlist, 1024), kInvalid,
+                 "list-major batch: w or k out of range");
+    MIVF_TRY(x.wq.ensure(sizeof(int32_t) * nq));
+    MIVF_TRY(x.kq.ensure(sizeof(int32_t) * nq));
+    MIVF_CHECK_HIP(hipMemcpyAsync(x.wq.p, d_w, sizeof(int32_t) * nq, hipMemcpyDeviceToDevice, st));
+    MIVF_CHECK_HIP(hipMemcpyAsync(x.kq.p, d_k, sizeof(int32_t) * nq, hipMemcpyDeviceToDevice, st));
+    MIVF_TRY(batched_enqueue(ix, x, XCfg{ix->batched_g, ix->batched_nt}, d_q, nq, wmax, kmax, d_ids, d_dis, st));
+    MIVF_CHECK_HIP(hipMemcpyAsync(h_flags, x.ovf.as<int>() + nq, sizeof(int), hipMemcpyDeviceToHost, st));
+    MIVF_CHECK_HIP(hipMemcpyAsync(h_flags + 1, ix->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    return kOk;
+}
+
+int list_major_finish(mivf_index *ix, void *ws, const float *d_q, int nq, const int32_t *h_w, const int32_t *h_k,
+                      int kmax, int32_t *d_ids, float *d_dis, const int *h_flags, hipStream_t st, int *n_over) {
+    *n_over = 0;
+    MIVF_REQUIRE(h_flags[1] == 0, kHip, "list-major batch: kernel precondition failed (dynamic LDS not at address 0)");
+    if (h_flags[0] == 0) return kOk;
+    return batched_fixup(ix, *static_cast<XWs *>(ws), d_q, nq, h_w, h_k, kmax, d_ids, d_dis, st, n_over);
 }
 
 }  // namespace mivf

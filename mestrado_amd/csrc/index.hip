@@ -1188,6 +1188,12 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     } else if (n == "list_major_min_queries") {
         MIVF_REQUIRE(value >= 1, kInvalid, "list_major_min_queries must be >= 1");
         ix->list_major_min = value;
+    } else if (n == "serve_min_fill") {
+        MIVF_REQUIRE(value >= 0 && value <= (1 << 20), kInvalid, "serve_min_fill must be in 0..2^20");
+        ix->serve_min_fill = value;
+    } else if (n == "serve_max_wait_us") {
+        MIVF_REQUIRE(value >= 0 && value <= 10000000, kInvalid, "serve_max_wait_us must be in 0..10^7");
+        ix->serve_max_wait_us = value;
     } else if (n == "fast_capacity") {
         MIVF_REQUIRE(value >= 1024 && value <= (1 << 20), kInvalid, "fast_capacity must be in 1024..2^20");
         ix->fast_cap = int(value);

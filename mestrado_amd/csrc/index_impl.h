@@ -154,6 +154,11 @@ struct mivf_index {
     // scale (both built from FP32 tables, FP32 sums)
     int fast_tables = 0;
     int64_t list_major_min = 2048;
+    // serving: while a micro-batch is in flight, a free slot waits until at
+    // least serve_min_fill queries wait or the oldest has waited
+    // serve_max_wait_us (min_fill 0: launch whatever waits as soon as a slot
+    // is free)
+    int64_t serve_min_fill = 1024, serve_max_wait_us = 300;
     int batched_last_overflow = 0;
     int ncu = 256;            // compute units of the device
     std::shared_ptr<void> xws;  // workspace of the batched exact form (batched.hip)
@@ -191,6 +196,19 @@ int read_vecs_rows(const char *path, int elem, int64_t row0, int64_t n, int d, T
 bool list_major_ok(const mivf_index *ix, const float *d_q, int nq);
 int search_list_major(mivf_index *ix, const float *d_q, int nq, int w, int k, int32_t *d_ids, float *d_dis,
                       hipStream_t st);
+// One serving micro-batch through the list-major kernels on the caller's
+// stream and workspace (batched.hip).  list_major_enqueue leaves the
+// overflow count and the kernel error flag in pinned h_flags[0..1] when the
+// stream reaches them; list_major_finish, called once the stream has passed
+// that point, reruns overflowed queries with the plain kernels (n_over > 0:
+// rows of d_ids/d_dis were rewritten after the enqueued work).
+std::shared_ptr<void> new_list_major_ws(hipStream_t st);
+bool list_major_fits(const mivf_index *ix, int nq);
+int64_t list_major_query_bytes(const mivf_index *ix, int kmax);
+int list_major_enqueue(mivf_index *ix, void *ws, const float *d_q, int nq, int wmax, const int32_t *d_w,
+                       const int32_t *d_k, int kmax, int32_t *d_ids, float *d_dis, int *h_flags, hipStream_t st);
+int list_major_finish(mivf_index *ix, void *ws, const float *d_q, int nq, const int32_t *h_w, const int32_t *h_k,
+                      int kmax, int32_t *d_ids, float *d_dis, const int *h_flags, hipStream_t st, int *n_over);
 // Coarse top-w of nq queries into probes/dis [nq][w] (ws.dmat sized nq * nlist).
 int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_t *probes, float *dis, hipStream_t st,
                  Timer *tm);
