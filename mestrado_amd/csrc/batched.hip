@@ -27,6 +27,25 @@ struct XCfg {
 struct XWs {
     Ws ws;
     DevBuf wq, kq, ns, tau, cnt, poff, ioff, fill, pairq, items, ctrl, cand, ccnt, ovf;
+    // side stream of the pair grouping (runs beside the seed scan)
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    XWs() = default;
+    XWs(const XWs &) = delete;
+    XWs &operator=(const XWs &) = delete;
+    ~XWs() {
+        if (side) (void)hipStreamSynchronize(side);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
+    }
+    int ensure_side() {
+        if (side) return kOk;
+        MIVF_CHECK_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        MIVF_CHECK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        MIVF_CHECK_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        return kOk;
+    }
 };
 
 int capacity(const mivf_index *ix, int kmax) { return std::max(ix->batched_cap, 2 * kmax); }
@@ -47,9 +66,12 @@ std::vector<XCfg> candidates(int m) {
 }
 
 // Enqueue the batched exact search (graph-capturable once the workspace has
-// been sized by a first run).
+// been sized by a first run, with overlap off).  With overlap, the grouping
+// of the (list, query) pairs into work items runs on the workspace's side
+// stream beside the seed scan: the items do not depend on the seed bounds,
+// which the scan reads per query.
 int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, int nq, int wmax, int kmax,
-                    int32_t *d_ids, float *d_dis, hipStream_t st, Timer *tm = nullptr) {
+                    int32_t *d_ids, float *d_dis, hipStream_t st, Timer *tm = nullptr, bool overlap = false) {
     const int nlist = ix->nlist;
     const int capq = capacity(ix, kmax);
     const int64_t np = int64_t(nq) * wmax;
@@ -75,6 +97,19 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     MIVF_TRY(coarse_stage(ix, x.ws, d_q, nq, wmax, probes, x.ws.coadis.as<float>(), st, nullptr));
     if (tm) MIVF_TRY(tm->mark());
     MIVF_CHECK_HIP(launch_mask_probes(probes, nq, wmax, x.wq.as<int32_t>(), st));
+    if (overlap) MIVF_TRY(x.ensure_side());
+    hipStream_t sp = overlap ? x.side : st;  // stream of the pair grouping
+    if (overlap) {
+        MIVF_CHECK_HIP(hipEventRecord(x.fork, st));
+        MIVF_CHECK_HIP(hipStreamWaitEvent(sp, x.fork, 0));
+    }
+    MIVF_CHECK_HIP(launch_pairs(probes, nq, wmax, nullptr, ix->off.as<int64_t>(), nullptr, nlist, cfg.g,
+                                x.cnt.as<int>(), x.poff.as<int>(), x.ioff.as<int>(), ctrl, x.fill.as<int>(),
+                                x.pairq.as<int32_t>(), x.items.as<FastItem>(), sp));
+    MIVF_CHECK_HIP(launch_fill_i32(x.ccnt.as<int>(), nq, 0, sp));
+    MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, sp));
+    MIVF_CHECK_HIP(launch_fill_i32(x.ovf.as<int>(), nq + 1, 0, sp));
+    if (overlap) MIVF_CHECK_HIP(hipEventRecord(x.join, sp));
     MIVF_CHECK_HIP(launch_seed_count(probes, nq, wmax, ix->off.as<int64_t>(), nlist, kmax, x.kq.as<int32_t>(), 1,
                                      x.ns.as<int32_t>(), st));
     SeedArgs sa{};
@@ -97,12 +132,7 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     // candidates, and so the select, small
     sa.tau = x.tau.as<float>();
     MIVF_CHECK_HIP(launch_seed_tau(sa, ix->m, st));
-    MIVF_CHECK_HIP(launch_pairs(probes, nq, wmax, nullptr, ix->off.as<int64_t>(), x.tau.as<float>(), nlist, cfg.g,
-                                x.cnt.as<int>(), x.poff.as<int>(), x.ioff.as<int>(), ctrl, x.fill.as<int>(),
-                                x.pairq.as<int32_t>(), x.items.as<FastItem>(), st));
-    MIVF_CHECK_HIP(launch_fill_i32(x.ccnt.as<int>(), nq, 0, st));
-    MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, st));
-    MIVF_CHECK_HIP(launch_fill_i32(x.ovf.as<int>(), nq + 1, 0, st));
+    if (overlap) MIVF_CHECK_HIP(hipStreamWaitEvent(st, x.join, 0));
     ExactArgs a{};
     a.queries = d_q;
     a.coarse = ix->coarse.as<float>();
@@ -118,6 +148,7 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     a.capq = capq;
     a.ks = ix->ks;
     a.err = ix->errflag.as<int>();
+    a.tau = x.tau.as<float>();
     const int per_cu = std::max(1, int((160 * 1024) / exact_lds_bytes(ix->m, cfg.g)));
     if (tm) MIVF_TRY(tm->mark());
     MIVF_CHECK_HIP(launch_exact_scan(a, ix->m, cfg.g, cfg.nt, ix->ncu * per_cu, st));
@@ -258,7 +289,7 @@ int search_list_major(mivf_index *ix, const float *d_q, int nq, int w, int k, in
     Timer tm;
     std::fill(ix->timings, ix->timings + 8, 0.0f);
     MIVF_TRY(tm.start(ix->profile, st, ix->ev));
-    MIVF_TRY(batched_enqueue(ix, x, cfg, d_q, nq, w, k, d_ids, d_dis, st, tm.on ? &tm : nullptr));
+    MIVF_TRY(batched_enqueue(ix, x, cfg, d_q, nq, w, k, d_ids, d_dis, st, tm.on ? &tm : nullptr, true));
     if (tm.on) {
         MIVF_CHECK_HIP(hipEventSynchronize(tm.ev[tm.n - 1]));
         for (int i = 0; i + 1 < tm.n && i < 8; ++i) {
@@ -302,7 +333,8 @@ int list_major_enqueue(mivf_index *ix, void *ws, const float *d_q, int nq, int w
     MIVF_TRY(x.kq.ensure(sizeof(int32_t) * nq));
     MIVF_CHECK_HIP(hipMemcpyAsync(x.wq.p, d_w, sizeof(int32_t) * nq, hipMemcpyDeviceToDevice, st));
     MIVF_CHECK_HIP(hipMemcpyAsync(x.kq.p, d_k, sizeof(int32_t) * nq, hipMemcpyDeviceToDevice, st));
-    MIVF_TRY(batched_enqueue(ix, x, XCfg{ix->batched_g, ix->batched_nt}, d_q, nq, wmax, kmax, d_ids, d_dis, st));
+    MIVF_TRY(batched_enqueue(ix, x, XCfg{ix->batched_g, ix->batched_nt}, d_q, nq, wmax, kmax, d_ids, d_dis, st,
+                             nullptr, true));
     MIVF_CHECK_HIP(hipMemcpyAsync(h_flags, x.ovf.as<int>() + nq, sizeof(int), hipMemcpyDeviceToHost, st));
     MIVF_CHECK_HIP(hipMemcpyAsync(h_flags + 1, ix->errflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
     return kOk;
@@ -370,7 +402,7 @@ int mivf_search_batched(mivf_index *ix, const float *d_q, int nq, const int32_t 
     Timer tm;
     std::fill(ix->timings, ix->timings + 8, 0.0f);
     MIVF_TRY(tm.start(ix->profile, st, ix->ev));
-    MIVF_TRY(batched_enqueue(ix, x, cfg, d_q, nq, wmax, kmax, d_ids, d_dis, st, tm.on ? &tm : nullptr));
+    MIVF_TRY(batched_enqueue(ix, x, cfg, d_q, nq, wmax, kmax, d_ids, d_dis, st, tm.on ? &tm : nullptr, true));
     if (tm.on) {
         MIVF_CHECK_HIP(hipEventSynchronize(tm.ev[tm.n - 1]));
         for (int i = 0; i + 1 < tm.n && i < 8; ++i) {

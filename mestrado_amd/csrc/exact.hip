@@ -221,7 +221,7 @@ __global__ __launch_bounds__(NT, (xwaves<M, G, NT>())) void exact_scan_kernel(Ex
         if (tid < 16) {
             const int q = tid < G ? rec->q[tid] : -1;
             qid[tid] = q;
-            lim[tid] = q >= 0 ? rec->tau[tid] : -__builtin_huge_valf();
+            lim[tid] = q >= 0 ? (a.tau ? a.tau[q] : rec->tau[tid]) : -__builtin_huge_valf();
         }
         if (tid < 3) pcnt[tid] = 0;
         __syncthreads();

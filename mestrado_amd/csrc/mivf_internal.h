@@ -241,6 +241,7 @@ struct ExactArgs {
     int capq;
     int ks;
     int *err;
+    const float *tau;           // [nq] per-query bounds (nullptr: the items' own)
 };
 size_t exact_lds_bytes(int m, int g);
 // Compiled (m, g, threads): (16,4,256) (16,8,256) (16,8,512) (8,4,256) (8,8,256) (8,8,512) (8,16,512).
