@@ -128,8 +128,9 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     sa.nlist = nlist;
     sa.ks = ix->ks;
     sa.kl = kmax;
-    // the whole seed lists: a tighter bound than a sample of them keeps the
-    // candidates, and so the select, small
+    // by default the whole seed lists: a tighter bound than a sample of them
+    // keeps the candidates, and so the select, small
+    sa.cap = ix->exact_seed_cap > 0 ? std::max(ix->exact_seed_cap, kmax) : 0;
     sa.tau = x.tau.as<float>();
     MIVF_CHECK_HIP(launch_seed_tau(sa, ix->m, st));
     if (overlap) MIVF_CHECK_HIP(hipStreamWaitEvent(st, x.join, 0));

@@ -421,6 +421,110 @@ __global__ __launch_bounds__(NTC) void coarse_select_wave_kernel(const float *__
     }
 }
 
+// The same per wavefront for any nlist up to the matrix-core limit and w up to
+// 64 GPL: the approximate distances stream from HBM twice instead of sitting
+// in registers.  Lane l owns the GPL groups of centroids {64 (GPL i + j) + l}
+// (j < GPL); the w-th smallest of the 64 GPL group minima bounds the w-th
+// smallest D' from above (the minima are distinct centroids), found by rank
+// counting.  The second pass compacts the recheck set {D' <= V + 2 M_q} by
+// ballots; its exact distances are ranked as in the register kernel.  A
+// recheck set above SCAP_W (or short of w) sends the query to `redo`.
+constexpr int SCAP_W = 512;
+
+template <int GPL>
+__global__ __launch_bounds__(NTC) void coarse_select_stream_kernel(const float *__restrict__ coarse,
+                                                                   const float *__restrict__ cn, float maxcn,
+                                                                   int nlist, const float *__restrict__ queries,
+                                                                   const double *__restrict__ qn, int nq,
+                                                                   const float *__restrict__ approx, int64_t lda,
+                                                                   int w, int32_t *__restrict__ probes,
+                                                                   float *__restrict__ pdis,
+                                                                   int32_t *__restrict__ redo,
+                                                                   int *__restrict__ redo_cnt) {
+    __shared__ uint32_t cidx_all[NTC / 64][SCAP_W];
+    __shared__ uint64_t keys_all[NTC / 64][SCAP_W];
+    __shared__ float qv_all[NTC / 64][CD];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t q = int64_t(blockIdx.x) * (NTC / 64) + wv;
+    if (q >= nq) return;  // wave-uniform; no workgroup barriers below
+    uint32_t *cidx = cidx_all[wv];
+    uint64_t *keys = keys_all[wv];
+    float *qv = qv_all[wv];
+    const double bn = qn[q];
+    qv[lane] = queries[q * CD + lane];
+    qv[lane + 64] = queries[q * CD + lane + 64];
+    const float *row = approx + q * lda;
+    const int nblk = (nlist + 63) / 64;  // blocks of 64 centroids; block b belongs to group set b % GPL
+    float gm[GPL];
+#pragma unroll
+    for (int j = 0; j < GPL; ++j) gm[j] = __builtin_huge_valf();
+    for (int b0 = 0; b0 < nblk; b0 += GPL) {
+#pragma unroll
+        for (int j = 0; j < GPL; ++j) {
+            const int c = (b0 + j) * 64 + lane;
+            if (c < nlist) gm[j] = fminf(gm[j], row[c]);
+        }
+    }
+    // the w-th smallest of the 64 GPL minima: the value x with
+    // #(< x) < w <= #(<= x) (ties counted once per equal value)
+    float vw = __builtin_huge_valf();
+    {
+        int lt[GPL], le[GPL];
+#pragma unroll
+        for (int j = 0; j < GPL; ++j) lt[j] = le[j] = 0;
+        for (int src = 0; src < 64; ++src) {
+#pragma unroll
+            for (int jj = 0; jj < GPL; ++jj) {
+                const float y = __shfl(gm[jj], src);
+#pragma unroll
+                for (int j = 0; j < GPL; ++j) {
+                    lt[j] += y < gm[j];
+                    le[j] += y <= gm[j];
+                }
+            }
+        }
+        bool mine = false;
+        float x = 0.0f;
+#pragma unroll
+        for (int j = 0; j < GPL; ++j)
+            if (lt[j] < w && w <= le[j]) {
+                mine = true;
+                x = gm[j];
+            }
+        const uint64_t bal = __ballot(mine);
+        if (bal != 0) vw = __shfl(x, int(__builtin_ctzll(bal)));
+    }
+    const float lim = vw + 3.0e-4f * float(bn + double(maxcn));
+    int n = 0;
+    for (int b = 0; b < nblk; ++b) {
+        const int c = b * 64 + lane;
+        const bool pass = c < nlist && row[c] <= lim;  // a non-finite limit passes nothing it should not
+        const uint64_t bal = __ballot(pass);
+        const int pos = n + int(__builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u)));
+        if (pass && pos < SCAP_W) cidx[pos] = uint32_t(c);
+        n += __popcll(bal);
+    }
+    if (n > SCAP_W || n < w || !(vw < __builtin_huge_valf())) {
+        if (lane == 0) redo[atomicAdd(redo_cnt, 1)] = int32_t(q);
+        return;
+    }
+    wave_sync();
+    for (int j = lane; j < n; j += 64) {
+        const int c = int(cidx[j]);
+        keys[j] = ckey(exact_dist(coarse + int64_t(c) * CD, qv, cn[c], bn), c);
+    }
+    wave_sync();
+    for (int j = lane; j < n; j += 64) {
+        const uint64_t k = keys[j];
+        int rank = 0;
+        for (int i = 0; i < n; ++i) rank += keys[i] < k;
+        if (rank < w) {
+            probes[q * w + rank] = key_id(k);
+            pdis[q * w + rank] = key_dist(k);
+        }
+    }
+}
+
 }  // namespace
 
 // The workgroup select holds a query's nlist approximate distances in LDS
@@ -463,11 +567,19 @@ hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxc
         if (e != hipSuccess) return e;
         lds_set = lds;
     }
-    if (nlist >= 64 && nlist <= 64 * NPL && w <= 64 && redo && redo_cnt) {
+    const unsigned wgrid = unsigned((nq + NTC / 64 - 1) / (NTC / 64));
+    if (nlist >= 64 && w <= 128 && redo && redo_cnt) {
         hipError_t e = launch_fill_i32(redo_cnt, 1, 0, s);  // a kernel, not a memset: graph-capturable
         if (e != hipSuccess) return e;
-        coarse_select_wave_kernel<<<unsigned((nq + NTC / 64 - 1) / (NTC / 64)), NTC, 0, s>>>(
-            coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda, w, probes, pdis, redo, redo_cnt);
+        if (nlist <= 64 * NPL && w <= 64)
+            coarse_select_wave_kernel<<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda,
+                                                            w, probes, pdis, redo, redo_cnt);
+        else if (w <= 64)
+            coarse_select_stream_kernel<1><<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx,
+                                                                 lda, w, probes, pdis, redo, redo_cnt);
+        else
+            coarse_select_stream_kernel<2><<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx,
+                                                                 lda, w, probes, pdis, redo, redo_cnt);
         coarse_select_kernel<<<unsigned(nq), NTC, coarse_select_lds(nlist), s>>>(
             coarse, cn, maxcn, nlist, queries, qn, approx, lda, w, probes, pdis, redo, redo_cnt);
     } else {

@@ -1160,6 +1160,9 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     } else if (n == "fast_seed_cap") {
         MIVF_REQUIRE(value == 0 || (value >= 1 && value <= 8192), kInvalid, "fast_seed_cap must be 0 or in 1..8192");
         ix->fast_seed_cap = int(value);
+    } else if (n == "exact_seed_cap") {
+        MIVF_REQUIRE(value == 0 || (value >= 1 && value <= 8192), kInvalid, "exact_seed_cap must be 0 or in 1..8192");
+        ix->exact_seed_cap = int(value);
     } else if (n == "fast_running_bound") {
         ix->fast_running = value != 0;
     } else if (n == "query_order") {

@@ -49,10 +49,27 @@ def test_generator_bitexact(mivf, orc):
 
 def test_coarse_topw_bitexact(mivf, orc, c1, c1_index):
     q = c1["gold_q"]
-    for w in (1, 8, 64, 256):
+    for w in (1, 8, 64, 65, 100, 128, 256):
         idx, dis = c1_index.coarse(c1["query"], w)
         ridx, rdis = orc.knn(q["coarse"], c1["query"], w)
         assert_same_results(idx, dis, ridx, rdis, f"coarse w={w}")
+
+
+def test_coarse_many_ties_large_nlist(mivf, orc, c1):
+    """6000 lists made of 250 distinct centroids, each repeated 24 times:
+    the streaming wavefront select (nlist > 4096) meets recheck sets of tied
+    centroids, some above its capacity (those queries go to the workgroup
+    select); the probes are the oracle's, ties broken by lower index."""
+    q = c1["gold_q"]
+    coarse = np.ascontiguousarray(np.repeat(q["coarse"][:250], 24, axis=0)[:6000])
+    ix = mivf.Index(coarse, q["pq"])
+    try:
+        for w in (16, 64, 100, 128):
+            idx, dis = ix.coarse(c1["query"], w)
+            ridx, rdis = orc.knn(coarse, c1["query"], w)
+            assert_same_results(idx, dis, ridx, rdis, f"coarse nlist=6000 (ties) w={w}")
+    finally:
+        ix.close()
 
 
 def test_encode_bitexact(mivf, orc, c1, c1_index):
@@ -383,7 +400,7 @@ def test_many_lists_coarse_and_search(orc, wide, mfma):
     coarse = index[0]
     ix.set_option("coarse_mfma", mfma)
     try:
-        for w in (1, 8, 64, 300):
+        for w in (1, 8, 64, 100, 128, 300):
             idx, dis = ix.coarse(query, w)
             ridx, rdis = orc.knn(coarse, query, w)
             assert_same_results(idx, dis, ridx, rdis, f"coarse nlist=9000 w={w} mfma={mfma}")
