@@ -422,13 +422,16 @@ __global__ __launch_bounds__(NTC) void coarse_select_wave_kernel(const float *__
 }
 
 // The same per wavefront for any nlist up to the matrix-core limit and w up to
-// 64 GPL: the approximate distances stream from HBM twice instead of sitting
-// in registers.  Lane l owns the GPL groups of centroids {64 (GPL i + j) + l}
-// (j < GPL); the w-th smallest of the 64 GPL group minima bounds the w-th
-// smallest D' from above (the minima are distinct centroids), found by rank
-// counting.  The second pass compacts the recheck set {D' <= V + 2 M_q} by
-// ballots; its exact distances are ranked as in the register kernel.  A
-// recheck set above SCAP_W (or short of w) sends the query to `redo`.
+// 128: the approximate distances stream from HBM twice instead of sitting in
+// registers.  Lane l owns the GPL groups of centroids {64 (GPL i + j) + l}
+// (j < GPL; GPL >= w / 16, so the bound sits near the quarter quantile of
+// the group minima rather than at their maximum).  The w-th smallest of the
+// 64 GPL group minima bounds the w-th smallest D' from above (the minima are
+// distinct centroids); it is found by a bitwise radix search over ballot
+// counts of the minima's order keys.  The second pass compacts the recheck
+// set {D' <= V + 2 M_q} by ballots; its exact distances are ranked as in the
+// register kernel.  A recheck set above SCAP_W (or short of w) sends the
+// query to `redo`.
 constexpr int SCAP_W = 512;
 
 template <int GPL>
@@ -455,54 +458,59 @@ __global__ __launch_bounds__(NTC) void coarse_select_stream_kernel(const float *
     qv[lane + 64] = queries[q * CD + lane + 64];
     const float *row = approx + q * lda;
     const int nblk = (nlist + 63) / 64;  // blocks of 64 centroids; block b belongs to group set b % GPL
+    constexpr int UL = 16;               // blocks loaded together (loads in flight per lane)
+    static_assert(UL % GPL == 0, "a load batch covers whole group sets");
     float gm[GPL];
 #pragma unroll
     for (int j = 0; j < GPL; ++j) gm[j] = __builtin_huge_valf();
-    for (int b0 = 0; b0 < nblk; b0 += GPL) {
+    for (int b0 = 0; b0 < nblk; b0 += UL) {
+        float v[UL];
 #pragma unroll
-        for (int j = 0; j < GPL; ++j) {
-            const int c = (b0 + j) * 64 + lane;
-            if (c < nlist) gm[j] = fminf(gm[j], row[c]);
+        for (int u = 0; u < UL; ++u) {
+            const int c = (b0 + u) * 64 + lane;
+            v[u] = c < nlist ? row[c] : __builtin_huge_valf();
+        }
+#pragma unroll
+        for (int u = 0; u < UL; ++u) gm[u % GPL] = fminf(gm[u % GPL], v[u]);
+    }
+    // the w-th smallest group minimum: MSB-first search for the smallest key
+    // with at least w minima at or below it (order keys: integer order =
+    // float order)
+    uint32_t gk[GPL];
+#pragma unroll
+    for (int j = 0; j < GPL; ++j) gk[j] = fkey(gm[j]);
+    uint32_t prefix = 0u;
+    int need = w;
+    for (int bit = 31; bit >= 0; --bit) {
+        // minima that agree with prefix above `bit` and have 0 at `bit`
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < GPL; ++j) cnt += __popcll(__ballot(((gk[j] ^ prefix) >> bit) == 0u));
+        if (cnt < need) {
+            need -= cnt;
+            prefix |= 1u << bit;
         }
     }
-    // the w-th smallest of the 64 GPL minima: the value x with
-    // #(< x) < w <= #(<= x) (ties counted once per equal value)
-    float vw = __builtin_huge_valf();
-    {
-        int lt[GPL], le[GPL];
-#pragma unroll
-        for (int j = 0; j < GPL; ++j) lt[j] = le[j] = 0;
-        for (int src = 0; src < 64; ++src) {
-#pragma unroll
-            for (int jj = 0; jj < GPL; ++jj) {
-                const float y = __shfl(gm[jj], src);
-#pragma unroll
-                for (int j = 0; j < GPL; ++j) {
-                    lt[j] += y < gm[j];
-                    le[j] += y <= gm[j];
-                }
-            }
-        }
-        bool mine = false;
-        float x = 0.0f;
-#pragma unroll
-        for (int j = 0; j < GPL; ++j)
-            if (lt[j] < w && w <= le[j]) {
-                mine = true;
-                x = gm[j];
-            }
-        const uint64_t bal = __ballot(mine);
-        if (bal != 0) vw = __shfl(x, int(__builtin_ctzll(bal)));
-    }
+    const float vw = key_dist(uint64_t(prefix) << 32);
     const float lim = vw + 3.0e-4f * float(bn + double(maxcn));
     int n = 0;
-    for (int b = 0; b < nblk; ++b) {
-        const int c = b * 64 + lane;
-        const bool pass = c < nlist && row[c] <= lim;  // a non-finite limit passes nothing it should not
-        const uint64_t bal = __ballot(pass);
-        const int pos = n + int(__builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u)));
-        if (pass && pos < SCAP_W) cidx[pos] = uint32_t(c);
-        n += __popcll(bal);
+    for (int b0 = 0; b0 < nblk; b0 += UL) {
+        float v[UL];
+#pragma unroll
+        for (int u = 0; u < UL; ++u) {
+            const int c = (b0 + u) * 64 + lane;
+            v[u] = c < nlist ? row[c] : __builtin_huge_valf();
+        }
+#pragma unroll
+        for (int u = 0; u < UL; ++u) {
+            const int c = (b0 + u) * 64 + lane;
+            const bool pass = c < nlist && v[u] <= lim;
+            const uint64_t bal = __ballot(pass);
+            const int pos = n + int(__builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u)));
+            if (pass && pos < SCAP_W) cidx[pos] = uint32_t(c);
+            n += __popcll(bal);
+        }
     }
     if (n > SCAP_W || n < w || !(vw < __builtin_huge_valf())) {
         if (lane == 0) redo[atomicAdd(redo_cnt, 1)] = int32_t(q);
@@ -571,15 +579,21 @@ hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxc
     if (nlist >= 64 && w <= 128 && redo && redo_cnt) {
         hipError_t e = launch_fill_i32(redo_cnt, 1, 0, s);  // a kernel, not a memset: graph-capturable
         if (e != hipSuccess) return e;
-        if (nlist <= 64 * NPL && w <= 64)
+#define MIVF_STREAM_SELECT(G)                                                                                   \
+    coarse_select_stream_kernel<G><<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda, w, \
+                                                         probes, pdis, redo, redo_cnt)
+        if (nlist <= 64 * NPL && w <= 32)
             coarse_select_wave_kernel<<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda,
                                                             w, probes, pdis, redo, redo_cnt);
+        else if (w <= 16)
+            MIVF_STREAM_SELECT(1);
+        else if (w <= 32)
+            MIVF_STREAM_SELECT(2);
         else if (w <= 64)
-            coarse_select_stream_kernel<1><<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx,
-                                                                 lda, w, probes, pdis, redo, redo_cnt);
+            MIVF_STREAM_SELECT(4);
         else
-            coarse_select_stream_kernel<2><<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx,
-                                                                 lda, w, probes, pdis, redo, redo_cnt);
+            MIVF_STREAM_SELECT(8);
+#undef MIVF_STREAM_SELECT
         coarse_select_kernel<<<unsigned(nq), NTC, coarse_select_lds(nlist), s>>>(
             coarse, cn, maxcn, nlist, queries, qn, approx, lda, w, probes, pdis, redo, redo_cnt);
     } else {
