@@ -648,15 +648,28 @@ def streamed_serving(args, cfg):
         M.synchronize()
         return (time.perf_counter() - t) / args.steps
 
-    t_plain = timed(lambda: ix.search(query, w, k, out=(oi, od)))
-    ix.set_option("profile", 1)
-    st = np.zeros(8)
-    for _ in range(args.steps):
-        ix.search(query, w, k, out=(oi, od))
-        st += ix.timings()
-    ix.set_option("profile", 0)
-    st /= args.steps
-    log(f"plain batch: {nq / t_plain:.0f} q/s, stages {np.round(st[:4], 4)}")
+    def stages(fn):
+        ix.set_option("profile", 1)
+        acc = np.zeros(8)
+        for _ in range(args.steps):
+            fn()
+            acc += ix.timings()
+        ix.set_option("profile", 0)
+        return acc / args.steps
+
+    plain_fn = lambda: ix.search(query, w, k, out=(oi, od))  # noqa: E731
+    t_plain = timed(plain_fn)
+    st = stages(plain_fn)
+    log(f"plain batch: {nq / t_plain:.0f} q/s (list-major), stages {np.round(st[:4], 4)}")
+    pi = M.DeviceArray((nq, k), np.int32)
+    pd = M.DeviceArray((nq, k), np.float32)
+    ix.set_option("list_major", 0)
+    pq_fn = lambda: ix.search(query, w, k, out=(pi, pd))  # noqa: E731
+    t_pq = timed(pq_fn)
+    st_pq = stages(pq_fn)
+    ix.set_option("list_major", 1)
+    pq_same = same_bits(oi.numpy(), od.numpy(), pi.numpy(), pd.numpy())
+    log(f"per-query kernels: {nq / t_pq:.0f} q/s, stages {np.round(st_pq[:4], 4)}, same {pq_same}")
 
     # recall of 1000 queries against the exact FP64 nearest neighbour over the
     # whole base, regenerated chunk by chunk (ties -> lowest id)
@@ -690,7 +703,8 @@ def streamed_serving(args, cfg):
         pts.append({"load": frac, "offered_qps": round(sb["offered_qps"], 1), "qps": round(sb["qps"], 1),
                     "p50_ms": round(sb["p50_ms"], 3), "p99_ms": round(sb["p99_ms"], 3),
                     "mean_ms": round(sb["mean_ms"], 3), "batches": int(sb["batches"]),
-                    "list_major_batches": int(sb["list_major_batches"])})
+                    "list_major_batches": int(sb["list_major_batches"]),
+                    "rerun_queries": int(sb["rerun_queries"])})
         log(f"serving at {frac:.2f} of saturation: {pts[-1]}")
     res = {
         "metric": "QPS at fixed recall@100 (10M x128 IVF4096,PQ16); ADC-scan HBM GB/s; p99 ms",
@@ -699,11 +713,15 @@ def streamed_serving(args, cfg):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded Gaussian mixture, integer-valued float32), streamed in chunks",
         "config": {"workload": args.workload, "nbase": nbase, "nlist": nlist, "m": m, "nquery": nq, "w": w, "k": k,
-                   "form": "plain exact FP32 (per-query kernels: more lists than the list-major kernels take)",
+                   "form": "plain exact FP32 (mivf_search: list-major kernels)",
                    "stream_chunk": chunk, "build_s": round(t_build, 1),
                    "kmeans_iterations": {"coarse": cfg["niter_coarse"], "pq": cfg["niter_pq"]},
                    "recall_queries": ng, **rec},
-        "stages_ms": {n: round(float(v), 4) for n, v in zip(("coarse", "select", "scan", "merge"), st)},
+        "stages_ms": {n: round(float(v), 4) for n, v in zip(("coarse", "seed_grouping", "scan", "select"), st)},
+        "per_query_kernels": {"qps": round(nq / t_pq, 1), "ms_per_step": round(t_pq * 1000, 4),
+                              "stages_ms": {n: round(float(v), 4)
+                                            for n, v in zip(("coarse", "select", "scan", "merge"), st_pq)},
+                              "bit_identical_to_list_major": pq_same},
         "serving": {"form": "plain exact FP32 (per-request w, k)", "w_choices": list(cfg["serve_w"]),
                     "k_choices": list(cfg["serve_k"]), "slots": args.serve_slots,
                     "max_batch": args.serve_max_batch,

@@ -220,7 +220,9 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * event timings of the next searches), "query_order" (1 = scan batches of
  * >= 2048 queries grouped by nearest list; schedule only, default 1),
  * "fast_seed_lists" (lists per query scanned exactly to seed
- * mivf_search_fast, default 1), "fast_capacity", "fast_running_bound" (1 =
+ * mivf_search_fast, default 1), "fast_seed_cap" (vectors of those lists
+ * that enter the seed bound, default 1024, 0 = all), "fast_capacity",
+ * "fast_running_bound" (1 =
  * the 8-bit scan tightens each query's filter as its candidates accumulate;
  * results unchanged, default 1), "batched_group", "batched_threads",
  * "batched_capacity" (mivf_search_batched), "list_major" (1 = mivf_search
@@ -232,8 +234,11 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * summed in FP32), "list_major_min_queries" (default 2048), "coarse_mfma"
  * (1 = coarse top-w through BF16 matrix-core distances and the exact
  * recheck, for d = 128 and nlist <= 32768; 0 = exact FP32 distance tiles for
- * every shape; the same probes either way), "serve_min_fill" and
- * "serve_max_wait_us" (servers over this index: while a micro-batch is in
+ * every shape; the same probes either way), "exact_seed_cap" (vectors of
+ * the nearest lists that set each query's bound in the list-major exact
+ * kernels: default 4096, 0 = the whole lists; results unchanged),
+ * "serve_min_fill" and "serve_max_wait_us" (servers over this index: while
+ * a micro-batch is in
  * flight, a free slot launches only once serve_min_fill queries wait or the
  * oldest waiting request has waited serve_max_wait_us; defaults 1024 and
  * 300; serve_min_fill 0 launches as soon as a slot is free).               */
@@ -293,7 +298,7 @@ int mivf_plan_free(mivf_plan *p);
  * requests up to max_batch queries and within mem_budget bytes of per-request
  * device state (0 = 4 GiB).  Searches are exact: a micro-batch of at least
  * the index's list_major_min_queries queries runs the list-major kernels of
- * mivf_search_batched (m in {8, 16}, nlist <= 8192), a smaller one the
+ * mivf_search_batched (m in {8, 16}), a smaller one the
  * per-query kernels of mivf_search_var; both give the same rows, so a
  * request's results are the same alone and in any mix.  The server is driven
  * by the calling thread: submit, then progress / wait / drain.                */

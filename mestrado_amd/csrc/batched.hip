@@ -128,8 +128,8 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     sa.nlist = nlist;
     sa.ks = ix->ks;
     sa.kl = kmax;
-    // by default the whole seed lists: a tighter bound than a sample of them
-    // keeps the candidates, and so the select, small
+    // the first exact_seed_cap vectors of the seed lists (a prefix stays in
+    // LDS in one pass; a longer list costs three passes of table and ADC)
     sa.cap = ix->exact_seed_cap > 0 ? std::max(ix->exact_seed_cap, kmax) : 0;
     sa.tau = x.tau.as<float>();
     MIVF_CHECK_HIP(launch_seed_tau(sa, ix->m, st));
@@ -209,7 +209,7 @@ int batched_fixup(mivf_index *ix, XWs &x, const float *d_q, int nq, const int32_
 
 int check_wk(const mivf_index *ix, int nq, const int32_t *w, const int32_t *k, int kmax, int *wmax) {
     MIVF_REQUIRE(ix->m == 8 || ix->m == 16, kUnsupported, "batched search: compiled for m in {8, 16}");
-    MIVF_REQUIRE(ix->nlist <= 8192, kUnsupported, "batched search: compiled for nlist <= 8192");
+    MIVF_REQUIRE(ix->nlist <= kMaxPairLists, kUnsupported, "batched search: nlist above 2^20");
     MIVF_REQUIRE(kmax >= 1 && kmax <= 1024, kInvalid, "batched search: kmax must be in 1..1024");
     *wmax = 1;
     for (int i = 0; i < nq; ++i) {
@@ -267,7 +267,7 @@ XWs &index_xws(mivf_index *ix);
 }  // namespace
 
 bool list_major_fits(const mivf_index *ix, int nq) {
-    return ix->list_major && nq >= ix->list_major_min && (ix->m == 8 || ix->m == 16) && ix->nlist <= 8192 &&
+    return ix->list_major && nq >= ix->list_major_min && (ix->m == 8 || ix->m == 16) && ix->nlist <= kMaxPairLists &&
            cfg_ok(ix->m, XCfg{ix->batched_g, ix->batched_nt});
 }
 

@@ -607,7 +607,11 @@ __global__ void fill_i32_kernel(int *__restrict__ p, int n, int v) {
 // ---- list-major grouping of (query, probe) pairs with rank >= rank0[q].
 // Each workgroup aggregates its PCH pairs in an LDS histogram over the lists
 // (nlist <= LBINS) and issues one global atomic per list it touches, so
-// popular lists do not serialise thousands of atomics.
+// popular lists do not serialise thousands of atomics.  With more lists the
+// pairs go straight to global atomics (pair_*_global_kernel); the order of a
+// list's queries then follows the atomics, which changes how queries share
+// work items but no result (every distance, bound and selection key is per
+// query).
 constexpr int LBINS = 8192;
 constexpr int PCH = 2048;
 
@@ -634,6 +638,24 @@ __global__ __launch_bounds__(256) void pair_count_kernel(const int32_t *__restri
     __syncthreads();
     for (int i = tid; i < nlist; i += 256)
         if (hc[i]) atomicAdd(cnt + i, hc[i]);
+}
+
+__global__ __launch_bounds__(256) void pair_count_global_kernel(const int32_t *__restrict__ probes, int nq, int w,
+                                                                const int32_t *__restrict__ rank0, int nlist,
+                                                                int *__restrict__ cnt) {
+    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    int l;
+    if (e < int64_t(nq) * w && pair_ok(probes, e, w, rank0, nlist, &l)) atomicAdd(cnt + l, 1);
+}
+
+__global__ __launch_bounds__(256) void pair_scatter_global_kernel(const int32_t *__restrict__ probes, int nq, int w,
+                                                                  const int32_t *__restrict__ rank0, int nlist,
+                                                                  const int *__restrict__ pair_off,
+                                                                  int *__restrict__ fill, int32_t *__restrict__ pair_q) {
+    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    int l;
+    if (e < int64_t(nq) * w && pair_ok(probes, e, w, rank0, nlist, &l))
+        pair_q[pair_off[l] + atomicAdd(fill + l, 1)] = int32_t(e / w);
 }
 
 // exclusive scans of pair counts and of work items per list (one workgroup)
@@ -1183,12 +1205,18 @@ hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t s) {
 hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *rank0, const int64_t *list_off,
                         const float *tau, int nlist, int G, int *cnt, int *pair_off, int *item_off, int *total_items,
                         int *fill, int32_t *pair_q, FastItem *items, hipStream_t s) {
-    if (nlist > LBINS || G < 1 || G > FG) return hipErrorInvalidValue;
+    if (nlist < 1 || nlist > kMaxPairLists || G < 1 || G > FG) return hipErrorInvalidValue;
     const int64_t np = int64_t(nq) * w;
     fill_i32_kernel<<<nblk(nlist, 256), 256, 0, s>>>(cnt, nlist, 0);
-    pair_count_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rank0, nlist, cnt);
+    if (nlist <= LBINS)
+        pair_count_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rank0, nlist, cnt);
+    else
+        pair_count_global_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rank0, nlist, cnt);
     pair_scan_kernel<<<1, 1024, 0, s>>>(cnt, nlist, G, pair_off, item_off, total_items, fill);
-    pair_scatter_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rank0, nlist, pair_off, fill, pair_q);
+    if (nlist <= LBINS)
+        pair_scatter_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rank0, nlist, pair_off, fill, pair_q);
+    else
+        pair_scatter_global_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rank0, nlist, pair_off, fill, pair_q);
     const int64_t max_slots = (np / G + nlist) * FG;  // >= FG * total items
     item_expand_kernel<<<nblk(max_slots, 256), 256, 0, s>>>(pair_off, item_off, list_off, pair_q, tau, nlist, G,
                                                           total_items, max_slots, items);

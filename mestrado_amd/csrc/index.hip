@@ -891,7 +891,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
                      int64_t nbase, int32_t *d_ids, float *d_dis, void *stream) try {
     MIVF_TRY(check_index(ix));
     MIVF_REQUIRE(ix->m == 8 || ix->m == 16, kUnsupported, "search_fast: compiled for m in {8, 16}");
-    MIVF_REQUIRE(ix->nlist <= 8192, kUnsupported, "search_fast: compiled for nlist <= 8192");
+    MIVF_REQUIRE(ix->nlist <= kMaxPairLists, kUnsupported, "search_fast: nlist above 2^20");
     MIVF_REQUIRE(w >= 1 && w <= ix->nlist && w <= 1024, kInvalid, "search_fast: w must be in 1..min(nlist,1024)");
     MIVF_REQUIRE(kl >= 1 && kl <= 1024 && k >= 1 && k <= kl, kInvalid, "search_fast: need 1 <= k <= kl <= 1024");
     if (nq <= 0) return kOk;

@@ -139,7 +139,7 @@ struct mivf_index {
     bool fast_running = true;  // running per-query bound in the reduced-precision scan
     int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
     int fast_seed_cap = 1024; // vectors entering the seed bound (0 = all of the seed lists)
-    int exact_seed_cap = 0;   // the same for the list-major exact kernels
+    int exact_seed_cap = 4096; // the same for the list-major exact kernels
     int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
     int batched_cap = 16384;  // candidate capacity per query of the batched exact form (>= 2 kmax)
     int batched_g = 4, batched_nt = 512;  // its work-item shape (queries per item, threads)

@@ -182,6 +182,9 @@ hipError_t launch_merge(const uint64_t *partial, int nq, int ngroups, int kstrid
 // 8-bit tables, candidates appended to per-query buffers).
 // Work item of the reduced-precision scan: list l with gc <= 16 of the
 // queries that probe it (q[i] = -1 past gc) and their seed thresholds.
+// Lists the list-major grouping takes (LDS histograms up to 8192, global
+// atomics above).
+constexpr int kMaxPairLists = 1 << 20;
 struct FastItem {
     int32_t l, gc, len, pad;
     int64_t beg, pad2;

@@ -378,7 +378,7 @@ def test_search_other_m(mivf, orc, c1, m):
 
 @pytest.fixture(scope="module")
 def wide(mivf, orc):
-    """An index with more lists than the 8192 of the list-major kernels:
+    """An index with more lists than the grouping's LDS histograms hold (8192):
     9000 lists (not a power of two), m = 16, 120k base vectors."""
     seed = 107
     cent = orc.gen_centres(seed, NCOMP, D)
@@ -389,6 +389,26 @@ def wide(mivf, orc):
     ix = mivf.Index(coarse, pq)
     ix.add(base)
     return ix, (coarse, pq) + tuple(ix.get_lists()), query
+
+
+def test_many_lists_list_major(orc, wide):
+    """nlist = 9000: the list-major kernels (pairs grouped through global
+    atomics) give the oracle's results, uniform and per-query w and k."""
+    ix, index, query = wide
+    ix.set_option("list_major_min_queries", 1)
+    try:
+        for w, k in ((8, 100), (64, 10)):
+            ids, dis = ix.search(query, w, k)
+            rids, rdis = orc.search(index, query, w, k)
+            assert_same_results(ids, dis, rids, rdis, f"list-major nlist=9000 w={w} k={k}")
+    finally:
+        ix.set_option("list_major_min_queries", 2048)
+    rng = np.random.default_rng(9)
+    wq = rng.integers(1, 65, query.shape[0]).astype(np.int32)
+    kq = rng.choice([1, 10, 100], query.shape[0]).astype(np.int32)
+    bids, bdis = ix.search_batched(query, wq, kq, 100)
+    rids, rdis = orc.search_var(index, query, wq, kq, 100)
+    assert_same_results(bids, bdis, rids, rdis, "batched nlist=9000")
 
 
 @pytest.mark.parametrize("mfma", [1, 0])
