@@ -225,7 +225,10 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * "fast_running_bound" (1 =
  * the 8-bit scan tightens each query's filter as its candidates accumulate;
  * results unchanged, default 1), "batched_group", "batched_threads",
- * "batched_capacity" (mivf_search_batched), "list_major" (1 = mivf_search
+ * "batched_capacity" and "batched_capacity_per_probe" (candidates per query
+ * of the list-major exact kernels: at least the first, default 16384, and
+ * at least the second times the query batch's largest w, default 256),
+ * "list_major" (1 = mivf_search
  * sends large batches to the list-major kernels, default 1),
  * "fast_tables" (table type of mivf_search_fast: 0 = 8-bit entries built on
  * the FP16 matrix cores with one scale per (query, list) and int8

@@ -48,7 +48,12 @@ struct XWs {
     }
 };
 
-int capacity(const mivf_index *ix, int kmax) { return std::max(ix->batched_cap, 2 * kmax); }
+// Candidates per query: the option's capacity, at least 2 kmax, and at least
+// batched_cap_pp per probe (the vectors within a query's bound grow with its
+// probes).
+int capacity(const mivf_index *ix, int kmax, int wmax) {
+    return std::max(std::max(ix->batched_cap, 2 * kmax), ix->batched_cap_pp * wmax);
+}
 
 bool cfg_ok(int m, const XCfg &c) { return exact_lds_bytes(m, c.g) > 0 && (c.nt == 256 || c.nt == 512); }
 
@@ -73,7 +78,7 @@ std::vector<XCfg> candidates(int m) {
 int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, int nq, int wmax, int kmax,
                     int32_t *d_ids, float *d_dis, hipStream_t st, Timer *tm = nullptr, bool overlap = false) {
     const int nlist = ix->nlist;
-    const int capq = capacity(ix, kmax);
+    const int capq = capacity(ix, kmax, wmax);
     const int64_t np = int64_t(nq) * wmax;
     MIVF_TRY(x.ws.qn.ensure(sizeof(double) * nq));
     MIVF_TRY(x.ws.dmat.ensure(sizeof(float) * size_t(nq) * nlist));
@@ -321,8 +326,10 @@ std::shared_ptr<void> new_list_major_ws(hipStream_t st) {
 }
 
 // Device bytes per query the list-major kernels hold beyond the plain path's:
-// the candidate buffer of capacity(kmax) (distance, id) pairs.
-int64_t list_major_query_bytes(const mivf_index *ix, int kmax) { return int64_t(capacity(ix, kmax)) * 8; }
+// the candidate buffer of capacity(kmax, wmax) (distance, id) pairs.
+int64_t list_major_query_bytes(const mivf_index *ix, int kmax, int wmax) {
+    return int64_t(capacity(ix, kmax, wmax)) * 8;
+}
 
 int list_major_enqueue(mivf_index *ix, void *ws, const float *d_q, int nq, int wmax, const int32_t *d_w,
                        const int32_t *d_k, int kmax, int32_t *d_ids, float *d_dis, int *h_flags, hipStream_t st) {

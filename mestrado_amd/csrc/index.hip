@@ -1175,6 +1175,9 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     } else if (n == "batched_capacity") {
         MIVF_REQUIRE(value >= 64 && value <= (1 << 20), kInvalid, "batched_capacity must be in 64..2^20");
         ix->batched_cap = int(value);
+    } else if (n == "batched_capacity_per_probe") {
+        MIVF_REQUIRE(value >= 0 && value <= 65536, kInvalid, "batched_capacity_per_probe must be in 0..65536");
+        ix->batched_cap_pp = int(value);
     } else if (n == "batched_group") {
         MIVF_REQUIRE(exact_lds_bytes(ix->m, int(value)) > 0, kInvalid, "batched_group: not a compiled shape");
         ix->batched_g = int(value);

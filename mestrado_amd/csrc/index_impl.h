@@ -142,6 +142,7 @@ struct mivf_index {
     int exact_seed_cap = 4096; // the same for the list-major exact kernels
     int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
     int batched_cap = 16384;  // candidate capacity per query of the batched exact form (>= 2 kmax)
+    int batched_cap_pp = 256; // ... and at least this many per probe
     int batched_g = 4, batched_nt = 512;  // its work-item shape (queries per item, threads)
     // mivf_search on batches of at least list_major_min queries runs the
     // batched list-major kernels (identical results)
@@ -205,7 +206,7 @@ int search_list_major(mivf_index *ix, const float *d_q, int nq, int w, int k, in
 // rows of d_ids/d_dis were rewritten after the enqueued work).
 std::shared_ptr<void> new_list_major_ws(hipStream_t st);
 bool list_major_fits(const mivf_index *ix, int nq);
-int64_t list_major_query_bytes(const mivf_index *ix, int kmax);
+int64_t list_major_query_bytes(const mivf_index *ix, int kmax, int wmax);
 int list_major_enqueue(mivf_index *ix, void *ws, const float *d_q, int nq, int wmax, const int32_t *d_w,
                        const int32_t *d_k, int kmax, int32_t *d_ids, float *d_dis, int *h_flags, hipStream_t st);
 int list_major_finish(mivf_index *ix, void *ws, const float *d_q, int nq, const int32_t *h_w, const int32_t *h_k,

@@ -309,7 +309,7 @@ int mivf_server_submit(mivf_server *sv, const float *d_q, int nq, int w, int k, 
     r.mem = request_mem(sv->ix, nq, w, k);
     // a micro-batch of this server may take the list-major kernels: charge
     // their candidate buffer as well
-    if (list_major_fits(sv->ix, sv->max_batch)) r.mem += int64_t(nq) * list_major_query_bytes(sv->ix, k);
+    if (list_major_fits(sv->ix, sv->max_batch)) r.mem += int64_t(nq) * list_major_query_bytes(sv->ix, k, w);
     *ticket = int64_t(sv->reqs.size());
     sv->reqs.push_back(r);
     sv->waiting.push_back(*ticket);
@@ -390,7 +390,7 @@ int warm_up(mivf_server *sv, const float *d_pool, int64_t pool_n, int w, int k) 
             r.d_dis = od.as<float>() + i * per_slot;
             r.t_submit = now_ms();
             r.mem = request_mem(sv->ix, nq, w, k);
-            if (list_major_fits(sv->ix, sv->max_batch)) r.mem += int64_t(nq) * list_major_query_bytes(sv->ix, k);
+            if (list_major_fits(sv->ix, sv->max_batch)) r.mem += int64_t(nq) * list_major_query_bytes(sv->ix, k, w);
             sv->reqs.push_back(r);
             sv->waiting.push_back(int64_t(sv->reqs.size()) - 1);
             sv->waiting_q += nq;

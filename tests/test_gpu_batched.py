@@ -41,11 +41,13 @@ def test_batched_overflow_fallback(mivf, zipf16):
     w, k = uneven(query.shape[0], seed=9)
     rids, rdis = ix.search_var(query, w, k, 100)
     ix.set_option("batched_capacity", 64)
+    ix.set_option("batched_capacity_per_probe", 0)
     try:
         ids, dis = ix.search_batched(query, w, k, 100)
         over = ix.batched_stats()
     finally:
-        ix.set_option("batched_capacity", 16384)  # the library default
+        ix.set_option("batched_capacity", 16384)  # the library defaults
+        ix.set_option("batched_capacity_per_probe", 256)
     assert over > 0, "the tiny buffer should overflow"
     assert_same(ids, dis, rids, rdis, "batched with overflow")
     ids, dis = ix.search_batched(query, w, k, 100)

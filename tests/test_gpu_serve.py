@@ -67,6 +67,7 @@ def test_serving_list_major_micro_batches(mivf, zipf16, cap):
     ix.set_option("list_major", 1)
     ix.set_option("list_major_min_queries", 48)
     ix.set_option("batched_capacity", cap)
+    ix.set_option("batched_capacity_per_probe", 0 if cap == 64 else 256)
     try:
         dq = [mivf.DeviceArray.from_numpy(q) for q, _, _ in reqs]
         sv = mivf.Server(ix, slots=2, max_batch=256)
@@ -87,6 +88,7 @@ def test_serving_list_major_micro_batches(mivf, zipf16, cap):
     finally:
         ix.set_option("list_major_min_queries", 2048)
         ix.set_option("batched_capacity", 16384)
+        ix.set_option("batched_capacity_per_probe", 256)
 
 
 def test_serving_min_fill_policy(mivf, zipf16):
