@@ -54,6 +54,9 @@ def main():
                     help="vectors entering the seed bound (0 = whole seed lists, -1 = library default)")
     ap.add_argument("--fast-w", type=int, default=0, help="probes of the reduced-precision form (0 = sweep)")
     ap.add_argument("--no-fast", action="store_true")
+    ap.add_argument("--fast-float-rows", action="store_true",
+                    help="rerank the reduced-precision form from the float32 copy of the base instead of the "
+                         "byte copy (the data are integers 0..255, so both give the same distances)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--batched-cap", type=int, default=0, help="candidate buffer per query of the batched form "
@@ -65,6 +68,9 @@ def main():
     ap.add_argument("--precision-study", action="store_true",
                     help="the reduced-precision table types on this workload's index (w=32, shortlist 1024, "
                          "k=100): recall and distance error against FP64 per table type")
+    ap.add_argument("--quantizer-cache", default="",
+                    help="npz file of the trained coarse and PQ centroids: loaded when it exists, else written "
+                         "after training (repeated runs on one workload skip the k-means)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run the timed steps of the search forms and a 1 GiB device copy "
                          "(counter calibration), then exit")
@@ -96,8 +102,16 @@ def main():
     log(f"data generated on device in {time.time() - t0:.1f}s: train {cfg['ntrain']} base {cfg['nbase']}")
 
     t0 = time.time()
-    coarse, pq = M.train(train, cfg["nlist"], cfg["m"], 256, cfg["niter_coarse"], cfg["niter_pq"], seed)
-    log(f"trained in {time.time() - t0:.1f}s ({cfg['niter_coarse']} coarse, {cfg['niter_pq']} PQ k-means iterations)")
+    if args.quantizer_cache and os.path.exists(args.quantizer_cache):
+        qc = np.load(args.quantizer_cache)
+        coarse, pq = qc["coarse"], qc["pq"]
+        log(f"quantizers loaded from {args.quantizer_cache}")
+    else:
+        coarse, pq = M.train(train, cfg["nlist"], cfg["m"], 256, cfg["niter_coarse"], cfg["niter_pq"], seed)
+        log(f"trained in {time.time() - t0:.1f}s ({cfg['niter_coarse']} coarse, {cfg['niter_pq']} PQ k-means "
+            f"iterations)")
+        if args.quantizer_cache:
+            np.savez(args.quantizer_cache, coarse=coarse, pq=pq)
     train.free()
     t0 = time.time()
     ix = M.Index(coarse, pq)
@@ -222,7 +236,11 @@ def main():
         ix.set_option("fast_seed_lists", args.fast_seed)
         if args.fast_seed_cap >= 0:
             ix.set_option("fast_seed_cap", args.fast_seed_cap)
-        ix.attach_base(base)  # list-ordered copy of the raw vectors (index build, untimed)
+        # list-ordered copy of the raw vectors (index build, untimed); one byte
+        # per value, exact for this integer-valued data
+        if not args.fast_float_rows:
+            ix.set_option("base_bytes", 1)
+        ix.attach_base(base)
         f_ids = M.DeviceArray((nq, k), np.int32)
         f_dis = M.DeviceArray((nq, k), np.float32)
         sweep = []
@@ -249,11 +267,26 @@ def main():
         t_fast = timed(fast_fn)
         fst = stages(fast_fn)
         cands, over = ix.fast_stats()
+        rows_note = None
+        if not args.fast_float_rows:
+            # the same operating point reranked from the float32 copy: same results, 4x the row bytes
+            h_ids, h_dis = f_ids.numpy(), f_dis.numpy()
+            ix.set_option("base_bytes", 0)
+            ix.attach_base(base)
+            t_f32 = timed(fast_fn)
+            f32_same = same_bits(h_ids, h_dis, f_ids.numpy(), f_dis.numpy())
+            ix.set_option("base_bytes", 1)
+            ix.attach_base(base)
+            rows_note = {"rerank_rows": "bytes (every base value is an integer in 0..255: the bytes convert to the "
+                                        "same floats)", "qps_float32_rows": round(nq / t_f32, 1),
+                         "float32_rows_bit_identical": f32_same}
         fast = {"qps": round(nq / t_fast, 1), "ms_per_step": round(t_fast * 1000, 4), "w": fw, "kl": kl,
                 "seed_lists": args.fast_seed, "candidates_per_query": round(cands / nq, 1),
                 "overflowed_queries": over,
                 "stages_ms": {n: round(float(v), 4) for n, v in
                               zip(("coarse", "seed", "pairs", "scan8", "select", "rerank"), fst)}}
+        if rows_note:
+            fast.update(rows_note)
         fast.update(recall(f_ids.numpy(), f_dis.numpy()))
         if sweep:
             fast["operating_point_rule"] = (

@@ -166,8 +166,13 @@ int mivf_search_rerank(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
 /* Keep a copy of the raw vectors d_base [nbase][d] in HBM in list order, so
  * that the rerank of mivf_search_fast reads each query's shortlist from at
  * most w contiguous regions (pass d_base = NULL there to use it).  The copy
- * is dropped when the lists change (add / set_lists).  No reference
- * counterpart: the reference reranks from vbase in host memory.             */
+ * is dropped when the lists change (add / set_lists).  With option
+ * "base_bytes" = 1 the copy holds one byte per value (a quarter of the HBM
+ * reads of the rerank); the call then fails with MIVF_EINVAL unless every
+ * value is an integer in 0..255 (as data read from bvecs files), so that the
+ * bytes convert back to the same floats and the rerank returns the same
+ * distances bit for bit.  No reference counterpart: the reference reranks
+ * from vbase in host memory.                                                */
 int mivf_index_attach_base(mivf_index *ix, const float *d_base, int64_t nbase);
 
 /* Batched reduced-precision form with exact rerank (no reference
@@ -240,6 +245,8 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * every shape; the same probes either way), "exact_seed_cap" (vectors of
  * the nearest lists that set each query's bound in the list-major exact
  * kernels: default 4096, 0 = the whole lists; results unchanged),
+ * "base_bytes" (1 = mivf_index_attach_base keeps a byte copy of an
+ * integer-valued base, default 0),
  * "serve_min_fill" and "serve_max_wait_us" (servers over this index: while
  * a micro-batch is in
  * flight, a free slot launches only once serve_min_fill queries wait or the

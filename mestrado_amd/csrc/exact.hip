@@ -137,22 +137,39 @@ __device__ void xtables(const ExactArgs &a, unsigned char *smem, int l) {
             for (int i = 0; i < DS; ++i) r[g][i] = R[(g0 + g) * D + s * DS + i];
             rn64[g] = double(rn[(g0 + g) * M + s]);
         }
+        // with 8-float sub-vectors, the centroid row and norm of code
+        // c0 + CPS (j + 1) are loaded while code c0 + CPS j is computed (rows
+        // of codes >= ks are clamped to a valid row and their entries never
+        // indexed); wider sub-vectors leave no registers for that
+        constexpr bool PF = DS <= 8;
+        float4 pn4[DS / 4];
+        double pnn;
+        auto load_p = [&](int c) {
+            const int cc = min(c, a.ks - 1);
+            const float4 *src = reinterpret_cast<const float4 *>(a.pq + (int64_t(s) * a.ks + cc) * DS);
+#pragma unroll
+            for (int i = 0; i < DS / 4; ++i) pn4[i] = src[i];
+            pnn = a.pq_norm[int64_t(s) * a.ks + cc];
+        };
+        if constexpr (PF) load_p(c0);
 #pragma unroll 1
         for (int j = 0; j < E; ++j) {
             const int c = c0 + CPS * j;
             float vals[GC];
-            if (c < a.ks) {
-                float pr[DS];
-                const float4 *src = reinterpret_cast<const float4 *>(a.pq + (int64_t(s) * a.ks + c) * DS);
+            if constexpr (!PF) load_p(c);
+            float pr[DS];
 #pragma unroll
-                for (int i = 0; i < DS / 4; ++i) {
-                    const float4 v = src[i];
-                    pr[4 * i] = v.x;
-                    pr[4 * i + 1] = v.y;
-                    pr[4 * i + 2] = v.z;
-                    pr[4 * i + 3] = v.w;
-                }
-                const double pnv = a.pq_norm[int64_t(s) * a.ks + c];
+            for (int i = 0; i < DS / 4; ++i) {
+                pr[4 * i] = pn4[i].x;
+                pr[4 * i + 1] = pn4[i].y;
+                pr[4 * i + 2] = pn4[i].z;
+                pr[4 * i + 3] = pn4[i].w;
+            }
+            const double pnv = pnn;
+            if constexpr (PF) {
+                if (j + 1 < E) load_p(c + CPS);
+            }
+            if (c < a.ks) {
 #pragma unroll
                 for (int g = 0; g < GC; ++g) {
                     float dot = r[g][0] * pr[0];

@@ -51,6 +51,7 @@ int upload_lists(mivf_index *ix) {
     ix->list_gen++;
     ix->rows.release();  // list positions changed: an attached copy is stale
     ix->rows_ok = false;
+    ix->rows_u8 = false;
     return kOk;
 }
 
@@ -1018,6 +1019,10 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     if (d_base)
         MIVF_CHECK_HIP(launch_rerank_pos(d_base, nbase, false, ix->ids.as<int32_t>(), d_q, nq, d,
                                          ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
+    else if (ix->rows_u8)
+        MIVF_CHECK_HIP(launch_rerank_pos_u8(ix->rows.as<uint8_t>(), ix->ntotal, ix->ids.as<int32_t>(), d_q, nq, d,
+                                            ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis,
+                                            st));
     else
         MIVF_CHECK_HIP(launch_rerank_pos(ix->rows.as<float>(), ix->ntotal, true, ix->ids.as<int32_t>(), d_q, nq, d,
                                          ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
@@ -1050,9 +1055,28 @@ int mivf_index_attach_base(mivf_index *ix, const float *d_base, int64_t nbase) t
     for (const auto &v : ix->h_ids)
         for (int32_t id : v) maxid = std::max<int64_t>(maxid, id);
     MIVF_REQUIRE(maxid < nbase, kInvalid, "attach_base: the lists hold ids >= nbase");
-    MIVF_TRY(ix->rows.ensure(sizeof(float) * size_t(std::max<int64_t>(ix->ntotal, 1)) * ix->d));
-    MIVF_CHECK_HIP(launch_gather_rows(d_base, ix->d, ix->ids.as<int32_t>(), ix->ntotal, ix->rows.as<float>(), nullptr));
-    MIVF_CHECK_HIP(hipDeviceSynchronize());
+    ix->rows_ok = false;
+    ix->rows_u8 = false;
+    const size_t nrow = size_t(std::max<int64_t>(ix->ntotal, 1));
+    if (ix->base_bytes) {
+        // byte copy: exact only when every value is an integer in 0..255
+        MIVF_TRY(ix->rows.ensure(nrow * ix->d));
+        MIVF_CHECK_HIP(hipMemset(ix->errflag.p, 0, sizeof(int)));
+        MIVF_CHECK_HIP(launch_gather_rows_u8(d_base, ix->d, ix->ids.as<int32_t>(), ix->ntotal, ix->rows.as<uint8_t>(),
+                                             ix->errflag.as<int>(), nullptr));
+        int bad = 0;
+        MIVF_CHECK_HIP(hipDeviceSynchronize());
+        MIVF_CHECK_HIP(hipMemcpy(&bad, ix->errflag.p, sizeof(int), hipMemcpyDeviceToHost));
+        MIVF_CHECK_HIP(hipMemset(ix->errflag.p, 0, sizeof(int)));  // the flag also reports kernel preconditions
+        MIVF_REQUIRE(bad == 0, kInvalid,
+                     "attach_base: option base_bytes needs every base value to be an integer in 0..255");
+        ix->rows_u8 = true;
+    } else {
+        MIVF_TRY(ix->rows.ensure(sizeof(float) * nrow * ix->d));
+        MIVF_CHECK_HIP(launch_gather_rows(d_base, ix->d, ix->ids.as<int32_t>(), ix->ntotal, ix->rows.as<float>(),
+                                          nullptr));
+        MIVF_CHECK_HIP(hipDeviceSynchronize());
+    }
     ix->rows_ok = true;
     return kOk;
 } MIVF_CATCH_ALL
@@ -1187,6 +1211,8 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     } else if (n == "fast_tables") {
         MIVF_REQUIRE(value >= 0 && value <= 2, kInvalid, "fast_tables must be 0, 1 or 2");
         ix->fast_tables = int(value);
+    } else if (n == "base_bytes") {
+        ix->base_bytes = value != 0;
     } else if (n == "coarse_mfma") {
         ix->coarse_mfma = value != 0;
     } else if (n == "list_major") {

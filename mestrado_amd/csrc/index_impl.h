@@ -127,6 +127,8 @@ struct mivf_index {
     DevBuf fpb;              // fp16 B operands of the reduced-precision tables
     DevBuf rows;             // raw vectors in list order (mivf_index_attach_base)
     bool rows_ok = false;
+    bool rows_u8 = false;    // rows holds bytes (option base_bytes: integer-valued data 0..255)
+    bool base_bytes = false; // option base_bytes: attach_base keeps a byte copy
     // batched reduced-precision form scratch
     DevBuf f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_items, f_ctrl, f_cand, f_ccnt,
         f_short, f_ns, f_rthr, f_rhist;

@@ -299,6 +299,14 @@ hipError_t launch_rerank_pos(const float *rows, int64_t nrows, bool list_ordered
                              int kl, int k, int32_t *out_ids, float *out_dis, hipStream_t s);
 // out[r] = base[ids[r]] (rows of d floats)
 hipError_t launch_gather_rows(const float *base, int d, const int32_t *ids, int64_t n, float *out, hipStream_t s);
+// Same as the list-ordered rerank with byte rows (integer-valued data 0..255:
+// the same float values, so the same distances bit for bit).
+hipError_t launch_rerank_pos_u8(const uint8_t *rows, int64_t nrows, const int32_t *list_ids, const float *queries,
+                                int nq, int d, const double *qnorm, const int32_t *short_pos, int kl, int k,
+                                int32_t *out_ids, float *out_dis, hipStream_t s);
+// out[r] = base[ids[r]] as bytes; *bad |= 1 when a value is not an integer in 0..255
+hipError_t launch_gather_rows_u8(const float *base, int d, const int32_t *ids, int64_t n, uint8_t *out, int *bad,
+                                 hipStream_t s);
 
 // ADC distances of one residual [m*ds] over n codes [n][m] (uint8), in code
 // order, with the reference table arithmetic (adc.hip).

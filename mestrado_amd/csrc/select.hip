@@ -122,15 +122,82 @@ __global__ __launch_bounds__(NT) void merge_kernel(const uint64_t *__restrict__ 
     sort_and_emit(buf, c, kstride, out_ids + int64_t(q) * kstride, out_dis + int64_t(q) * kstride);
 }
 
+// Element t of row x as a float: rows of float, or of bytes holding the
+// integers 0..255 exactly (a byte row converts to the same floats).
+__device__ inline float row_el(const float *x, int t) { return x[t]; }
+__device__ inline float row_el(const uint8_t *x, int t) { return float(x[t]); }
+
+// The squared norm (f32 accumulator) and the dot product with qv of one row,
+// both sequential in element order with every product rounded (no FMA).
+__device__ inline void row_sums(const float *x, const float *qv, int d, bool vec, float &na, float &dot) {
+    int t = 0;
+    if (vec) {
+        // 16-byte loads, issued 8 at a time; the sums keep the element order
+        const float4 *x4 = reinterpret_cast<const float4 *>(x);
+        for (; t + 32 <= d; t += 32) {
+            float4 v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = x4[t / 4 + j];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float el[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    na = na + el[u] * el[u];
+                    dot = dot + el[u] * qv[t + 4 * j + u];
+                }
+            }
+        }
+    }
+    for (; t < d; ++t) {
+        const float xv = x[t];
+        const float p = xv * xv;
+        na = na + p;
+        const float pd = xv * qv[t];
+        dot = dot + pd;
+    }
+}
+
+__device__ inline void row_sums(const uint8_t *x, const float *qv, int d, bool vec, float &na, float &dot) {
+    int t = 0;
+    if (vec) {
+        // one 128-byte row in eight 16-byte loads issued together
+        const uint4 *x16 = reinterpret_cast<const uint4 *>(x);
+        for (; t + 128 <= d; t += 128) {
+            uint4 v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = x16[t / 16 + j];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t wd[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const float el = float((wd[u >> 2] >> (8 * (u & 3))) & 255u);
+                    na = na + el * el;
+                    dot = dot + el * qv[t + 16 * j + u];
+                }
+            }
+        }
+    }
+    for (; t < d; ++t) {
+        const float xv = row_el(x, t);
+        const float p = xv * xv;
+        na = na + p;
+        const float pd = xv * qv[t];
+        dot = dot + pd;
+    }
+}
+
 // rerank: a = base row (f32-accumulated norm), b = query (f64-accumulated norm),
 // as knn_full(2, 1, ktmp, d, k, nearest, query) in sequential/ivf_pq/ivf_search.cpp:130.
 // MODE 0: the shortlist holds ids, rows base[id].  MODE 1: it holds list
 // positions, id = list_ids[pos], rows base[id].  MODE 2: list positions, rows
 // from the list-ordered copy base[pos] (nbase = its row count).  qnorm: the
 // queries' f64-accumulated norms when the caller has them (the same sum as
-// below), else nullptr.
-template <int MODE>
-__global__ __launch_bounds__(NT) void rerank_kernel(const float *__restrict__ base, int64_t nbase,
+// below), else nullptr.  T: float rows, or byte rows of integer-valued data
+// (the same float values, so the same distances bit for bit).
+template <int MODE, typename T = float>
+__global__ __launch_bounds__(NT) void rerank_kernel(const T *__restrict__ base, int64_t nbase,
                                                     const int32_t *__restrict__ list_ids,
                                                     const float *__restrict__ queries, int d,
                                                     const double *__restrict__ qnorm,
@@ -161,7 +228,7 @@ __global__ __launch_bounds__(NT) void rerank_kernel(const float *__restrict__ ba
     }
     __syncthreads();
     const double nb = qn;
-    const bool vec4 = (d & 3) == 0 && (reinterpret_cast<uintptr_t>(base) & 15u) == 0;
+    const bool vec = (d % (64 / int(sizeof(T)))) == 0 && (reinterpret_cast<uintptr_t>(base) & 15u) == 0;
     for (int i = tid; i < kl; i += NT) {
         const int32_t e = short_ids[int64_t(q) * kl + i];
         int32_t id = e;
@@ -170,35 +237,8 @@ __global__ __launch_bounds__(NT) void rerank_kernel(const float *__restrict__ ba
         if (MODE == 1) row = id;
         uint64_t key = kNoKey;
         if (e >= 0 && id >= 0 && row < nbase) {
-            const float *x = base + row * d;
             float na = 0.0f, dot = 0.0f;
-            int t = 0;
-            if (vec4) {
-                // 16-byte loads, issued 8 at a time; the sums keep the
-                // sequential element order
-                const float4 *x4 = reinterpret_cast<const float4 *>(x);
-                for (; t + 32 <= d; t += 32) {
-                    float4 v[8];
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) v[j] = x4[t / 4 + j];
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const float el[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            na = na + el[u] * el[u];
-                            dot = dot + el[u] * qv[t + 4 * j + u];
-                        }
-                    }
-                }
-            }
-            for (; t < d; ++t) {
-                float xv = x[t];
-                float p = xv * xv;
-                na = na + p;
-                float pd = xv * qv[t];
-                dot = dot + pd;
-            }
+            row_sums(base + row * d, qv, d, vec, na, dot);
             float c = float(nb + double(na));
             float m2 = -2.0f * dot;
             key = ckey(m2 + c, id);
@@ -218,6 +258,19 @@ __global__ void gather_rows_kernel(const float *__restrict__ base, int d, const 
     if (e >= n * d) return;
     const int64_t r = e / d;
     out[e] = base[int64_t(ids[r]) * d + (e - r * d)];
+}
+
+// Byte copy of the rows in list order; *bad is set when a value is not an
+// integer in 0..255 (the copy would not be exact).
+__global__ void gather_rows_u8_kernel(const float *__restrict__ base, int d, const int32_t *__restrict__ ids,
+                                      int64_t n, uint8_t *__restrict__ out, int *__restrict__ bad) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n * d) return;
+    const int64_t r = e / d;
+    const float v = base[int64_t(ids[r]) * d + (e - r * d)];
+    const bool ok = v >= 0.0f && v <= 255.0f && v == __builtin_truncf(v);
+    if (!ok) atomicOr(bad, 1);
+    out[e] = ok ? uint8_t(v) : uint8_t(0);
 }
 
 }  // namespace
@@ -267,6 +320,23 @@ hipError_t launch_rerank_pos(const float *rows, int64_t nrows, bool list_ordered
     else
         rerank_kernel<1><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm, short_pos, kl, k,
                                                      out_ids, out_dis);
+    return hipGetLastError();
+}
+
+hipError_t launch_rerank_pos_u8(const uint8_t *rows, int64_t nrows, const int32_t *list_ids, const float *queries,
+                                int nq, int d, const double *qnorm, const int32_t *short_pos, int kl, int k,
+                                int32_t *out_ids, float *out_dis, hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    if (kl < 1 || kl > KBUF || k < 1 || k > KBUF || d > 512) return hipErrorInvalidValue;
+    rerank_kernel<2, uint8_t><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm, short_pos, kl, k,
+                                                          out_ids, out_dis);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_rows_u8(const float *base, int d, const int32_t *ids, int64_t n, uint8_t *out, int *bad,
+                                 hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    gather_rows_u8_kernel<<<unsigned((n * d + 255) / 256), 256, 0, s>>>(base, d, ids, n, out, bad);
     return hipGetLastError();
 }
 

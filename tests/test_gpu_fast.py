@@ -192,6 +192,43 @@ def test_fast_attached_base(mivf, orc, c1):
         ix.search_fast(query, 8, 128, 100)          # stale copy dropped
 
 
+@pytest.mark.parametrize("which,w,kl,k", [("c1", 8, 128, 100), ("c1", 256, 1024, 1000), ("m16", 32, 176, 100),
+                                          ("m16", 17, 1024, 10)])
+def test_fast_byte_base(mivf, orc, c1, zipf16, which, w, kl, k):
+    """Option base_bytes: the attached copy of an integer-valued base holds one
+    byte per value and the rerank returns the same ids and distances, bit for
+    bit, as with the float copy; a base with a value that is not an integer
+    in 0..255 is refused and leaves the index searchable."""
+    if which == "c1":
+        q = c1["gold_q"]
+        base, query = c1["base"], c1["query"]
+        ix = mivf.Index(q["coarse"], q["pq"])
+        ix.add(base)
+    else:
+        ix, _, query, base = zipf16
+    assert np.array_equal(base, np.clip(np.rint(base), 0, 255)), "the fixture base is integer-valued"
+    try:
+        ix.attach_base(base)
+        f_ids, f_dis = ix.search_fast(query, w, kl, k)
+        ix.set_option("base_bytes", 1)
+        ix.attach_base(base)
+        b_ids, b_dis = ix.search_fast(query, w, kl, k)
+        assert np.array_equal(f_ids, b_ids) and np.array_equal(f_dis.view(np.uint32), b_dis.view(np.uint32))
+        for bad_value in (0.5, -1.0, 256.0, np.nan):
+            bad = base.copy()
+            bad[base.shape[0] // 2, 7] = bad_value
+            with pytest.raises(mivf.MivfError):
+                ix.attach_base(bad)
+            with pytest.raises(mivf.MivfError):
+                ix.search_fast(query, w, kl, k)      # no copy after a refused one
+        ix.set_option("base_bytes", 0)
+        ix.attach_base(base)
+        a_ids, a_dis = ix.search_fast(query, w, kl, k)
+        assert np.array_equal(f_ids, a_ids) and np.array_equal(f_dis.view(np.uint32), a_dis.view(np.uint32))
+    finally:
+        ix.set_option("base_bytes", 0)
+
+
 @pytest.mark.parametrize("which,kl,seed_lists,cap", [("m16", 256, 1, 0), ("m16", 64, 2, 0), ("m8", 128, 1, 0),
                                                       ("m8", 1000, 1, 0), ("m16", 128, 1, 300), ("m16", 64, 2, 1024),
                                                       ("m8", 128, 1, 129)])
