@@ -316,6 +316,8 @@ def main():
             "algorithmic_GBps_vs_hbm_peak": rf(f_vectors * m / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
             if f_ms > 0 else None,
             "int8_mfma_busy_recorded": recorded("counters", "mfma_busy fast_scan_kernel"),
+            "unit_busy_recorded": {u: recorded("counters", f"{u}_busy fast_scan_kernel") for u in ("valu", "lds", "mfma")},
+            "traffic": recorded("counters", "traffic_bytes_per_launch fast_scan_kernel"),
             "note": "one 16-byte LDS read per (vector, subquantizer) of a work item serves its 16 queries' "
                     "8-bit entries; the peak is this device's measured rate of random 16-byte LDS reads"}
         log(f"fast form: {fast}")
@@ -438,20 +440,30 @@ def main():
     def ldsrate(ms):
         return table_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
 
+    def busy(kernel):
+        # busy fractions of the kernel's units from the recorded counter runs,
+        # and the busiest of them (the unit that binds)
+        b = {u: recorded("counters", f"{u}_busy {kernel}") for u in ("valu", "lds", "mfma")}
+        b = {u: v for u, v in b.items() if v is not None}
+        return b, (max(b, key=b.get) if b else None)
+
     traffic = recorded("counters", "traffic_bytes_per_launch exact_scan_kernel")
+    xb, xbind = busy("exact_scan_kernel")
+    qb, qbind = busy("scan_kernel")
     roofline = {
         "bound": "hbm", "achieved": rf(hbm(scan_ms), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": rf(hbm(scan_ms) / HBM_PEAK_GBS) if scan_ms > 0 else None,
         "traffic": int(traffic) if traffic else None,
         "kernel": "exact_scan_kernel", "algorithmic_bytes_per_launch": scan_bytes,
-        "binding_unit": "lds",
+        "binding_unit": xbind, "unit_busy_recorded": xb,
         "lds": {"achieved": rf(ldsrate(scan_ms), 1), "peak": round(lds["random16"], 1), "unit": "GB/s",
                 "frac": rf(ldsrate(scan_ms) / lds["random16"]) if scan_ms > 0 else None,
                 "conflict_free_peak": round(lds["peak16"], 1), "bytes_per_launch": table_bytes},
         "note": "achieved = codes of every (query, probed list) pair / kernel time; the kernel reads each list "
-                "once per work item of 4 queries (HBM traffic far below the algorithmic bytes) and is bound by "
-                "its random 16-byte FP32 table reads from LDS: lds.peak is this device's measured rate of such "
-                "reads, conflict_free_peak the same reads on distinct banks"}
+                "once per work item of 4 queries (HBM traffic far below the algorithmic bytes); its binding unit "
+                "is the busiest in the recorded counter runs (the VALU: exact FP32 tables and sums), the LDS "
+                "(random 16-byte FP32 table reads) next: lds.peak is this device's measured rate of such reads, "
+                "conflict_free_peak the same reads on distinct banks"}
     pq_traffic = recorded("counters", "traffic_bytes_per_launch scan_kernel")
     per_query = {
         "qps": round(nq / t_pq, 1), "ms_per_step": round(t_pq * 1000, 4),
@@ -459,7 +471,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": rf(hbm(pq_scan_ms), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": rf(hbm(pq_scan_ms) / HBM_PEAK_GBS) if pq_scan_ms > 0 else None,
                      "traffic": int(pq_traffic) if pq_traffic else None, "kernel": "scan_kernel",
-                     "binding_unit": "lds",
+                     "binding_unit": qbind, "unit_busy_recorded": qb,
                      "lds": {"achieved": rf(ldsrate(pq_scan_ms), 1), "peak": round(lds["random4"], 1),
                              "unit": "GB/s",
                              "frac": rf(ldsrate(pq_scan_ms) / lds["random4"]) if pq_scan_ms > 0 else None,
