@@ -246,7 +246,10 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * the nearest lists that set each query's bound in the list-major exact
  * kernels: default 4096, 0 = the whole lists; results unchanged),
  * "base_bytes" (1 = mivf_index_attach_base keeps a byte copy of an
- * integer-valued base, default 0),
+ * integer-valued base, default 0), "fast_front_ranks" (n > 0 = the 8-bit
+ * scan takes the work items of each query's n nearest lists first, so that
+ * their candidates set the running bounds early; results unchanged; default
+ * 0: the extra, less full work items cost more than the candidates saved),
  * "serve_min_fill" and "serve_max_wait_us" (servers over this index: while
  * a micro-batch is in
  * flight, a free slot launches only once serve_min_fill queries wait or the

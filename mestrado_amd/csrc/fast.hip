@@ -604,7 +604,7 @@ __global__ void fill_i32_kernel(int *__restrict__ p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-// ---- list-major grouping of (query, probe) pairs with rank >= rank0[q].
+// ---- list-major grouping of (query, probe) pairs with probe rank in [rlo, rhi).
 // Each workgroup aggregates its PCH pairs in an LDS histogram over the lists
 // (nlist <= LBINS) and issues one global atomic per list it touches, so
 // popular lists do not serialise thousands of atomics.  With more lists the
@@ -615,14 +615,14 @@ __global__ void fill_i32_kernel(int *__restrict__ p, int n, int v) {
 constexpr int LBINS = 8192;
 constexpr int PCH = 2048;
 
-__device__ inline bool pair_ok(const int32_t *probes, int64_t e, int w, const int32_t *rank0, int nlist, int *l) {
+__device__ inline bool pair_ok(const int32_t *probes, int64_t e, int w, int rlo, int rhi, int nlist, int *l) {
     const int r = int(e % w);
     *l = probes[e];
-    return r >= (rank0 ? rank0[e / w] : 0) && *l >= 0 && *l < nlist;
+    return r >= rlo && r < rhi && *l >= 0 && *l < nlist;
 }
 
 __global__ __launch_bounds__(256) void pair_count_kernel(const int32_t *__restrict__ probes, int nq, int w,
-                                                         const int32_t *__restrict__ rank0, int nlist,
+                                                         int rlo, int rhi, int nlist,
                                                          int *__restrict__ cnt) {
     __shared__ int hc[LBINS];
     const int tid = threadIdx.x;
@@ -633,7 +633,7 @@ __global__ __launch_bounds__(256) void pair_count_kernel(const int32_t *__restri
     for (int i = tid; i < PCH; i += 256) {
         const int64_t e = e0 + i;
         int l;
-        if (e < np && pair_ok(probes, e, w, rank0, nlist, &l)) atomicAdd(&hc[l], 1);
+        if (e < np && pair_ok(probes, e, w, rlo, rhi, nlist, &l)) atomicAdd(&hc[l], 1);
     }
     __syncthreads();
     for (int i = tid; i < nlist; i += 256)
@@ -641,29 +641,32 @@ __global__ __launch_bounds__(256) void pair_count_kernel(const int32_t *__restri
 }
 
 __global__ __launch_bounds__(256) void pair_count_global_kernel(const int32_t *__restrict__ probes, int nq, int w,
-                                                                const int32_t *__restrict__ rank0, int nlist,
+                                                                int rlo, int rhi, int nlist,
                                                                 int *__restrict__ cnt) {
     const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
     int l;
-    if (e < int64_t(nq) * w && pair_ok(probes, e, w, rank0, nlist, &l)) atomicAdd(cnt + l, 1);
+    if (e < int64_t(nq) * w && pair_ok(probes, e, w, rlo, rhi, nlist, &l)) atomicAdd(cnt + l, 1);
 }
 
 __global__ __launch_bounds__(256) void pair_scatter_global_kernel(const int32_t *__restrict__ probes, int nq, int w,
-                                                                  const int32_t *__restrict__ rank0, int nlist,
+                                                                  int rlo, int rhi, int nlist,
                                                                   const int *__restrict__ pair_off,
                                                                   int *__restrict__ fill, int32_t *__restrict__ pair_q) {
     const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
     int l;
-    if (e < int64_t(nq) * w && pair_ok(probes, e, w, rank0, nlist, &l))
+    if (e < int64_t(nq) * w && pair_ok(probes, e, w, rlo, rhi, nlist, &l))
         pair_q[pair_off[l] + atomicAdd(fill + l, 1)] = int32_t(e / w);
 }
 
-// exclusive scans of pair counts and of work items per list (one workgroup)
+// exclusive scans of pair counts and of work items per list (one workgroup);
+// items are numbered from *item_base (0 if null), and *total_items becomes
+// the end of the numbering (item_base may equal total_items)
 __global__ void pair_scan_kernel(const int *__restrict__ cnt, int nlist, int G, int *__restrict__ pair_off,
-                                 int *__restrict__ item_off, int *__restrict__ total_items,
-                                 int *__restrict__ fill) {
+                                 int *__restrict__ item_off, int *total_items, int *__restrict__ fill,
+                                 const int *item_base) {
     __shared__ int part_p[1024], part_i[1024];
     const int tid = threadIdx.x;
+    const int base = item_base ? *item_base : 0;  // read by every thread before the first barrier
     const int per = (nlist + 1023) / 1024;
     int sp = 0, si = 0;
     for (int j = tid * per; j < min(nlist, (tid + 1) * per); ++j) {
@@ -674,7 +677,7 @@ __global__ void pair_scan_kernel(const int *__restrict__ cnt, int nlist, int G, 
     part_i[tid] = si;
     __syncthreads();
     if (tid == 0) {
-        int ap = 0, ai = 0;
+        int ap = 0, ai = base;
         for (int i = 0; i < 1024; ++i) {
             const int tp = part_p[i], ti = part_i[i];
             part_p[i] = ap;
@@ -698,7 +701,7 @@ __global__ void pair_scan_kernel(const int *__restrict__ cnt, int nlist, int G, 
 }
 
 __global__ __launch_bounds__(256) void pair_scatter_kernel(const int32_t *__restrict__ probes, int nq, int w,
-                                                           const int32_t *__restrict__ rank0, int nlist,
+                                                           int rlo, int rhi, int nlist,
                                                            const int *__restrict__ pair_off, int *__restrict__ fill,
                                                            int32_t *__restrict__ pair_q) {
     __shared__ int hc[LBINS];
@@ -711,7 +714,7 @@ __global__ __launch_bounds__(256) void pair_scatter_kernel(const int32_t *__rest
     for (int i = tid; i < PCH; i += 256) {
         const int64_t e = e0 + i;
         int l;
-        rk[i] = (e < np && pair_ok(probes, e, w, rank0, nlist, &l)) ? atomicAdd(&hc[l], 1) : -1;
+        rk[i] = (e < np && pair_ok(probes, e, w, rlo, rhi, nlist, &l)) ? atomicAdd(&hc[l], 1) : -1;
     }
     __syncthreads();
     for (int i = tid; i < nlist; i += 256)
@@ -725,7 +728,8 @@ __global__ __launch_bounds__(256) void pair_scatter_kernel(const int32_t *__rest
 }
 
 // One thread per (work item, query slot); items are numbered list by list
-// (item_off), so a binary search over item_off finds the item's list.
+// (item_off) from item_off[0] (the range's first item), so a binary search
+// over item_off finds the item's list.
 __global__ void item_expand_kernel(const int *__restrict__ pair_off, const int *__restrict__ item_off,
                                    const int64_t *__restrict__ list_off, const int32_t *__restrict__ pair_q,
                                    const float *__restrict__ tau, int nlist, int G,
@@ -733,7 +737,7 @@ __global__ void item_expand_kernel(const int *__restrict__ pair_off, const int *
                                    FastItem *__restrict__ items) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= max_slots) return;
-    const int it = int(e / FG), slot = int(e % FG);  // FG threads per record, slots >= G empty
+    const int it = int(e / FG) + item_off[0], slot = int(e % FG);  // FG threads per record, slots >= G empty
     if (it >= *total_items) return;
     int lo = 0, hi = nlist;  // last list with item_off[l] <= it (lists without items are skipped)
     while (hi - lo > 1) {
@@ -1202,22 +1206,26 @@ hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_pairs(const int32_t *probes, int nq, int w, const int32_t *rank0, const int64_t *list_off,
+hipError_t launch_pairs(const int32_t *probes, int nq, int w, int rlo, int rhi, const int64_t *list_off,
                         const float *tau, int nlist, int G, int *cnt, int *pair_off, int *item_off, int *total_items,
-                        int *fill, int32_t *pair_q, FastItem *items, hipStream_t s) {
+                        const int *item_base, int *fill, int32_t *pair_q, FastItem *items, hipStream_t s) {
     if (nlist < 1 || nlist > kMaxPairLists || G < 1 || G > FG) return hipErrorInvalidValue;
+    rlo = max(rlo, 0);
+    rhi = min(rhi, w);
     const int64_t np = int64_t(nq) * w;
+    const int64_t npr = int64_t(nq) * max(rhi - rlo, 0);  // pairs of the rank range, at most
     fill_i32_kernel<<<nblk(nlist, 256), 256, 0, s>>>(cnt, nlist, 0);
     if (nlist <= LBINS)
-        pair_count_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rank0, nlist, cnt);
+        pair_count_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rlo, rhi, nlist, cnt);
     else
-        pair_count_global_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rank0, nlist, cnt);
-    pair_scan_kernel<<<1, 1024, 0, s>>>(cnt, nlist, G, pair_off, item_off, total_items, fill);
+        pair_count_global_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rlo, rhi, nlist, cnt);
+    pair_scan_kernel<<<1, 1024, 0, s>>>(cnt, nlist, G, pair_off, item_off, total_items, fill, item_base);
     if (nlist <= LBINS)
-        pair_scatter_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rank0, nlist, pair_off, fill, pair_q);
+        pair_scatter_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rlo, rhi, nlist, pair_off, fill, pair_q);
     else
-        pair_scatter_global_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rank0, nlist, pair_off, fill, pair_q);
-    const int64_t max_slots = (np / G + nlist) * FG;  // >= FG * total items
+        pair_scatter_global_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rlo, rhi, nlist, pair_off, fill,
+                                                                 pair_q);
+    const int64_t max_slots = (npr / G + nlist) * FG;  // >= FG * items of the range
     item_expand_kernel<<<nblk(max_slots, 256), 256, 0, s>>>(pair_off, item_off, list_off, pair_q, tau, nlist, G,
                                                           total_items, max_slots, items);
     return hipGetLastError();

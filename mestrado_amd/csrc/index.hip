@@ -940,10 +940,19 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     hipStream_t sa = ix->aux;
     MIVF_CHECK_HIP(hipEventRecord(ix->aux_ev[0], st));
     MIVF_CHECK_HIP(hipStreamWaitEvent(sa, ix->aux_ev[0], 0));
-    MIVF_CHECK_HIP(launch_pairs(ix->ws.probes.as<int32_t>(), nq, w, nullptr, ix->off.as<int64_t>(), nullptr, nlist,
-                                fast_group_size(), ix->f_cnt.as<int>(), ix->f_poff.as<int>(), ix->f_ioff.as<int>(),
-                                ctrl, ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(), ix->f_items.as<FastItem>(),
-                                sa));
+    // work items of each query's fast_front nearest lists come first, so that
+    // their candidates set the queries' running bounds before the other
+    // lists are scanned (the order changes no result)
+    const int front = (ix->fast_running && ix->fast_front > 0 && ix->fast_front < w) ? ix->fast_front : 0;
+    MIVF_CHECK_HIP(launch_pairs(ix->ws.probes.as<int32_t>(), nq, w, 0, front > 0 ? front : w, ix->off.as<int64_t>(),
+                                nullptr, nlist, fast_group_size(), ix->f_cnt.as<int>(), ix->f_poff.as<int>(),
+                                ix->f_ioff.as<int>(), ctrl, nullptr, ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(),
+                                ix->f_items.as<FastItem>(), sa));
+    if (front > 0)
+        MIVF_CHECK_HIP(launch_pairs(ix->ws.probes.as<int32_t>(), nq, w, front, w, ix->off.as<int64_t>(), nullptr,
+                                    nlist, fast_group_size(), ix->f_cnt.as<int>(), ix->f_poff.as<int>(),
+                                    ix->f_ioff.as<int>(), ctrl, ctrl, ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(),
+                                    ix->f_items.as<FastItem>(), sa));
     MIVF_CHECK_HIP(launch_fill_i32(ix->f_ccnt.as<int>(), nq, 0, sa));
     MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, sa));
     // running bounds (+inf) and their histograms (0)
@@ -1211,6 +1220,9 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     } else if (n == "fast_tables") {
         MIVF_REQUIRE(value >= 0 && value <= 2, kInvalid, "fast_tables must be 0, 1 or 2");
         ix->fast_tables = int(value);
+    } else if (n == "fast_front_ranks") {
+        MIVF_REQUIRE(value >= 0 && value <= 1024, kInvalid, "fast_front_ranks must be in 0..1024");
+        ix->fast_front = int(value);
     } else if (n == "base_bytes") {
         ix->base_bytes = value != 0;
     } else if (n == "coarse_mfma") {

@@ -129,6 +129,7 @@ struct mivf_index {
     bool rows_ok = false;
     bool rows_u8 = false;    // rows holds bytes (option base_bytes: integer-valued data 0..255)
     bool base_bytes = false; // option base_bytes: attach_base keeps a byte copy
+    int fast_front = 0;      // option fast_front_ranks: probe ranks whose work items the 8-bit scan takes first
     // batched reduced-precision form scratch
     DevBuf f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_items, f_ctrl, f_cand, f_ccnt,
         f_short, f_ns, f_rthr, f_rhist;

@@ -320,11 +320,13 @@ def test_fast_select_picks_the_kl_smallest_keys(mivf, orc, case):
             f"{case} q={q} n={n}: not the kl smallest keys"
 
 
-def test_fast_running_bound_changes_nothing(mivf, zipf16):
+@pytest.mark.parametrize("front", [0, 1, 3, 31])
+def test_fast_running_bound_changes_nothing(mivf, zipf16, front):
     """The running per-query bound (shared by the workgroups through
     candidate-distance histograms) only drops vectors that cannot enter the
     shortlist: results equal those without it, run after run, although the
-    number of candidates varies with the schedule."""
+    number of candidates varies with the schedule; so does taking the work
+    items of each query's `front` nearest lists first."""
     ix, index, query, base = zipf16
     ix.set_option("fast_capacity", 1 << 16)
     ix.set_option("profile", 1)
@@ -333,13 +335,20 @@ def test_fast_running_bound_changes_nothing(mivf, zipf16):
         ref_ids, ref_dis = ix.search_fast(query, 32, 256, 100, base)
         ref_c = ix.fast_stats()[0]
         ix.set_option("fast_running_bound", 1)
+        ix.set_option("fast_front_ranks", front)
         cands = []
         for _ in range(4):
             ids, dis = ix.search_fast(query, 32, 256, 100, base)
             cands.append(ix.fast_stats()[0])
             assert np.array_equal(ids, ref_ids) and np.array_equal(dis.view(np.uint32), ref_dis.view(np.uint32))
+        # a wider batch after a narrower one: the two grouping ranges fill the work items anew
+        n_ids, n_dis = ix.search_fast(query[:37], 7, 64, 10, base)
+        ix.set_option("fast_running_bound", 0)
+        r_ids, r_dis = ix.search_fast(query[:37], 7, 64, 10, base)
+        assert np.array_equal(n_ids, r_ids) and np.array_equal(n_dis.view(np.uint32), r_dis.view(np.uint32))
     finally:
         ix.set_option("fast_running_bound", 1)
+        ix.set_option("fast_front_ranks", 0)
         ix.set_option("fast_capacity", 16384)
         ix.set_option("profile", 0)
     assert max(cands) <= ref_c
