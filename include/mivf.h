@@ -192,8 +192,10 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
                      void *stream);
 /* Candidates kept by the last profiled mivf_search_fast, and the number of
  * queries whose candidates exceeded the per-query capacity
- * max("fast_capacity" (default 16384), 64 kl); the excess is dropped, which
- * lowers recall and makes that query's shortlist depend on arrival order.  */
+ * max("fast_capacity" (default 65536), 64 kl); the excess is dropped, which
+ * lowers recall and makes that query's shortlist depend on arrival order.
+ * The buffer takes nq x capacity x 8 bytes of HBM (5.2 GB at 10k queries
+ * and the default); only the kept candidates are written.                 */
 int mivf_fast_stats(mivf_index *ix, int64_t *candidates, int64_t *overflowed_queries);
 
 /* The 8-bit tables of mivf_search_fast for queries d_q[0..n-1] (n <= 16)
@@ -250,6 +252,8 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * scan takes the work items of each query's n nearest lists first, so that
  * their candidates set the running bounds early; results unchanged; default
  * 0: the extra, less full work items cost more than the candidates saved),
+ * "item_order" (1 = the list-major kernels number their work items longest
+ * list first; a schedule only, results unchanged; default 0),
  * "serve_min_fill" and "serve_max_wait_us" (servers over this index: while
  * a micro-batch is in
  * flight, a free slot launches only once serve_min_fill queries wait or the

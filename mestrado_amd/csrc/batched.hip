@@ -109,7 +109,8 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
         MIVF_CHECK_HIP(hipStreamWaitEvent(sp, x.fork, 0));
     }
     MIVF_CHECK_HIP(launch_pairs(probes, nq, wmax, 0, wmax, ix->off.as<int64_t>(), nullptr, nlist, cfg.g,
-                                x.cnt.as<int>(), x.poff.as<int>(), x.ioff.as<int>(), ctrl, nullptr, x.fill.as<int>(),
+                                x.cnt.as<int>(), x.poff.as<int>(), x.ioff.as<int>(), ctrl, nullptr,
+                                ix->item_order ? ix->lorder.as<int32_t>() : nullptr, x.fill.as<int>(),
                                 x.pairq.as<int32_t>(), x.items.as<FastItem>(), sp));
     MIVF_CHECK_HIP(launch_fill_i32(x.ccnt.as<int>(), nq, 0, sp));
     MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, sp));

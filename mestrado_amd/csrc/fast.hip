@@ -658,45 +658,54 @@ __global__ __launch_bounds__(256) void pair_scatter_global_kernel(const int32_t 
         pair_q[pair_off[l] + atomicAdd(fill + l, 1)] = int32_t(e / w);
 }
 
-// exclusive scans of pair counts and of work items per list (one workgroup);
-// items are numbered from *item_base (0 if null), and *total_items becomes
-// the end of the numbering (item_base may equal total_items)
+// exclusive scans (one workgroup) of the pair counts in list order
+// (pair_off[l]) and of the work items per list in the order lorder gives
+// (item_off[p] for the list lorder[p]; list order if lorder is null); items
+// are numbered from *item_base (0 if null), and *total_items becomes the end
+// of the numbering (item_base may equal total_items)
 __global__ void pair_scan_kernel(const int *__restrict__ cnt, int nlist, int G, int *__restrict__ pair_off,
                                  int *__restrict__ item_off, int *total_items, int *__restrict__ fill,
-                                 const int *item_base) {
-    __shared__ int part_p[1024], part_i[1024];
-    const int tid = threadIdx.x;
+                                 const int *item_base, const int32_t *__restrict__ lorder) {
+    __shared__ int wsum_p[16], wsum_i[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int base = item_base ? *item_base : 0;  // read by every thread before the first barrier
     const int per = (nlist + 1023) / 1024;
     int sp = 0, si = 0;
     for (int j = tid * per; j < min(nlist, (tid + 1) * per); ++j) {
         sp += cnt[j];
-        si += (cnt[j] + G - 1) / G;
+        si += (cnt[lorder ? lorder[j] : j] + G - 1) / G;
     }
-    part_p[tid] = sp;
-    part_i[tid] = si;
-    __syncthreads();
-    if (tid == 0) {
-        int ap = 0, ai = base;
-        for (int i = 0; i < 1024; ++i) {
-            const int tp = part_p[i], ti = part_i[i];
-            part_p[i] = ap;
-            part_i[i] = ai;
-            ap += tp;
-            ai += ti;
+    // exclusive block scan of (sp, si): inclusive wave scans, then the wave totals
+    int ip = sp, ii = si;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int op = __shfl_up(ip, off, 64), oi = __shfl_up(ii, off, 64);
+        if (lane >= off) {
+            ip += op;
+            ii += oi;
         }
-        pair_off[nlist] = ap;
-        item_off[nlist] = ai;
-        *total_items = ai;
+    }
+    if (lane == 63) {
+        wsum_p[wave] = ip;
+        wsum_i[wave] = ii;
     }
     __syncthreads();
-    int ap = part_p[tid], ai = part_i[tid];
+    int ap = ip - sp, ai = base + ii - si;
+    for (int v = 0; v < wave; ++v) {
+        ap += wsum_p[v];
+        ai += wsum_i[v];
+    }
+    if (tid == 1023) {
+        pair_off[nlist] = ap + sp;
+        item_off[nlist] = ai + si;
+        *total_items = ai + si;
+    }
     for (int j = tid * per; j < min(nlist, (tid + 1) * per); ++j) {
         pair_off[j] = ap;
         item_off[j] = ai;
         fill[j] = 0;
         ap += cnt[j];
-        ai += (cnt[j] + G - 1) / G;
+        ai += (cnt[lorder ? lorder[j] : j] + G - 1) / G;
     }
 }
 
@@ -728,25 +737,25 @@ __global__ __launch_bounds__(256) void pair_scatter_kernel(const int32_t *__rest
 }
 
 // One thread per (work item, query slot); items are numbered list by list
-// (item_off) from item_off[0] (the range's first item), so a binary search
-// over item_off finds the item's list.
+// in lorder's order (item_off by position) from item_off[0] (the range's
+// first item), so a binary search over item_off finds the item's list.
 __global__ void item_expand_kernel(const int *__restrict__ pair_off, const int *__restrict__ item_off,
                                    const int64_t *__restrict__ list_off, const int32_t *__restrict__ pair_q,
                                    const float *__restrict__ tau, int nlist, int G,
                                    const int *__restrict__ total_items, int64_t max_slots,
-                                   FastItem *__restrict__ items) {
+                                   const int32_t *__restrict__ lorder, FastItem *__restrict__ items) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= max_slots) return;
     const int it = int(e / FG) + item_off[0], slot = int(e % FG);  // FG threads per record, slots >= G empty
     if (it >= *total_items) return;
-    int lo = 0, hi = nlist;  // last list with item_off[l] <= it (lists without items are skipped)
+    int lo = 0, hi = nlist;  // last position with item_off[p] <= it (lists without items are skipped)
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (item_off[mid] <= it) lo = mid;
         else hi = mid;
     }
-    const int l = lo;
-    const int j = it - item_off[l];
+    const int l = lorder ? lorder[lo] : lo;
+    const int j = it - item_off[lo];
     const int pb = pair_off[l] + j * G;
     const int gc = min(G, pair_off[l + 1] - pb);
     FastItem &x = items[it];
@@ -862,10 +871,12 @@ __device__ float seed_table(const float *__restrict__ queries, const float *__re
 
 constexpr int SCAP = 8192;  // seed keys held in LDS (at most)
 
-// SC: keys held in LDS.  A capped seed (cap <= 1024) runs with SC = 1024,
-// a third of the LDS, so that several more blocks share a CU.
+// SC: keys held in LDS.  A capped seed runs with the smallest SC that holds
+// the cap, so that more blocks share a CU; with SC = 1024 the launch bound
+// (6 waves per SIMD, at most 80 VGPRs) lets six of them run at once (larger
+// key buffers hold the CU to four blocks or fewer through their LDS).
 template <int M, int SC>
-__global__ __launch_bounds__(256) void seed_tau_kernel(const float *__restrict__ queries,
+__global__ __launch_bounds__(256, (SC <= 1024 ? 6 : 4)) void seed_tau_kernel(const float *__restrict__ queries,
                                                        const float *__restrict__ coarse,
                                                        const float *__restrict__ pq, const float *__restrict__ pnorm,
                                                        const float *__restrict__ pmax,
@@ -1208,7 +1219,8 @@ hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t s) {
 
 hipError_t launch_pairs(const int32_t *probes, int nq, int w, int rlo, int rhi, const int64_t *list_off,
                         const float *tau, int nlist, int G, int *cnt, int *pair_off, int *item_off, int *total_items,
-                        const int *item_base, int *fill, int32_t *pair_q, FastItem *items, hipStream_t s) {
+                        const int *item_base, const int32_t *lorder, int *fill, int32_t *pair_q, FastItem *items,
+                        hipStream_t s) {
     if (nlist < 1 || nlist > kMaxPairLists || G < 1 || G > FG) return hipErrorInvalidValue;
     rlo = max(rlo, 0);
     rhi = min(rhi, w);
@@ -1219,7 +1231,7 @@ hipError_t launch_pairs(const int32_t *probes, int nq, int w, int rlo, int rhi, 
         pair_count_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rlo, rhi, nlist, cnt);
     else
         pair_count_global_kernel<<<nblk(np, 256), 256, 0, s>>>(probes, nq, w, rlo, rhi, nlist, cnt);
-    pair_scan_kernel<<<1, 1024, 0, s>>>(cnt, nlist, G, pair_off, item_off, total_items, fill, item_base);
+    pair_scan_kernel<<<1, 1024, 0, s>>>(cnt, nlist, G, pair_off, item_off, total_items, fill, item_base, lorder);
     if (nlist <= LBINS)
         pair_scatter_kernel<<<nblk(np, PCH), 256, 0, s>>>(probes, nq, w, rlo, rhi, nlist, pair_off, fill, pair_q);
     else
@@ -1227,7 +1239,7 @@ hipError_t launch_pairs(const int32_t *probes, int nq, int w, int rlo, int rhi, 
                                                                  pair_q);
     const int64_t max_slots = (npr / G + nlist) * FG;  // >= FG * items of the range
     item_expand_kernel<<<nblk(max_slots, 256), 256, 0, s>>>(pair_off, item_off, list_off, pair_q, tau, nlist, G,
-                                                          total_items, max_slots, items);
+                                                          total_items, max_slots, lorder, items);
     return hipGetLastError();
 }
 
@@ -1243,10 +1255,15 @@ hipError_t launch_seed_tau(const SeedArgs &a, int m, hipStream_t s) {
     if (a.kl < 1) return hipErrorInvalidValue;
     // a cap must hold the kl-th key: cap >= kl (per-query klq[q] are <= kl)
     if (a.cap > 0 && (a.cap < a.kl || a.cap > SCAP)) return hipErrorInvalidValue;
-    const bool small = a.cap > 0 && a.cap <= 1024;
-    switch (m) {
-        case 8: return small ? launch_seed_m<8, 1024>(a, s) : launch_seed_m<8, SCAP>(a, s);
-        case 16: return small ? launch_seed_m<16, 1024>(a, s) : launch_seed_m<16, SCAP>(a, s);
+    // the smallest key buffer that holds the cap: less LDS, more blocks per CU
+    const int sc = (a.cap > 0 && a.cap <= 1024) ? 1024 : (a.cap > 0 && a.cap <= 4096) ? 4096 : SCAP;
+    switch (m * 10000 + sc) {
+        case 81024: return launch_seed_m<8, 1024>(a, s);
+        case 84096: return launch_seed_m<8, 4096>(a, s);
+        case 88192: return launch_seed_m<8, SCAP>(a, s);
+        case 161024: return launch_seed_m<16, 1024>(a, s);
+        case 164096: return launch_seed_m<16, 4096>(a, s);
+        case 168192: return launch_seed_m<16, SCAP>(a, s);
         default: return hipErrorInvalidValue;
     }
 }

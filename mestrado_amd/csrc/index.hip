@@ -45,6 +45,16 @@ int upload_lists(mivf_index *ix) {
         MIVF_CHECK_HIP(hipMemcpy(ix->ids.p, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
         MIVF_CHECK_HIP(hipMemcpy(ix->codes.p, codes.data(), codes.size(), hipMemcpyHostToDevice));
     }
+    // the lists by decreasing length (ties by index): the order in which the
+    // list-major kernels number their work items when option item_order = 1
+    {
+        std::vector<int32_t> order(ix->nlist);
+        for (int l = 0; l < ix->nlist; ++l) order[l] = l;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t a, int32_t b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+        MIVF_TRY(ix->lorder.ensure(sizeof(int32_t) * ix->nlist));
+        MIVF_CHECK_HIP(hipMemcpy(ix->lorder.p, order.data(), sizeof(int32_t) * ix->nlist, hipMemcpyHostToDevice));
+    }
     ix->h_off = off;
     ix->ntotal = n;
     ix->dirty = false;
@@ -944,15 +954,16 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     // their candidates set the queries' running bounds before the other
     // lists are scanned (the order changes no result)
     const int front = (ix->fast_running && ix->fast_front > 0 && ix->fast_front < w) ? ix->fast_front : 0;
+    const int32_t *lorder = ix->item_order ? ix->lorder.as<int32_t>() : nullptr;
     MIVF_CHECK_HIP(launch_pairs(ix->ws.probes.as<int32_t>(), nq, w, 0, front > 0 ? front : w, ix->off.as<int64_t>(),
                                 nullptr, nlist, fast_group_size(), ix->f_cnt.as<int>(), ix->f_poff.as<int>(),
-                                ix->f_ioff.as<int>(), ctrl, nullptr, ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(),
-                                ix->f_items.as<FastItem>(), sa));
+                                ix->f_ioff.as<int>(), ctrl, nullptr, lorder, ix->f_fill.as<int>(),
+                                ix->f_pairq.as<int32_t>(), ix->f_items.as<FastItem>(), sa));
     if (front > 0)
         MIVF_CHECK_HIP(launch_pairs(ix->ws.probes.as<int32_t>(), nq, w, front, w, ix->off.as<int64_t>(), nullptr,
                                     nlist, fast_group_size(), ix->f_cnt.as<int>(), ix->f_poff.as<int>(),
-                                    ix->f_ioff.as<int>(), ctrl, ctrl, ix->f_fill.as<int>(), ix->f_pairq.as<int32_t>(),
-                                    ix->f_items.as<FastItem>(), sa));
+                                    ix->f_ioff.as<int>(), ctrl, ctrl, lorder, ix->f_fill.as<int>(),
+                                    ix->f_pairq.as<int32_t>(), ix->f_items.as<FastItem>(), sa));
     MIVF_CHECK_HIP(launch_fill_i32(ix->f_ccnt.as<int>(), nq, 0, sa));
     MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, sa));
     // running bounds (+inf) and their histograms (0)
@@ -1220,6 +1231,9 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     } else if (n == "fast_tables") {
         MIVF_REQUIRE(value >= 0 && value <= 2, kInvalid, "fast_tables must be 0, 1 or 2");
         ix->fast_tables = int(value);
+    } else if (n == "item_order") {
+        MIVF_REQUIRE(value == 0 || value == 1, kInvalid, "item_order must be 0 (list order) or 1 (longest lists first)");
+        ix->item_order = int(value);
     } else if (n == "fast_front_ranks") {
         MIVF_REQUIRE(value >= 0 && value <= 1024, kInvalid, "fast_front_ranks must be in 0..1024");
         ix->fast_front = int(value);

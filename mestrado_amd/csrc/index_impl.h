@@ -130,6 +130,8 @@ struct mivf_index {
     bool rows_u8 = false;    // rows holds bytes (option base_bytes: integer-valued data 0..255)
     bool base_bytes = false; // option base_bytes: attach_base keeps a byte copy
     int fast_front = 0;      // option fast_front_ranks: probe ranks whose work items the 8-bit scan takes first
+    int item_order = 0;      // option item_order: 1 = list-major work items numbered longest list first
+    DevBuf lorder;           // [nlist] lists by decreasing length (set by upload_lists)
     // batched reduced-precision form scratch
     DevBuf f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_items, f_ctrl, f_cand, f_ccnt,
         f_short, f_ns, f_rthr, f_rhist;
@@ -143,7 +145,7 @@ struct mivf_index {
     int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
     int fast_seed_cap = 1024; // vectors entering the seed bound (0 = all of the seed lists)
     int exact_seed_cap = 4096; // the same for the list-major exact kernels
-    int fast_cap = 16384;     // candidate capacity per query of the reduced-precision form
+    int fast_cap = 65536;     // candidate capacity per query of the reduced-precision form
     int batched_cap = 16384;  // candidate capacity per query of the batched exact form (>= 2 kmax)
     int batched_cap_pp = 256; // ... and at least this many per probe
     int batched_g = 4, batched_nt = 512;  // its work-item shape (queries per item, threads)

@@ -223,13 +223,15 @@ hipError_t launch_seed_count(const int32_t *probes, int nq, int w, const int64_t
 hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t s);
 // (list, query) pairs of probe ranks in [rlo, rhi), grouped by list, and the
 // work items (list, first pair) of at most G = 1..16 pairs, numbered from
-// *item_base (0 if item_base is null); *total_items becomes the end of the
-// numbering (item_base may point at total_items, to append a second range).
-// No memset: the counters are cleared by kernels, so the sequence can be
-// captured in a graph.
+// *item_base (0 if item_base is null) list by list in the order lorder gives
+// (a permutation of the lists; list order if null); *total_items becomes the
+// end of the numbering (item_base may point at total_items, to append a
+// second range).  No memset: the counters are cleared by kernels, so the
+// sequence can be captured in a graph.
 hipError_t launch_pairs(const int32_t *probes, int nq, int w, int rlo, int rhi, const int64_t *list_off,
                         const float *tau, int nlist, int G, int *cnt, int *pair_off, int *item_off, int *total_items,
-                        const int *item_base, int *fill, int32_t *pair_q, FastItem *items, hipStream_t s);
+                        const int *item_base, const int32_t *lorder, int *fill, int32_t *pair_q, FastItem *items,
+                        hipStream_t s);
 // Batched exact search (exact.hip).
 struct ExactArgs {
     const float *queries;       // [nq][d]

@@ -252,6 +252,119 @@ __global__ __launch_bounds__(NT) void rerank_kernel(const T *__restrict__ base, 
     sort_and_emit(buf, c, k, out_ids + int64_t(q) * k, out_dis + int64_t(q) * k);
 }
 
+// The same rerank for shortlists of at most NT (one row per thread): the
+// shortlist entry, the query and its norm are loaded together, a 128-byte
+// row (bytes, d = 128) is loaded as soon as its position is known (MODE 0
+// and 2: the row is the entry itself) while the id is still on its way, and
+// each key then goes straight to its rank (the count of smaller keys, equal
+// keys ordered by position).
+// Byte rows against a query whose values are all integers in 0..255 (checked
+// per query): every product and partial sum of |x|^2 and x.q is then an
+// integer below 128 * 255^2 < 2^24, so the sequential FP32 sums are exact
+// and equal the integer sums, which v_dot4_u32_u8 forms four bytes at a time.
+template <int MODE, typename T = float>
+__global__ __launch_bounds__(NT) void rerank_small_kernel(const T *__restrict__ base, int64_t nbase,
+                                                          const int32_t *__restrict__ list_ids,
+                                                          const float *__restrict__ queries, int d,
+                                                          const double *__restrict__ qnorm,
+                                                          const int32_t *__restrict__ short_ids,
+                                                          int kl, int k, int32_t *__restrict__ out_ids,
+                                                          float *__restrict__ out_dis) {
+    __shared__ uint64_t cand[NT];
+    __shared__ float qv[512];
+    __shared__ uint32_t qb[32];  // the query as bytes (d = 128, integer-valued queries)
+    __shared__ double qn;
+    const int tid = threadIdx.x;
+    const int q = blockIdx.x;
+    const int32_t e = tid < kl ? short_ids[int64_t(q) * kl + tid] : -1;
+    bool qint = true;
+    for (int t = tid; t < d; t += NT) {
+        const float v = queries[int64_t(q) * d + t];
+        qv[t] = v;
+        qint = qint && v >= 0.0f && v <= 255.0f && v == __builtin_truncf(v);
+        if (t < 128) reinterpret_cast<uint8_t *>(qb)[t] = uint8_t(qint ? v : 0.0f);
+    }
+    if (tid == 0 && qnorm) qn = qnorm[q];
+    // whole byte rows of 128 values in registers (eight 16-byte loads)
+    constexpr bool kRow = sizeof(T) == 1 && MODE != 1;
+    const bool whole = kRow && d == 128 && (reinterpret_cast<uintptr_t>(base) & 15u) == 0;
+    uint4 pre[8];
+    if (whole && e >= 0 && int64_t(e) < nbase) {
+        const uint4 *x16 = reinterpret_cast<const uint4 *>(reinterpret_cast<const uint8_t *>(base) + int64_t(e) * 128);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) pre[j] = x16[j];
+    }
+    int32_t id = e;
+    if (MODE != 0 && e >= 0) id = list_ids[e];
+    const int64_t row = MODE == 1 ? int64_t(id) : int64_t(e);
+    const bool ibytes = __syncthreads_and(qint) && whole;  // qv, qb and qn are ready
+    if (!qnorm) {
+        if (tid == 0) {
+            double s = 0.0;
+            for (int t = 0; t < d; ++t) {
+                const float p = qv[t] * qv[t];
+                s = s + double(p);
+            }
+            qn = s;
+        }
+        __syncthreads();
+    }
+    uint64_t key = kNoKey;
+    if (e >= 0 && id >= 0 && row < nbase) {
+        float na = 0.0f, dot = 0.0f;
+        if (ibytes) {
+            uint32_t ina = 0u, idot = 0u;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t wd[4] = {pre[j].x, pre[j].y, pre[j].z, pre[j].w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    ina = __builtin_amdgcn_udot4(wd[u], wd[u], ina, false);
+                    idot = __builtin_amdgcn_udot4(wd[u], qb[4 * j + u], idot, false);
+                }
+            }
+            na = float(ina);
+            dot = float(idot);
+        } else if (whole) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t wd[4] = {pre[j].x, pre[j].y, pre[j].z, pre[j].w};
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const float el = float((wd[u >> 2] >> (8 * (u & 3))) & 255u);
+                    na = na + el * el;
+                    dot = dot + el * qv[16 * j + u];
+                }
+            }
+        } else {
+            const bool vec = (d % (64 / int(sizeof(T)))) == 0 && (reinterpret_cast<uintptr_t>(base) & 15u) == 0;
+            row_sums(base + row * d, qv, d, vec, na, dot);
+        }
+        const float c = float(qn + double(na));
+        const float m2 = -2.0f * dot;
+        key = ckey(m2 + c, id);
+    }
+    cand[tid] = key;
+    const int nreal = __syncthreads_count(key != kNoKey);
+    int32_t *oi = out_ids + int64_t(q) * k;
+    float *od = out_dis + int64_t(q) * k;
+    if (key != kNoKey) {
+        int rank = 0;
+        for (int j = 0; j < kl; ++j) {
+            const uint64_t o = cand[j];
+            rank += o < key || (o == key && j < tid);
+        }
+        if (rank < k) {
+            oi[rank] = key_id(key);
+            od[rank] = key_dist(key);
+        }
+    }
+    for (int i = nreal + tid; i < k; i += NT) {
+        oi[i] = -1;
+        od[i] = __builtin_huge_valf();
+    }
+}
+
 __global__ void gather_rows_kernel(const float *__restrict__ base, int d, const int32_t *__restrict__ ids,
                                    int64_t n, float *__restrict__ out) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -304,8 +417,12 @@ hipError_t launch_rerank(const float *base, int64_t nbase, const float *queries,
                          int32_t *out_ids, float *out_dis, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     if (kl < 1 || kl > KBUF || k < 1 || k > KBUF || d > 512) return hipErrorInvalidValue;
-    rerank_kernel<0><<<unsigned(nq), NT, 0, s>>>(base, nbase, nullptr, queries, d, nullptr, short_ids, kl, k,
-                                                 out_ids, out_dis);
+    if (kl <= NT)
+        rerank_small_kernel<0><<<unsigned(nq), NT, 0, s>>>(base, nbase, nullptr, queries, d, nullptr, short_ids, kl,
+                                                           k, out_ids, out_dis);
+    else
+        rerank_kernel<0><<<unsigned(nq), NT, 0, s>>>(base, nbase, nullptr, queries, d, nullptr, short_ids, kl, k,
+                                                     out_ids, out_dis);
     return hipGetLastError();
 }
 
@@ -314,9 +431,15 @@ hipError_t launch_rerank_pos(const float *rows, int64_t nrows, bool list_ordered
                              int kl, int k, int32_t *out_ids, float *out_dis, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     if (kl < 1 || kl > KBUF || k < 1 || k > KBUF || d > 512) return hipErrorInvalidValue;
-    if (list_ordered)
+    if (list_ordered && kl <= NT)
+        rerank_small_kernel<2><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm, short_pos, kl,
+                                                           k, out_ids, out_dis);
+    else if (list_ordered)
         rerank_kernel<2><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm, short_pos, kl, k,
                                                      out_ids, out_dis);
+    else if (kl <= NT)
+        rerank_small_kernel<1><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm, short_pos, kl,
+                                                           k, out_ids, out_dis);
     else
         rerank_kernel<1><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm, short_pos, kl, k,
                                                      out_ids, out_dis);
@@ -328,8 +451,12 @@ hipError_t launch_rerank_pos_u8(const uint8_t *rows, int64_t nrows, const int32_
                                 int32_t *out_ids, float *out_dis, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     if (kl < 1 || kl > KBUF || k < 1 || k > KBUF || d > 512) return hipErrorInvalidValue;
-    rerank_kernel<2, uint8_t><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm, short_pos, kl, k,
-                                                          out_ids, out_dis);
+    if (kl <= NT)
+        rerank_small_kernel<2, uint8_t><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm,
+                                                                    short_pos, kl, k, out_ids, out_dis);
+    else
+        rerank_kernel<2, uint8_t><<<unsigned(nq), NT, 0, s>>>(rows, nrows, list_ids, queries, d, qnorm, short_pos,
+                                                              kl, k, out_ids, out_dis);
     return hipGetLastError();
 }
 

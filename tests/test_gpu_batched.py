@@ -34,6 +34,38 @@ def test_batched_equals_plain_m16(mivf, zipf16, g, nt):
     assert_same(ids, dis, rids, rdis, f"batched g={g} nt={nt}")
 
 
+@pytest.mark.parametrize("order", [0, 1])
+def test_item_order_changes_nothing(mivf, zipf16, order):
+    """Numbering the list-major work items longest list first (option
+    item_order) is a schedule only: the batched exact form, the list-major
+    mivf_search and the reduced-precision form return what they return with
+    the lists in index order, bit for bit."""
+    ix, index, query, base = zipf16
+    w, k = uneven(query.shape[0], seed=5)
+    rids, rdis = ix.search_var(query, w, k, 100)
+    ix.set_option("profile", 1)
+    ix.set_option("item_order", 0)
+    f0 = ix.search_fast(query, 24, 192, 100, base)
+    ix.set_option("item_order", order)
+    try:
+        ids, dis = ix.search_batched(query, w, k, 100)
+        assert_same(ids, dis, rids, rdis, f"batched, item order {order}")
+        ix.set_option("list_major_min_queries", 1)
+        ids, dis = ix.search(query, 32, 100)
+        ix.set_option("list_major", 0)
+        pids, pdis = ix.search(query, 32, 100)
+        assert_same(ids, dis, pids, pdis, f"list-major search, item order {order}")
+        ix.set_option("list_major", 1)
+        f1 = ix.search_fast(query, 24, 192, 100, base)
+        assert ix.fast_stats()[1] == 0
+        assert_same(f1[0], f1[1], f0[0], f0[1], f"reduced precision, item order {order}")
+    finally:
+        ix.set_option("item_order", 0)
+        ix.set_option("list_major", 1)
+        ix.set_option("list_major_min_queries", 2048)
+        ix.set_option("profile", 0)
+
+
 def test_batched_overflow_fallback(mivf, zipf16):
     """A tiny candidate buffer overflows for most queries; the plain rerun of
     those queries keeps the results exact."""

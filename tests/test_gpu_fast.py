@@ -59,7 +59,7 @@ def test_fast_exact_rows_m16(mivf, orc, zipf16, zipf16_gt, w, kl, k, seed_lists)
         ids2, dis2 = ix.search_fast(query, w, kl, k, base)
     finally:
         ix.set_option("fast_seed_lists", 1)
-        ix.set_option("fast_capacity", 16384)
+        ix.set_option("fast_capacity", 1 << 16)  # the default
         ix.set_option("profile", 0)
     assert over == 0, f"{over} queries overflowed the candidate buffer"
     assert np.array_equal(ids, ids2) and np.array_equal(dis.view(np.uint32), dis2.view(np.uint32)), \
@@ -197,8 +197,10 @@ def test_fast_attached_base(mivf, orc, c1):
 def test_fast_byte_base(mivf, orc, c1, zipf16, which, w, kl, k):
     """Option base_bytes: the attached copy of an integer-valued base holds one
     byte per value and the rerank returns the same ids and distances, bit for
-    bit, as with the float copy; a base with a value that is not an integer
-    in 0..255 is refused and leaves the index searchable."""
+    bit, as with the float copy, for integer-valued queries (integer dot
+    products) and for queries that are not (FP32 sums); a base with a value
+    that is not an integer in 0..255 is refused and leaves the index
+    searchable."""
     if which == "c1":
         q = c1["gold_q"]
         base, query = c1["base"], c1["query"]
@@ -207,13 +209,28 @@ def test_fast_byte_base(mivf, orc, c1, zipf16, which, w, kl, k):
     else:
         ix, _, query, base = zipf16
     assert np.array_equal(base, np.clip(np.rint(base), 0, 255)), "the fixture base is integer-valued"
+    ix.set_option("profile", 1)  # candidate statistics of every call
+    qf = query.astype(np.float32).copy()
+    qf[::2] += np.float32(0.375)       # every other query is not integer-valued
+    qf[1::4, 5] = np.float32(256.0)    # and some are integers outside 0..255
     try:
         ix.attach_base(base)
         f_ids, f_dis = ix.search_fast(query, w, kl, k)
+        assert ix.fast_stats()[1] == 0, "a query overflowed its candidate buffer (its shortlist is incomplete)"
+        ff_ids, ff_dis = ix.search_fast(qf, w, kl, k)
         ix.set_option("base_bytes", 1)
         ix.attach_base(base)
         b_ids, b_dis = ix.search_fast(query, w, kl, k)
+        assert ix.fast_stats()[1] == 0
         assert np.array_equal(f_ids, b_ids) and np.array_equal(f_dis.view(np.uint32), b_dis.view(np.uint32))
+        bf_ids, bf_dis = ix.search_fast(qf, w, kl, k)
+        assert np.array_equal(ff_ids, bf_ids) and np.array_equal(ff_dis.view(np.uint32), bf_dis.view(np.uint32))
+        # the byte-row distances are the oracle's FP32 arithmetic, bit for bit
+        for qs, ids_, dis_ in ((query, b_ids, b_dis), (qf, bf_ids, bf_dis)):
+            for q in range(0, len(qs), 7):
+                n = int((ids_[q] >= 0).sum())
+                ex = orc.cross_dist(base[ids_[q, :n]], qs[q:q + 1])[0]
+                assert np.array_equal(ex.view(np.uint32), dis_[q, :n].view(np.uint32)), f"q={q}"
         for bad_value in (0.5, -1.0, 256.0, np.nan):
             bad = base.copy()
             bad[base.shape[0] // 2, 7] = bad_value
@@ -227,6 +244,7 @@ def test_fast_byte_base(mivf, orc, c1, zipf16, which, w, kl, k):
         assert np.array_equal(f_ids, a_ids) and np.array_equal(f_dis.view(np.uint32), a_dis.view(np.uint32))
     finally:
         ix.set_option("base_bytes", 0)
+        ix.set_option("profile", 0)
 
 
 @pytest.mark.parametrize("which,kl,seed_lists,cap", [("m16", 256, 1, 0), ("m16", 64, 2, 0), ("m8", 128, 1, 0),
@@ -349,7 +367,7 @@ def test_fast_running_bound_changes_nothing(mivf, zipf16, front):
     finally:
         ix.set_option("fast_running_bound", 1)
         ix.set_option("fast_front_ranks", 0)
-        ix.set_option("fast_capacity", 16384)
+        ix.set_option("fast_capacity", 1 << 16)  # the default
         ix.set_option("profile", 0)
     assert max(cands) <= ref_c
 
@@ -420,7 +438,7 @@ def test_fast_var_tables_exact_rows(mivf, orc, zipf16, zipf16_gt, tables, w, kl,
         ids2, dis2 = ix.search_fast(query, w, kl, k, base)
     finally:
         ix.set_option("fast_tables", 0)
-        ix.set_option("fast_capacity", 16384)
+        ix.set_option("fast_capacity", 1 << 16)  # the default
         ix.set_option("profile", 0)
     assert over == 0
     assert np.array_equal(ids, ids2) and np.array_equal(dis.view(np.uint32), dis2.view(np.uint32)), "not deterministic"
