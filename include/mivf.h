@@ -228,7 +228,7 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * >= 2048 queries grouped by nearest list; schedule only, default 1),
  * "fast_seed_lists" (lists per query scanned exactly to seed
  * mivf_search_fast, default 1), "fast_seed_cap" (vectors of those lists
- * that enter the seed bound, default 1024, 0 = all), "fast_capacity",
+ * that enter the seed bound, default 2048, 0 = all), "fast_capacity",
  * "fast_running_bound" (1 =
  * the 8-bit scan tightens each query's filter as its candidates accumulate;
  * results unchanged, default 1), "batched_group", "batched_threads",
@@ -254,6 +254,10 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * 0: the extra, less full work items cost more than the candidates saved),
  * "item_order" (1 = the list-major kernels number their work items longest
  * list first; a schedule only, results unchanged; default 0),
+ * "exact_running_bound" (1 = the list-major exact scan lowers each query's
+ * bound as its kept distances accumulate, from histograms shared by the
+ * workgroups; results unchanged; default 0: fewer candidates, but the
+ * histogram traffic costs the scan more than the select saves),
  * "serve_min_fill" and "serve_max_wait_us" (servers over this index: while
  * a micro-batch is in
  * flight, a free slot launches only once serve_min_fill queries wait or the

@@ -26,7 +26,7 @@ struct XCfg {
 
 struct XWs {
     Ws ws;
-    DevBuf wq, kq, ns, tau, cnt, poff, ioff, fill, pairq, items, ctrl, cand, ccnt, ovf;
+    DevBuf wq, kq, ns, tau, cnt, poff, ioff, fill, pairq, items, ctrl, cand, ccnt, ovf, runthr, rhist;
     // side stream of the pair grouping (runs beside the seed scan)
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
@@ -96,8 +96,29 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     MIVF_TRY(x.cand.ensure(sizeof(uint64_t) * size_t(nq) * capq));
     MIVF_TRY(x.ccnt.ensure(sizeof(int) * nq));
     MIVF_TRY(x.ovf.ensure(sizeof(int) * (size_t(nq) + 1)));
+    const bool running = ix->exact_running;
+    if (running) {
+        MIVF_TRY(x.runthr.ensure(sizeof(float) * nq));
+        MIVF_TRY(x.rhist.ensure(sizeof(int) * size_t(nq) * kExactBins));
+    }
     int32_t *probes = x.ws.probes.as<int32_t>();
     int *ctrl = x.ctrl.as<int>();  // [0] total items, [1] next item
+    {
+        // candidate counts, the work counter and the overflow flags start
+        // cleared; the stream's order puts this after every earlier search
+        // with this workspace and before the coarse stage
+        FillSet f{};
+        f.p[0] = x.ccnt.as<int>(), f.n[0] = nq, f.v[0] = 0;
+        f.p[1] = ctrl + 1, f.n[1] = 1, f.v[1] = 0;
+        f.p[2] = x.ovf.as<int>(), f.n[2] = int64_t(nq) + 1, f.v[2] = 0;
+        f.count = 3;
+        if (running) {  // running bounds (+inf) and their histograms (0)
+            f.p[3] = x.runthr.as<int>(), f.n[3] = nq, f.v[3] = 0x7f800000;
+            f.p[4] = x.rhist.as<int>(), f.n[4] = int64_t(nq) * kExactBins, f.v[4] = 0;
+            f.count = 5;
+        }
+        MIVF_CHECK_HIP(launch_fill_multi(f, st));
+    }
 
     MIVF_TRY(coarse_stage(ix, x.ws, d_q, nq, wmax, probes, x.ws.coadis.as<float>(), st, nullptr));
     if (tm) MIVF_TRY(tm->mark());
@@ -112,9 +133,6 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
                                 x.cnt.as<int>(), x.poff.as<int>(), x.ioff.as<int>(), ctrl, nullptr,
                                 ix->item_order ? ix->lorder.as<int32_t>() : nullptr, x.fill.as<int>(),
                                 x.pairq.as<int32_t>(), x.items.as<FastItem>(), sp));
-    MIVF_CHECK_HIP(launch_fill_i32(x.ccnt.as<int>(), nq, 0, sp));
-    MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, sp));
-    MIVF_CHECK_HIP(launch_fill_i32(x.ovf.as<int>(), nq + 1, 0, sp));
     if (overlap) MIVF_CHECK_HIP(hipEventRecord(x.join, sp));
     MIVF_CHECK_HIP(launch_seed_count(probes, nq, wmax, ix->off.as<int64_t>(), nlist, kmax, x.kq.as<int32_t>(), 1,
                                      x.ns.as<int32_t>(), st));
@@ -156,6 +174,10 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     a.ks = ix->ks;
     a.err = ix->errflag.as<int>();
     a.tau = x.tau.as<float>();
+    a.runthr = running ? x.runthr.as<float>() : nullptr;
+    a.rhist = running ? x.rhist.as<int>() : nullptr;
+    a.kq = x.kq.as<int32_t>();
+    a.kuni = kmax;
     const int per_cu = std::max(1, int((160 * 1024) / exact_lds_bytes(ix->m, cfg.g)));
     if (tm) MIVF_TRY(tm->mark());
     MIVF_CHECK_HIP(launch_exact_scan(a, ix->m, cfg.g, cfg.nt, ix->ncu * per_cu, st));
@@ -320,7 +342,7 @@ int search_list_major(mivf_index *ix, const float *d_q, int nq, int w, int k, in
 std::shared_ptr<void> new_list_major_ws(hipStream_t st) {
     auto x = std::make_shared<XWs>();
     x->ws.order_on(st);
-    for (DevBuf *b : {&x->wq, &x->kq, &x->ns, &x->tau, &x->cnt, &x->poff, &x->ioff, &x->fill, &x->pairq, &x->items,
+    for (DevBuf *b : {&x->runthr, &x->rhist, &x->wq, &x->kq, &x->ns, &x->tau, &x->cnt, &x->poff, &x->ioff, &x->fill, &x->pairq, &x->items,
                       &x->ctrl, &x->cand, &x->ccnt, &x->ovf})
         b->ord = st;
     return x;

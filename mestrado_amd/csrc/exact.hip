@@ -42,8 +42,12 @@ struct XLay {
     static constexpr size_t pcnt = nrec + sizeof(FastItem);         // int [4] rotating append counters
     static constexpr size_t item = pcnt + 16;                       // int
     static constexpr size_t nidx = item + 4;                        // int
-    static constexpr size_t total = nidx + 8;
+    static constexpr size_t hsc = nidx + 12;                        // float [16] histogram scale kExactBins / tau
+    static constexpr size_t lhist = hsc + 64;                       // int [G][kExactBins] flush histogram
+    static constexpr size_t total = lhist + size_t(G) * kExactBins * 4;
 };
+// two workgroups of the G = 4 shape share a CU's 160 KiB
+static_assert(2 * XLay<16, 4>::total <= 160 * 1024, "two workgroups per CU");
 
 template <int M>
 __device__ inline void xload_codes(const uint8_t *lc, int v, uint32_t (&cw)[M / 4]) {
@@ -58,6 +62,13 @@ __device__ inline void xload_codes(const uint8_t *lc, int v, uint32_t (&cw)[M / 
 
 // Staged candidates -> per-query global buffers as (distance, id) keys.
 // Entry: x = list position | g << 28, y = distance bits.
+// With running bounds, each flush also adds its distances to the queries'
+// histograms (bin b holds [b, b + 1) tau_q / kExactBins) and lowers runthr[q]
+// to the upper edge of the first bin whose cumulative count reaches k_q: at
+// least k_q kept vectors are then within it, so a vector beyond it cannot be
+// among the query's k best (by distance, then id: equal distances stay).
+// Counts read back may miss concurrent additions; a smaller count only gives
+// a larger, still valid, bound.
 template <int M, int G, int NT>
 __device__ void xflush(const ExactArgs &a, unsigned char *smem, int64_t beg, int filled) {
     using L = XLay<M, G>;
@@ -66,9 +77,12 @@ __device__ void xflush(const ExactArgs &a, unsigned char *smem, int64_t beg, int
     int *gcnt = reinterpret_cast<int *>(smem + L::gcnt);
     int *gbase = reinterpret_cast<int *>(smem + L::gbase);
     const int *qid = reinterpret_cast<const int *>(smem + L::qid);
+    const float *hsc = reinterpret_cast<const float *>(smem + L::hsc);
+    int *lhist = reinterpret_cast<int *>(smem + L::lhist);
     const int tid = threadIdx.x;
     const int n = min(filled, POOLX);
     if (tid < 16) gcnt[tid] = 0;
+    if (tid < G * kExactBins) lhist[tid] = 0;
     lds_barrier();
     for (int i = tid; i < n; i += NT) prank[i] = uint16_t(atomicAdd(gcnt + (pool[i].x >> 28), 1));
     lds_barrier();
@@ -78,9 +92,37 @@ __device__ void xflush(const ExactArgs &a, unsigned char *smem, int64_t beg, int
         const uint2 e = pool[i];
         const int g = int(e.x >> 28);
         const int pos = gbase[g] + int(prank[i]);
-        if (pos < a.capq)
-            a.cand[int64_t(qid[g]) * a.capq + pos] =
-                ckey(__builtin_bit_cast(float, e.y), a.list_ids[beg + (e.x & 0x0FFFFFFFu)]);
+        const float dist = __builtin_bit_cast(float, e.y);
+        if (pos < a.capq) {
+            a.cand[int64_t(qid[g]) * a.capq + pos] = ckey(dist, a.list_ids[beg + (e.x & 0x0FFFFFFFu)]);
+            if (hsc[g] > 0.0f) atomicAdd(lhist + g * kExactBins + min(kExactBins - 1, int(fmaxf(dist, 0.0f) * hsc[g])), 1);
+        }
+    }
+    lds_barrier();  // the pool and the flush histograms are complete
+    if (a.runthr) {
+        if (tid < G * kExactBins) {
+            const int c = lhist[tid];
+            if (c > 0) atomicAdd(a.rhist + int64_t(qid[tid / kExactBins]) * kExactBins + (tid % kExactBins), c);
+        }
+        const int lane = tid & 63;
+        for (int g = tid >> 6; g < G; g += NT / 64) {  // one wavefront per slot
+            if (!(hsc[g] > 0.0f) || gcnt[g] == 0) continue;  // wave-uniform
+            const int q = qid[g];
+            const int kq = a.kq ? a.kq[q] : a.kuni;
+            int c = lane < kExactBins ? __hip_atomic_load(a.rhist + int64_t(q) * kExactBins + lane, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT) : 0;
+#pragma unroll
+            for (int off = 1; off < kExactBins; off <<= 1) {
+                const int o = __shfl_up(c, off, 64);
+                if (lane >= off) c += o;
+            }
+            const uint64_t reach = __ballot(lane < kExactBins && c >= kq);
+            if (reach != 0 && lane == 0) {
+                const int b = __ffsll((unsigned long long)reach) - 1;
+                const float R = (float(b + 1) + 0.01f) / hsc[g];
+                atomicMin(reinterpret_cast<int *>(a.runthr) + q, __float_as_int(R));
+            }
+        }
     }
     lds_barrier();  // the pool is free again
 }
@@ -137,39 +179,25 @@ __device__ void xtables(const ExactArgs &a, unsigned char *smem, int l) {
             for (int i = 0; i < DS; ++i) r[g][i] = R[(g0 + g) * D + s * DS + i];
             rn64[g] = double(rn[(g0 + g) * M + s]);
         }
-        // with 8-float sub-vectors, the centroid row and norm of code
-        // c0 + CPS (j + 1) are loaded while code c0 + CPS j is computed (rows
-        // of codes >= ks are clamped to a valid row and their entries never
-        // indexed); wider sub-vectors leave no registers for that
-        constexpr bool PF = DS <= 8;
-        float4 pn4[DS / 4];
-        double pnn;
-        auto load_p = [&](int c) {
-            const int cc = min(c, a.ks - 1);
-            const float4 *src = reinterpret_cast<const float4 *>(a.pq + (int64_t(s) * a.ks + cc) * DS);
-#pragma unroll
-            for (int i = 0; i < DS / 4; ++i) pn4[i] = src[i];
-            pnn = a.pq_norm[int64_t(s) * a.ks + cc];
-        };
-        if constexpr (PF) load_p(c0);
+        // (loading the next code's centroid row during this code's arithmetic
+        // gains nothing here: the phase is bound by the VALU, and the copies
+        // of the prefetched row cost instructions)
 #pragma unroll 1
         for (int j = 0; j < E; ++j) {
             const int c = c0 + CPS * j;
             float vals[GC];
-            if constexpr (!PF) load_p(c);
-            float pr[DS];
-#pragma unroll
-            for (int i = 0; i < DS / 4; ++i) {
-                pr[4 * i] = pn4[i].x;
-                pr[4 * i + 1] = pn4[i].y;
-                pr[4 * i + 2] = pn4[i].z;
-                pr[4 * i + 3] = pn4[i].w;
-            }
-            const double pnv = pnn;
-            if constexpr (PF) {
-                if (j + 1 < E) load_p(c + CPS);
-            }
             if (c < a.ks) {
+                float pr[DS];
+                const float4 *src = reinterpret_cast<const float4 *>(a.pq + (int64_t(s) * a.ks + c) * DS);
+#pragma unroll
+                for (int i = 0; i < DS / 4; ++i) {
+                    const float4 v = src[i];
+                    pr[4 * i] = v.x;
+                    pr[4 * i + 1] = v.y;
+                    pr[4 * i + 2] = v.z;
+                    pr[4 * i + 3] = v.w;
+                }
+                const double pnv = a.pq_norm[int64_t(s) * a.ks + c];
 #pragma unroll
                 for (int g = 0; g < GC; ++g) {
                     float dot = r[g][0] * pr[0];
@@ -208,6 +236,7 @@ __global__ __launch_bounds__(NT, (xwaves<M, G, NT>())) void exact_scan_kernel(Ex
     static_assert(G % 4 == 0 && G <= 16, "G must be 4, 8, 12 or 16");
     extern __shared__ __align__(16) unsigned char smem[];
     float *lim = reinterpret_cast<float *>(smem + L::lim);
+    float *hsc = reinterpret_cast<float *>(smem + L::hsc);
     int *qid = reinterpret_cast<int *>(smem + L::qid);
     int *pcnt = reinterpret_cast<int *>(smem + L::pcnt);
     int *item_sh = reinterpret_cast<int *>(smem + L::item);
@@ -238,7 +267,12 @@ __global__ __launch_bounds__(NT, (xwaves<M, G, NT>())) void exact_scan_kernel(Ex
         if (tid < 16) {
             const int q = tid < G ? rec->q[tid] : -1;
             qid[tid] = q;
-            lim[tid] = q >= 0 ? (a.tau ? a.tau[q] : rec->tau[tid]) : -__builtin_huge_valf();
+            const float tau = q >= 0 ? (a.tau ? a.tau[q] : rec->tau[tid]) : -__builtin_huge_valf();
+            // the running bound R_q (>= k_q kept vectors within it; +inf until then)
+            const float R = (q >= 0 && a.runthr) ? __hip_atomic_load(a.runthr + q, __ATOMIC_RELAXED,
+                                                                     __HIP_MEMORY_SCOPE_AGENT) : __builtin_huge_valf();
+            lim[tid] = fminf(tau, R);
+            hsc[tid] = (q >= 0 && a.runthr && tau > 0.0f && tau < 3.0e38f) ? float(kExactBins) / tau : 0.0f;
         }
         if (tid < 3) pcnt[tid] = 0;
         __syncthreads();

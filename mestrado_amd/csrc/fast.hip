@@ -604,6 +604,18 @@ __global__ void fill_i32_kernel(int *__restrict__ p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
+// up to six fills in one launch (grid-stride over their concatenation)
+__global__ void fill_multi_kernel(FillSet f) {
+    int64_t total = 0;
+    for (int j = 0; j < f.count; ++j) total += f.n[j];
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
+        int64_t r = i;
+        int j = 0;
+        while (r >= f.n[j]) r -= f.n[j++];
+        f.p[j][r] = f.v[j];
+    }
+}
+
 // ---- list-major grouping of (query, probe) pairs with probe rank in [rlo, rhi).
 // Each workgroup aggregates its PCH pairs in an LDS histogram over the lists
 // (nlist <= LBINS) and issues one global atomic per list it touches, so
@@ -872,11 +884,11 @@ __device__ float seed_table(const float *__restrict__ queries, const float *__re
 constexpr int SCAP = 8192;  // seed keys held in LDS (at most)
 
 // SC: keys held in LDS.  A capped seed runs with the smallest SC that holds
-// the cap, so that more blocks share a CU; with SC = 1024 the launch bound
+// the cap, so that more blocks share a CU; with SC <= 2048 the launch bound
 // (6 waves per SIMD, at most 80 VGPRs) lets six of them run at once (larger
 // key buffers hold the CU to four blocks or fewer through their LDS).
 template <int M, int SC>
-__global__ __launch_bounds__(256, (SC <= 1024 ? 6 : 4)) void seed_tau_kernel(const float *__restrict__ queries,
+__global__ __launch_bounds__(256, (SC <= 2048 ? 6 : 4)) void seed_tau_kernel(const float *__restrict__ queries,
                                                        const float *__restrict__ coarse,
                                                        const float *__restrict__ pq, const float *__restrict__ pnorm,
                                                        const float *__restrict__ pmax,
@@ -1217,6 +1229,18 @@ hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_fill_multi(const FillSet &f, hipStream_t s) {
+    if (f.count < 0 || f.count > 6) return hipErrorInvalidValue;
+    int64_t total = 0;
+    for (int j = 0; j < f.count; ++j) {
+        if (f.n[j] < 0) return hipErrorInvalidValue;
+        total += f.n[j];
+    }
+    if (total == 0) return hipSuccess;
+    fill_multi_kernel<<<unsigned(std::min<int64_t>(nblk(total, 256), 2048)), 256, 0, s>>>(f);
+    return hipGetLastError();
+}
+
 hipError_t launch_pairs(const int32_t *probes, int nq, int w, int rlo, int rhi, const int64_t *list_off,
                         const float *tau, int nlist, int G, int *cnt, int *pair_off, int *item_off, int *total_items,
                         const int *item_base, const int32_t *lorder, int *fill, int32_t *pair_q, FastItem *items,
@@ -1256,12 +1280,16 @@ hipError_t launch_seed_tau(const SeedArgs &a, int m, hipStream_t s) {
     // a cap must hold the kl-th key: cap >= kl (per-query klq[q] are <= kl)
     if (a.cap > 0 && (a.cap < a.kl || a.cap > SCAP)) return hipErrorInvalidValue;
     // the smallest key buffer that holds the cap: less LDS, more blocks per CU
-    const int sc = (a.cap > 0 && a.cap <= 1024) ? 1024 : (a.cap > 0 && a.cap <= 4096) ? 4096 : SCAP;
+    int sc = SCAP;
+    for (int c : {1024, 2048, 4096})
+        if (a.cap > 0 && a.cap <= c && sc == SCAP) sc = c;
     switch (m * 10000 + sc) {
         case 81024: return launch_seed_m<8, 1024>(a, s);
+        case 82048: return launch_seed_m<8, 2048>(a, s);
         case 84096: return launch_seed_m<8, 4096>(a, s);
         case 88192: return launch_seed_m<8, SCAP>(a, s);
         case 161024: return launch_seed_m<16, 1024>(a, s);
+        case 162048: return launch_seed_m<16, 2048>(a, s);
         case 164096: return launch_seed_m<16, 4096>(a, s);
         case 168192: return launch_seed_m<16, SCAP>(a, s);
         default: return hipErrorInvalidValue;

@@ -924,14 +924,6 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(ix->ws.dmat.ensure(sizeof(float) * size_t(nq) * nlist));
     MIVF_TRY(ix->ws.probes.ensure(sizeof(int32_t) * size_t(nq) * w));
     MIVF_TRY(ix->ws.coadis.ensure(sizeof(float) * size_t(nq) * w));
-    MIVF_TRY(coarse_stage(ix, ix->ws, d_q, nq, w, ix->ws.probes.as<int32_t>(), ix->ws.coadis.as<float>(), st, nullptr));
-    MIVF_TRY(tm.mark());
-    // 2. (list, query) grouping and the buffer clears on the second stream,
-    //    overlapped with 3. the seed thresholds on the caller's stream
-    if (!ix->aux) {
-        MIVF_CHECK_HIP(hipStreamCreateWithFlags(&ix->aux, hipStreamNonBlocking));
-        for (auto &e : ix->aux_ev) MIVF_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
     const int64_t np = int64_t(nq) * w;
     MIVF_TRY(ix->f_cnt.ensure(sizeof(int) * (nlist + 1)));
     MIVF_TRY(ix->f_poff.ensure(sizeof(int) * (nlist + 1)));
@@ -947,6 +939,26 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
     MIVF_TRY(ix->f_tau.ensure(sizeof(float) * nq));
     int *ctrl = ix->f_ctrl.as<int>();  // [0] total items, [1] next item
+    {
+        // candidate counts, the work counter, the running bounds (+inf) and
+        // their histograms start cleared; the stream's order puts this after
+        // every earlier search on the index and before the coarse stage
+        FillSet f{};
+        f.p[0] = ix->f_ccnt.as<int>(), f.n[0] = nq, f.v[0] = 0;
+        f.p[1] = ctrl + 1, f.n[1] = 1, f.v[1] = 0;
+        f.p[2] = ix->f_rthr.as<int>(), f.n[2] = nq, f.v[2] = 0x7f800000;
+        f.p[3] = ix->f_rhist.as<int>(), f.n[3] = int64_t(nq) * fast_hist_bins(), f.v[3] = 0;
+        f.count = 4;
+        MIVF_CHECK_HIP(launch_fill_multi(f, st));
+    }
+    MIVF_TRY(coarse_stage(ix, ix->ws, d_q, nq, w, ix->ws.probes.as<int32_t>(), ix->ws.coadis.as<float>(), st, nullptr));
+    MIVF_TRY(tm.mark());
+    // 2. (list, query) grouping and the buffer clears on the second stream,
+    //    overlapped with 3. the seed thresholds on the caller's stream
+    if (!ix->aux) {
+        MIVF_CHECK_HIP(hipStreamCreateWithFlags(&ix->aux, hipStreamNonBlocking));
+        for (auto &e : ix->aux_ev) MIVF_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     hipStream_t sa = ix->aux;
     MIVF_CHECK_HIP(hipEventRecord(ix->aux_ev[0], st));
     MIVF_CHECK_HIP(hipStreamWaitEvent(sa, ix->aux_ev[0], 0));
@@ -964,11 +976,6 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
                                     nlist, fast_group_size(), ix->f_cnt.as<int>(), ix->f_poff.as<int>(),
                                     ix->f_ioff.as<int>(), ctrl, ctrl, lorder, ix->f_fill.as<int>(),
                                     ix->f_pairq.as<int32_t>(), ix->f_items.as<FastItem>(), sa));
-    MIVF_CHECK_HIP(launch_fill_i32(ix->f_ccnt.as<int>(), nq, 0, sa));
-    MIVF_CHECK_HIP(launch_fill_i32(ctrl + 1, 1, 0, sa));
-    // running bounds (+inf) and their histograms (0)
-    MIVF_CHECK_HIP(launch_fill_i32(ix->f_rthr.as<int>(), nq, 0x7f800000, sa));
-    MIVF_CHECK_HIP(launch_fill_i32(ix->f_rhist.as<int>(), nq * fast_hist_bins(), 0, sa));
     MIVF_CHECK_HIP(hipEventRecord(ix->aux_ev[1], sa));
     // seed threshold from each query's ns[q] nearest lists (at least
     // fast_seed, then as many as it takes to hold kl vectors)
@@ -1231,6 +1238,8 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     } else if (n == "fast_tables") {
         MIVF_REQUIRE(value >= 0 && value <= 2, kInvalid, "fast_tables must be 0, 1 or 2");
         ix->fast_tables = int(value);
+    } else if (n == "exact_running_bound") {
+        ix->exact_running = value != 0;
     } else if (n == "item_order") {
         MIVF_REQUIRE(value == 0 || value == 1, kInvalid, "item_order must be 0 (list order) or 1 (longest lists first)");
         ix->item_order = int(value);

@@ -131,6 +131,7 @@ struct mivf_index {
     bool base_bytes = false; // option base_bytes: attach_base keeps a byte copy
     int fast_front = 0;      // option fast_front_ranks: probe ranks whose work items the 8-bit scan takes first
     int item_order = 0;      // option item_order: 1 = list-major work items numbered longest list first
+    bool exact_running = false;  // option exact_running_bound: running per-query bounds in the exact list-major scan
     DevBuf lorder;           // [nlist] lists by decreasing length (set by upload_lists)
     // batched reduced-precision form scratch
     DevBuf f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_items, f_ctrl, f_cand, f_ccnt,
@@ -143,7 +144,7 @@ struct mivf_index {
     int query_order = 1;      // scan queries grouped by nearest list, per XCD (scheduling only)
     bool fast_running = true;  // running per-query bound in the reduced-precision scan
     int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
-    int fast_seed_cap = 1024; // vectors entering the seed bound (0 = all of the seed lists)
+    int fast_seed_cap = 2048; // vectors entering the seed bound (0 = all of the seed lists)
     int exact_seed_cap = 4096; // the same for the list-major exact kernels
     int fast_cap = 65536;     // candidate capacity per query of the reduced-precision form
     int batched_cap = 16384;  // candidate capacity per query of the batched exact form (>= 2 kmax)

@@ -221,6 +221,14 @@ hipError_t launch_fast_prep(const float *pq, const float *pmax, int m, int ks, v
 hipError_t launch_seed_count(const int32_t *probes, int nq, int w, const int64_t *list_off, int nlist, int kl,
                              const int32_t *klq, int min_seed, int32_t *ns, hipStream_t s);
 hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t s);
+// Up to six int fills (p[j][0..n[j]) = v[j]) in one kernel launch.
+struct FillSet {
+    int *p[6];
+    int64_t n[6];
+    int v[6];
+    int count;
+};
+hipError_t launch_fill_multi(const FillSet &f, hipStream_t s);
 // (list, query) pairs of probe ranks in [rlo, rhi), grouped by list, and the
 // work items (list, first pair) of at most G = 1..16 pairs, numbered from
 // *item_base (0 if item_base is null) list by list in the order lorder gives
@@ -249,7 +257,15 @@ struct ExactArgs {
     int ks;
     int *err;
     const float *tau;           // [nq] per-query bounds (nullptr: the items' own)
+    // running bounds (nullptr: none): runthr [nq] (+inf at launch) and the
+    // per-query histograms rhist [nq][kExactBins] of kept distances over
+    // [0, tau_q) (zero at launch); kq [nq] per-query k (nullptr: kuni)
+    float *runthr;
+    int *rhist;
+    const int32_t *kq;
+    int kuni;
 };
+constexpr int kExactBins = 16;
 size_t exact_lds_bytes(int m, int g);
 // Compiled (m, g, threads): (16,4,256) (16,8,256) (16,8,512) (8,4,256) (8,8,256) (8,8,512) (8,16,512).
 hipError_t launch_exact_scan(const ExactArgs &a, int m, int g, int nt, int grid, hipStream_t s);

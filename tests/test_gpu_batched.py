@@ -66,6 +66,40 @@ def test_item_order_changes_nothing(mivf, zipf16, order):
         ix.set_option("profile", 0)
 
 
+@pytest.mark.parametrize("running", [0, 1])
+def test_exact_running_bound_changes_nothing(mivf, zipf16, running):
+    """The list-major exact scan's running per-query bounds (candidate
+    distance histograms shared by the workgroups) only drop vectors that
+    cannot be among a query's k best: the batched form and the list-major
+    mivf_search equal the per-query kernels bit for bit with or without them,
+    for uneven and uniform batches, run after run."""
+    ix, index, query, base = zipf16
+    w, k = uneven(query.shape[0], seed=11)
+    rids, rdis = ix.search_var(query, w, k, 100)
+    ix.set_option("list_major", 0)
+    pids, pdis = ix.search(query, 32, 100)
+    ix.set_option("list_major", 1)
+    ix.set_option("exact_running_bound", running)
+    ix.set_option("list_major_min_queries", 1)
+    try:
+        for rep in range(3):
+            ids, dis = ix.search_batched(query, w, k, 100)
+            assert_same(ids, dis, rids, rdis, f"batched, running bound {running}, run {rep}")
+            ids, dis = ix.search(query, 32, 100)
+            assert_same(ids, dis, pids, pdis, f"list-major search, running bound {running}, run {rep}")
+        # k = 1 and k = 1000 (the bound then needs one / a thousand kept vectors)
+        for kk in (1, 1000):
+            ix.set_option("list_major", 0)
+            r1 = ix.search(query, 16, kk)
+            ix.set_option("list_major", 1)
+            l1 = ix.search(query, 16, kk)
+            assert_same(l1[0], l1[1], r1[0], r1[1], f"list-major k={kk}, running bound {running}")
+    finally:
+        ix.set_option("exact_running_bound", 0)  # the default
+        ix.set_option("list_major", 1)
+        ix.set_option("list_major_min_queries", 2048)
+
+
 def test_batched_overflow_fallback(mivf, zipf16):
     """A tiny candidate buffer overflows for most queries; the plain rerun of
     those queries keeps the results exact."""

@@ -249,7 +249,8 @@ def test_fast_byte_base(mivf, orc, c1, zipf16, which, w, kl, k):
 
 @pytest.mark.parametrize("which,kl,seed_lists,cap", [("m16", 256, 1, 0), ("m16", 64, 2, 0), ("m8", 128, 1, 0),
                                                       ("m8", 1000, 1, 0), ("m16", 128, 1, 300), ("m16", 64, 2, 1024),
-                                                      ("m8", 128, 1, 129)])
+                                                      ("m8", 128, 1, 129), ("m16", 176, 1, 2048),
+                                                      ("m8", 256, 2, 3000)])
 def test_fast_seed_threshold(mivf, orc, zipf16, c1, which, kl, seed_lists, cap):
     """tau_q bounds the kl-th smallest reference ADC distance over the query's
     seed lists from above (the filter keeps everything up to it) and is
@@ -273,7 +274,7 @@ def test_fast_seed_threshold(mivf, orc, zipf16, c1, which, kl, seed_lists, cap):
         tau = ix.fast_seed_tau(query, w, kl)
     finally:
         ix.set_option("fast_seed_lists", 1)
-        ix.set_option("fast_seed_cap", 1024)  # the library default
+        ix.set_option("fast_seed_cap", 2048)  # the library default
     probes, _ = ix.coarse(query, w)
     lens = np.diff(off)
     for qi in range(query.shape[0]):
