@@ -29,6 +29,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def apply_opts(ix, opts):
+    for o in opts:
+        name, value = o.split("=", 1)
+        ix.set_option(name, int(value))
+        log(f"option {name} = {value}")
+
+
 def same_bits(a_ids, a_dis, b_ids, b_dis):
     return bool(np.array_equal(a_ids, b_ids) and np.array_equal(a_dis.view(np.uint32), b_dis.view(np.uint32)))
 
@@ -68,6 +75,8 @@ def main():
     ap.add_argument("--precision-study", action="store_true",
                     help="the reduced-precision table types on this workload's index (w=32, shortlist 1024, "
                          "k=100): recall and distance error against FP64 per table type")
+    ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
+                    help="index option (mivf_set_option) applied before the searches; repeatable")
     ap.add_argument("--quantizer-cache", default="",
                     help="npz file of the trained coarse and PQ centroids: loaded when it exists, else written "
                          "after training (repeated runs on one workload skip the k-means)")
@@ -115,6 +124,7 @@ def main():
     train.free()
     t0 = time.time()
     ix = M.Index(coarse, pq)
+    apply_opts(ix, args.opt)
     ix.add(base)
     off, ids_l, codes_l = ix.get_lists()
     log(f"encoded {ix.ntotal} vectors in {time.time() - t0:.1f}s")
@@ -534,6 +544,7 @@ def precision_study(args, cfg):
     base = M.synth_rows(seed, STREAM_BASE, 0, cfg["nbase"], cent, thr, SIGMA)
     query = M.synth_rows(seed, STREAM_QUERY, 0, nq, cent, thr, SIGMA)
     ix = M.Index(coarse, pq)
+    apply_opts(ix, args.opt)
     ix.add(base)
     off, ids_l, codes_l = ix.get_lists()
     log(f"index built in {time.time() - t0:.1f}s")
@@ -658,11 +669,19 @@ def streamed_serving(args, cfg):
     cent = M.synth_centres(seed, NCOMP, D)
     thr = M.zipf_thresholds(seed, NCOMP, cfg["zipf"]) if cfg["zipf"] else None
     train = M.synth_rows(seed, STREAM_TRAIN, 0, cfg["ntrain"], cent, thr, SIGMA)
-    coarse, pq = M.train(train, nlist, m, 256, cfg["niter_coarse"], cfg["niter_pq"], seed)
+    if args.quantizer_cache and os.path.exists(args.quantizer_cache):
+        qc = np.load(args.quantizer_cache)
+        coarse, pq = qc["coarse"], qc["pq"]
+        log(f"quantizers loaded from {args.quantizer_cache}")
+    else:
+        coarse, pq = M.train(train, nlist, m, 256, cfg["niter_coarse"], cfg["niter_pq"], seed)
+        log(f"trained {nlist} lists on {cfg['ntrain']} vectors in {time.time() - t0:.1f}s")
+        if args.quantizer_cache:
+            np.savez(args.quantizer_cache, coarse=coarse, pq=pq)
     train.free()
-    log(f"trained {nlist} lists on {cfg['ntrain']} vectors in {time.time() - t0:.1f}s")
     t0 = time.time()
     ix = M.Index(coarse, pq)
+    apply_opts(ix, args.opt)
     rows = M.DeviceArray((chunk, D), np.float32)
     for r0 in range(0, nbase, chunk):
         n = min(chunk, nbase - r0)
@@ -759,7 +778,7 @@ def streamed_serving(args, cfg):
         "data": "synthetic (seeded Gaussian mixture, integer-valued float32), streamed in chunks",
         "config": {"workload": args.workload, "nbase": nbase, "nlist": nlist, "m": m, "nquery": nq, "w": w, "k": k,
                    "form": "plain exact FP32 (mivf_search: list-major kernels)",
-                   "stream_chunk": chunk, "build_s": round(t_build, 1),
+                   "stream_chunk": chunk, "build_s": round(t_build, 1), "options": args.opt,
                    "kmeans_iterations": {"coarse": cfg["niter_coarse"], "pq": cfg["niter_pq"]},
                    "recall_queries": ng, **rec},
         "stages_ms": {n: round(float(v), 4) for n, v in zip(("coarse", "seed_grouping", "scan", "select"), st)},

@@ -256,8 +256,10 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * list first; a schedule only, results unchanged; default 0),
  * "exact_running_bound" (1 = the list-major exact scan lowers each query's
  * bound as its kept distances accumulate, from histograms shared by the
- * workgroups; results unchanged; default 0: fewer candidates, but the
- * histogram traffic costs the scan more than the select saves),
+ * workgroups; 0 = off; 2 = on when a batch expects more than 3,000
+ * candidates per query, w * mean list length * k / exact_seed_cap, the
+ * default; results unchanged either way: it costs histogram traffic and
+ * saves candidates, overflow reruns and select time),
  * "serve_min_fill" and "serve_max_wait_us" (servers over this index: while
  * a micro-batch is in
  * flight, a free slot launches only once serve_min_fill queries wait or the

@@ -51,6 +51,12 @@ struct XWs {
 // Candidates per query: the option's capacity, at least 2 kmax, and at least
 // batched_cap_pp per probe (the vectors within a query's bound grow with its
 // probes).
+// Expected candidates per query above which option exact_running_bound = 2
+// (automatic) turns the running bounds on: measured at 1,900 (10M vectors,
+// 4096 lists, w 32, k 100: 3 % slower with them) and at 4,800 (100M, 16384
+// lists, w 32: 5 % faster, and no overflowed query left to rerun).
+constexpr double kRunningMinCandidates = 3000.0;
+
 int capacity(const mivf_index *ix, int kmax, int wmax) {
     return std::max(std::max(ix->batched_cap, 2 * kmax), ix->batched_cap_pp * wmax);
 }
@@ -96,7 +102,11 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     MIVF_TRY(x.cand.ensure(sizeof(uint64_t) * size_t(nq) * capq));
     MIVF_TRY(x.ccnt.ensure(sizeof(int) * nq));
     MIVF_TRY(x.ovf.ensure(sizeof(int) * (size_t(nq) + 1)));
-    const bool running = ix->exact_running;
+    // running bounds pay when many vectors pass the seed bound: about
+    // w * (mean list length) * k / seed sample of them per query
+    const int seed_cap = ix->exact_seed_cap > 0 ? std::max(ix->exact_seed_cap, kmax) : 0;
+    const double expect = seed_cap > 0 ? double(wmax) * double(ix->ntotal) / nlist * kmax / seed_cap : 0.0;
+    const bool running = ix->exact_running == 1 || (ix->exact_running == 2 && expect > kRunningMinCandidates);
     if (running) {
         MIVF_TRY(x.runthr.ensure(sizeof(float) * nq));
         MIVF_TRY(x.rhist.ensure(sizeof(int) * size_t(nq) * kExactBins));

@@ -1239,7 +1239,8 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
         MIVF_REQUIRE(value >= 0 && value <= 2, kInvalid, "fast_tables must be 0, 1 or 2");
         ix->fast_tables = int(value);
     } else if (n == "exact_running_bound") {
-        ix->exact_running = value != 0;
+        MIVF_REQUIRE(value >= 0 && value <= 2, kInvalid, "exact_running_bound must be 0, 1 or 2 (automatic)");
+        ix->exact_running = int(value);
     } else if (n == "item_order") {
         MIVF_REQUIRE(value == 0 || value == 1, kInvalid, "item_order must be 0 (list order) or 1 (longest lists first)");
         ix->item_order = int(value);

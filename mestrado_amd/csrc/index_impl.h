@@ -131,7 +131,7 @@ struct mivf_index {
     bool base_bytes = false; // option base_bytes: attach_base keeps a byte copy
     int fast_front = 0;      // option fast_front_ranks: probe ranks whose work items the 8-bit scan takes first
     int item_order = 0;      // option item_order: 1 = list-major work items numbered longest list first
-    bool exact_running = false;  // option exact_running_bound: running per-query bounds in the exact list-major scan
+    int exact_running = 2;   // option exact_running_bound: running per-query bounds in the exact list-major scan (2: automatic)
     DevBuf lorder;           // [nlist] lists by decreasing length (set by upload_lists)
     // batched reduced-precision form scratch
     DevBuf f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_items, f_ctrl, f_cand, f_ccnt,

@@ -66,7 +66,7 @@ def test_item_order_changes_nothing(mivf, zipf16, order):
         ix.set_option("profile", 0)
 
 
-@pytest.mark.parametrize("running", [0, 1])
+@pytest.mark.parametrize("running", [0, 1, 2])
 def test_exact_running_bound_changes_nothing(mivf, zipf16, running):
     """The list-major exact scan's running per-query bounds (candidate
     distance histograms shared by the workgroups) only drop vectors that
@@ -95,7 +95,7 @@ def test_exact_running_bound_changes_nothing(mivf, zipf16, running):
             l1 = ix.search(query, 16, kk)
             assert_same(l1[0], l1[1], r1[0], r1[1], f"list-major k={kk}, running bound {running}")
     finally:
-        ix.set_option("exact_running_bound", 0)  # the default
+        ix.set_option("exact_running_bound", 2)  # the default
         ix.set_option("list_major", 1)
         ix.set_option("list_major_min_queries", 2048)
 
