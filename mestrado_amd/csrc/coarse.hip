@@ -38,14 +38,16 @@ constexpr int CP = CD + 4;    // padded LDS row (floats): conflict-free column r
 constexpr int NTC = 256;
 constexpr int RCAP = 1024;    // recheck capacity per query
 
-// One 64-query x 256-centroid tile of D'.  The query tile sits in LDS (FP32,
-// split to BF16 in registers); wave wv owns centroids 64 wv .. 64 wv + 63 of
-// the tile (four 16-wide column tiles whose BF16 hi/lo rows are read from
-// global memory one k block ahead) and all four 16-query row tiles.  In a
-// 32-wide k block, lane group h = l/16 supplies elements k0 + 8h .. k0 + 8h + 7
-// of both operands (the same k permutation on both sides of the sum).
+// One 64-query x (64 TW)-centroid tile of D'.  The query tile sits in LDS
+// (FP32, split to BF16 in registers); wave wv owns centroids 16 TW wv ..
+// 16 TW wv + 16 TW - 1 of the tile (TW 16-wide column tiles whose BF16 hi/lo
+// rows are read from global memory one k block ahead) and all four 16-query
+// row tiles.  In a 32-wide k block, lane group h = l/16 supplies elements
+// k0 + 8h .. k0 + 8h + 7 of both operands (the same k permutation on both
+// sides of the sum).  TW = 2 halves the accumulators of TW = 4, so four waves
+// per SIMD fit instead of two.
 constexpr int CQ = 64;   // queries per tile
-constexpr int CC = 256;  // centroids per tile
+constexpr int CTW = 2;   // column tiles per wave of the launched shape
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -57,7 +59,8 @@ __device__ inline void split_bf16(const float (&x)[8], bf16x8 &hi, bf16x8 &lo) {
     }
 }
 
-__global__ __launch_bounds__(NTC) void coarse_approx_kernel(const __bf16 *__restrict__ chi,
+template <int TW>
+__global__ __launch_bounds__(NTC, (TW <= 2 ? 4 : 2)) void coarse_approx_kernel(const __bf16 *__restrict__ chi,
                                                             const __bf16 *__restrict__ clo,
                                                             const float *__restrict__ cn, int nlist,
                                                             const float *__restrict__ queries,
@@ -75,33 +78,33 @@ __global__ __launch_bounds__(NTC) void coarse_approx_kernel(const __bf16 *__rest
     }
     const int lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 4, i16 = lane & 15;
-    const int c0 = blockIdx.x * CC + wave * 64;
-    int64_t crow[4];
+    const int c0 = blockIdx.x * (64 * TW) + wave * (16 * TW);
+    int64_t crow[TW];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) crow[t] = int64_t(min(c0 + 16 * t + i16, nlist - 1)) * CD + 8 * h;
-    bf16x8 bhn[4], bln[4];
+    for (int t = 0; t < TW; ++t) crow[t] = int64_t(min(c0 + 16 * t + i16, nlist - 1)) * CD + 8 * h;
+    bf16x8 bhn[TW], bln[TW];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < TW; ++t) {
         bhn[t] = *reinterpret_cast<const bf16x8 *>(chi + crow[t]);
         bln[t] = *reinterpret_cast<const bf16x8 *>(clo + crow[t]);
     }
-    f32x4 acc[4][4];
+    f32x4 acc[4][TW];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[a][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < TW; ++t) acc[a][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     __syncthreads();
 #pragma unroll 1
     for (int k0 = 0; k0 < CD; k0 += 32) {
-        bf16x8 bh[4], bl[4];
+        bf16x8 bh[TW], bl[TW];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < TW; ++t) {
             bh[t] = bhn[t];
             bl[t] = bln[t];
         }
         if (k0 + 32 < CD) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < TW; ++t) {
                 bhn[t] = *reinterpret_cast<const bf16x8 *>(chi + crow[t] + k0 + 32);
                 bln[t] = *reinterpret_cast<const bf16x8 *>(clo + crow[t] + k0 + 32);
             }
@@ -115,7 +118,7 @@ __global__ __launch_bounds__(NTC) void coarse_approx_kernel(const __bf16 *__rest
             bf16x8 ah, al;
             split_bf16(xv, ah, al);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < TW; ++t) {
                 acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[qt][t], 0, 0, 0);
                 acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[qt][t], 0, 0, 0);
                 acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[qt][t], 0, 0, 0);
@@ -123,19 +126,23 @@ __global__ __launch_bounds__(NTC) void coarse_approx_kernel(const __bf16 *__rest
         }
     }
     // accumulator r of lane l: query row 16 qt + 4h + r, centroid column 16 t + l % 16.
-    // Each 16 x 64 block of the wave goes through LDS so that a store writes a
-    // whole row piece (64 centroids, 256 contiguous bytes).  Row stride TS = 68
-    // floats puts the four rows a write touches (h = 0..3) in distinct banks.
-    constexpr int TS = 68;
+    // Each 16 x 16 TW block of the wave goes through LDS so that a store
+    // writes whole row pieces (16 TW centroids, 64 TW contiguous bytes; with
+    // TW = 2 the two halves of the wave store two rows).  Row stride
+    // TS = 16 TW + 4 floats puts the four rows a write touches (h = 0..3) in
+    // distinct banks.
+    constexpr int TS = 16 * TW + 4;
+    constexpr int RPS = 64 / (16 * TW);  // rows per store instruction
     __syncthreads();  // every wave is done with the query tile
     float *T = lds + wave * (16 * TS);
-    double cnc[4];
+    double cnc[TW];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < TW; ++t) {
         const int c = c0 + 16 * t + i16;
         cnc[t] = c < nlist ? double(cn[c]) : 0.0;
     }
-    const int cl = c0 + lane;
+    const int sc = lane % (16 * TW), sr = lane / (16 * TW);  // this lane's column and row within a store
+    const int cl = c0 + sc;
 #pragma unroll
     for (int qt = 0; qt < 4; ++qt) {
 #pragma unroll
@@ -143,16 +150,16 @@ __global__ __launch_bounds__(NTC) void coarse_approx_kernel(const __bf16 *__rest
             const int64_t q = q0 + 16 * qt + 4 * h + r;
             const double qq = q < nq ? qn[q] : 0.0;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) T[(4 * h + r) * TS + 16 * t + i16] = float(qq + cnc[t]) - 2.0f * acc[qt][t][r];
+            for (int t = 0; t < TW; ++t) T[(4 * h + r) * TS + 16 * t + i16] = float(qq + cnc[t]) - 2.0f * acc[qt][t][r];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (cl < nlist) {
 #pragma unroll
-            for (int row = 0; row < 16; ++row) {
-                const int64_t q = q0 + 16 * qt + row;
-                if (q < nq) out[q * ldo + cl] = T[row * TS + lane];
+            for (int row = 0; row < 16; row += RPS) {
+                const int64_t q = q0 + 16 * qt + row + sr;
+                if (q < nq) out[q * ldo + cl] = T[(row + sr) * TS + sc];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -555,10 +562,11 @@ hipError_t launch_coarse_split(const float *coarse, int64_t n, void *hi, void *l
 hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const float *queries,
                                 const double *qn, int nq, float *approx, int64_t lda, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
+    constexpr int CC = 64 * CTW;  // centroids per tile
     dim3 grid(unsigned((nlist + CC - 1) / CC), unsigned((nq + CQ - 1) / CQ));
-    coarse_approx_kernel<<<grid, NTC, CQ * CP * 4, s>>>(static_cast<const __bf16 *>(chi),
-                                                        static_cast<const __bf16 *>(clo), cn, nlist, queries, qn, nq,
-                                                        approx, lda);
+    coarse_approx_kernel<CTW><<<grid, NTC, CQ * CP * 4, s>>>(static_cast<const __bf16 *>(chi),
+                                                             static_cast<const __bf16 *>(clo), cn, nlist, queries, qn,
+                                                             nq, approx, lda);
     return hipGetLastError();
 }
 
