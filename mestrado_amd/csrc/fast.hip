@@ -30,6 +30,7 @@
 // staged in LDS and flushed to per-query global buffers; fast_select keeps
 // the kl best and the exact FP32 rerank orders them by (distance, id).
 #include "select_dev.h"
+#include "rerank_dev.h"
 
 namespace mivf {
 
@@ -1026,19 +1027,26 @@ __global__ __launch_bounds__(256, (SC <= 2048 ? 6 : 4)) void seed_tau_kernel(con
 // more than SELB keys (heavy ties), a radix select over all keys is used.
 constexpr int SELB = 1024;
 
-__global__ __launch_bounds__(256) void fast_select_kernel(int kl, const uint64_t *__restrict__ cand,
-                                                          const int *__restrict__ cand_cnt, int capq,
-                                                          int32_t *__restrict__ short_ids) {
-    __shared__ SelectShared sh;
-    __shared__ uint64_t buf[SELB];
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t wmn[4], wmx[4];
-    __shared__ int sbin, sbelow, cnt, nb, eqt;
-    const int q = blockIdx.x;
+struct ShortShared {
+    SelectShared sh;
+    uint64_t buf[SELB];
+    uint32_t hist[256];
+    uint32_t wmn[4], wmx[4];
+    int sbin, sbelow, cnt, nb, eqt;
+};
+
+// The kl best keys of query q's candidates into out[0..kl) (list positions,
+// -1 past the candidates; any order), by the whole workgroup.
+__device__ void select_shortlist(int kl, const uint64_t *__restrict__ cand, const int *__restrict__ cand_cnt,
+                                 int capq, int q, int32_t *out, ShortShared &ss) {
+    SelectShared &sh = ss.sh;
+    uint64_t *buf = ss.buf;
+    uint32_t *hist = ss.hist;
+    uint32_t *wmn = ss.wmn, *wmx = ss.wmx;
+    int &sbin = ss.sbin, &sbelow = ss.sbelow, &cnt = ss.cnt, &nb = ss.nb, &eqt = ss.eqt;
     const int tid = threadIdx.x;
     const int n = min(cand_cnt[q], capq);
     const uint64_t *C = cand + int64_t(q) * capq;
-    int32_t *out = short_ids + int64_t(q) * kl;
     if (n <= kl) {  // uniform: every candidate is in
         for (int i = tid; i < kl; i += 256) out[i] = i < n ? key_id(C[i]) : -1;
         return;
@@ -1157,6 +1165,35 @@ __global__ __launch_bounds__(256) void fast_select_kernel(int kl, const uint64_t
         if (!take && r.eq > 0 && key == r.thr) take = atomicAdd(&eqt, 1) < r.eq;
         if (take) out[atomicAdd(&cnt, 1)] = key_id(key);  // cnt starts at `below`
     }
+}
+
+__global__ __launch_bounds__(256) void fast_select_kernel(int kl, const uint64_t *__restrict__ cand,
+                                                          const int *__restrict__ cand_cnt, int capq,
+                                                          int32_t *__restrict__ short_ids) {
+    __shared__ ShortShared ss;
+    const int q = blockIdx.x;
+    select_shortlist(kl, cand, cand_cnt, capq, q, short_ids + int64_t(q) * kl, ss);
+}
+
+// Shortlist and exact rerank in one workgroup per query (kl <= 256): the
+// list positions stay in LDS instead of a round trip through HBM.
+template <int MODE, typename T>
+__global__ __launch_bounds__(256) void fast_select_rerank_kernel(int kl, const uint64_t *__restrict__ cand,
+                                                                 const int *__restrict__ cand_cnt, int capq,
+                                                                 const T *__restrict__ rows, int64_t nrows,
+                                                                 const int32_t *__restrict__ list_ids,
+                                                                 const float *__restrict__ queries, int d,
+                                                                 const double *__restrict__ qnorm, int k,
+                                                                 int32_t *__restrict__ out_ids,
+                                                                 float *__restrict__ out_dis) {
+    __shared__ ShortShared ss;
+    __shared__ RerankShared rs;
+    __shared__ int32_t sl[256];
+    const int q = blockIdx.x;
+    select_shortlist(kl, cand, cand_cnt, capq, q, sl, ss);
+    __syncthreads();
+    const int32_t e = int(threadIdx.x) < kl ? sl[threadIdx.x] : -1;
+    rerank_block<MODE, T, 256>(rows, nrows, list_ids, queries, d, qnorm, q, e, kl, k, out_ids, out_dis, rs);
 }
 
 inline unsigned nblk(int64_t n, int t) { return unsigned((n + t - 1) / t); }
@@ -1311,6 +1348,29 @@ hipError_t launch_fast_tables(const FastArgs &a, int m, int l, int n, uint8_t *t
         case 16: return launch_tables_m<16>(a, l, n, t8, delta, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_fast_select_rerank(int kl, const uint64_t *cand, const int *cand_cnt, int capq, int nq,
+                                     const void *rows, bool rows_u8, bool list_ordered, int64_t nrows,
+                                     const int32_t *list_ids, const float *queries, int d, const double *qnorm, int k,
+                                     int32_t *out_ids, float *out_dis, hipStream_t s) {
+    if (kl < 1 || kl > 256 || k < 1 || k > kl || d > 512) return hipErrorInvalidValue;
+    if (nq <= 0) return hipSuccess;
+    if (rows_u8 && list_ordered)
+        fast_select_rerank_kernel<2, uint8_t><<<nq, 256, 0, s>>>(kl, cand, cand_cnt, capq,
+                                                                 static_cast<const uint8_t *>(rows), nrows, list_ids,
+                                                                 queries, d, qnorm, k, out_ids, out_dis);
+    else if (list_ordered)
+        fast_select_rerank_kernel<2, float><<<nq, 256, 0, s>>>(kl, cand, cand_cnt, capq,
+                                                               static_cast<const float *>(rows), nrows, list_ids,
+                                                               queries, d, qnorm, k, out_ids, out_dis);
+    else if (!rows_u8)
+        fast_select_rerank_kernel<1, float><<<nq, 256, 0, s>>>(kl, cand, cand_cnt, capq,
+                                                               static_cast<const float *>(rows), nrows, list_ids,
+                                                               queries, d, qnorm, k, out_ids, out_dis);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 hipError_t launch_fast_select(int kl, const uint64_t *cand, const int *cand_cnt, int capq, int nq,

@@ -1037,23 +1037,33 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         }
     }
     MIVF_TRY(tm.mark());
-    // 5. shortlist and exact rerank
-    MIVF_TRY(ix->f_short.ensure(sizeof(int32_t) * size_t(nq) * kl));
-    MIVF_CHECK_HIP(launch_fast_select(kl, ix->f_cand.as<uint64_t>(), ix->f_ccnt.as<int>(), capq, nq,
-                                      ix->f_short.as<int32_t>(), st));
-    MIVF_TRY(tm.mark());
-    // the coarse stage left the queries' f64 norms in ws.qn
-    if (d_base)
-        MIVF_CHECK_HIP(launch_rerank_pos(d_base, nbase, false, ix->ids.as<int32_t>(), d_q, nq, d,
-                                         ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
-    else if (ix->rows_u8)
-        MIVF_CHECK_HIP(launch_rerank_pos_u8(ix->rows.as<uint8_t>(), ix->ntotal, ix->ids.as<int32_t>(), d_q, nq, d,
-                                            ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis,
-                                            st));
-    else
-        MIVF_CHECK_HIP(launch_rerank_pos(ix->rows.as<float>(), ix->ntotal, true, ix->ids.as<int32_t>(), d_q, nq, d,
-                                         ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
-    MIVF_TRY(tm.mark());
+    // 5. shortlist and exact rerank (the coarse stage left the queries' f64
+    //    norms in ws.qn); shortlists of up to 256 in one kernel
+    if (kl <= 256) {
+        const void *rows = d_base ? static_cast<const void *>(d_base) : ix->rows.p;
+        MIVF_CHECK_HIP(launch_fast_select_rerank(kl, ix->f_cand.as<uint64_t>(), ix->f_ccnt.as<int>(), capq, nq, rows,
+                                                 !d_base && ix->rows_u8, !d_base, d_base ? nbase : ix->ntotal,
+                                                 ix->ids.as<int32_t>(), d_q, d, ix->ws.qn.as<double>(), k, d_ids,
+                                                 d_dis, st));
+        MIVF_TRY(tm.mark());
+        MIVF_TRY(tm.mark());  // (no separate rerank stage)
+    } else {
+        MIVF_TRY(ix->f_short.ensure(sizeof(int32_t) * size_t(nq) * kl));
+        MIVF_CHECK_HIP(launch_fast_select(kl, ix->f_cand.as<uint64_t>(), ix->f_ccnt.as<int>(), capq, nq,
+                                          ix->f_short.as<int32_t>(), st));
+        MIVF_TRY(tm.mark());
+        if (d_base)
+            MIVF_CHECK_HIP(launch_rerank_pos(d_base, nbase, false, ix->ids.as<int32_t>(), d_q, nq, d,
+                                             ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
+        else if (ix->rows_u8)
+            MIVF_CHECK_HIP(launch_rerank_pos_u8(ix->rows.as<uint8_t>(), ix->ntotal, ix->ids.as<int32_t>(), d_q, nq, d,
+                                                ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis,
+                                                st));
+        else
+            MIVF_CHECK_HIP(launch_rerank_pos(ix->rows.as<float>(), ix->ntotal, true, ix->ids.as<int32_t>(), d_q, nq, d,
+                                             ix->ws.qn.as<double>(), ix->f_short.as<int32_t>(), kl, k, d_ids, d_dis, st));
+        MIVF_TRY(tm.mark());
+    }
     MIVF_TRY(finish_timer(ix, tm));
     if (!ix->scan_checked || ix->profile) {
         int err = 0;

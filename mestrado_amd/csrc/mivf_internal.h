@@ -303,6 +303,14 @@ hipError_t launch_fast_var_adc(const FastArgs &a, int m, int tables, int l, int 
 // The 8-bit tables and scales of one work item (queries 0..n-1 of a.queries,
 // n <= 16, list l): t8 [m][256][16] uint8, delta [16].  Test hook.
 hipError_t launch_fast_tables(const FastArgs &a, int m, int l, int n, uint8_t *t8, float *delta, hipStream_t s);
+// The shortlist of launch_fast_select and the exact rerank of
+// launch_rerank_pos / launch_rerank_pos_u8 in one kernel (kl <= 256): rows
+// are the list-ordered copy (float or bytes) or, with list_ordered false, the
+// float base indexed by id.
+hipError_t launch_fast_select_rerank(int kl, const uint64_t *cand, const int *cand_cnt, int capq, int nq,
+                                     const void *rows, bool rows_u8, bool list_ordered, int64_t nrows,
+                                     const int32_t *list_ids, const float *queries, int d, const double *qnorm, int k,
+                                     int32_t *out_ids, float *out_dis, hipStream_t s);
 hipError_t launch_fast_select(int kl, const uint64_t *cand, const int *cand_cnt, int capq, int nq,
                               int32_t *short_ids, hipStream_t s);
 

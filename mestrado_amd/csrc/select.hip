@@ -2,6 +2,7 @@
 // lists, and the exact rerank of a shortlist.  All selections order by
 // (distance ascending, id ascending) through 64-bit keys and are exact.
 #include "select_dev.h"
+#include "rerank_dev.h"
 
 namespace mivf {
 
@@ -122,72 +123,6 @@ __global__ __launch_bounds__(NT) void merge_kernel(const uint64_t *__restrict__ 
     sort_and_emit(buf, c, kstride, out_ids + int64_t(q) * kstride, out_dis + int64_t(q) * kstride);
 }
 
-// Element t of row x as a float: rows of float, or of bytes holding the
-// integers 0..255 exactly (a byte row converts to the same floats).
-__device__ inline float row_el(const float *x, int t) { return x[t]; }
-__device__ inline float row_el(const uint8_t *x, int t) { return float(x[t]); }
-
-// The squared norm (f32 accumulator) and the dot product with qv of one row,
-// both sequential in element order with every product rounded (no FMA).
-__device__ inline void row_sums(const float *x, const float *qv, int d, bool vec, float &na, float &dot) {
-    int t = 0;
-    if (vec) {
-        // 16-byte loads, issued 8 at a time; the sums keep the element order
-        const float4 *x4 = reinterpret_cast<const float4 *>(x);
-        for (; t + 32 <= d; t += 32) {
-            float4 v[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = x4[t / 4 + j];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const float el[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    na = na + el[u] * el[u];
-                    dot = dot + el[u] * qv[t + 4 * j + u];
-                }
-            }
-        }
-    }
-    for (; t < d; ++t) {
-        const float xv = x[t];
-        const float p = xv * xv;
-        na = na + p;
-        const float pd = xv * qv[t];
-        dot = dot + pd;
-    }
-}
-
-__device__ inline void row_sums(const uint8_t *x, const float *qv, int d, bool vec, float &na, float &dot) {
-    int t = 0;
-    if (vec) {
-        // one 128-byte row in eight 16-byte loads issued together
-        const uint4 *x16 = reinterpret_cast<const uint4 *>(x);
-        for (; t + 128 <= d; t += 128) {
-            uint4 v[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = x16[t / 16 + j];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint32_t wd[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
-#pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const float el = float((wd[u >> 2] >> (8 * (u & 3))) & 255u);
-                    na = na + el * el;
-                    dot = dot + el * qv[t + 16 * j + u];
-                }
-            }
-        }
-    }
-    for (; t < d; ++t) {
-        const float xv = row_el(x, t);
-        const float p = xv * xv;
-        na = na + p;
-        const float pd = xv * qv[t];
-        dot = dot + pd;
-    }
-}
-
 // rerank: a = base row (f32-accumulated norm), b = query (f64-accumulated norm),
 // as knn_full(2, 1, ktmp, d, k, nearest, query) in sequential/ivf_pq/ivf_search.cpp:130.
 // MODE 0: the shortlist holds ids, rows base[id].  MODE 1: it holds list
@@ -252,16 +187,8 @@ __global__ __launch_bounds__(NT) void rerank_kernel(const T *__restrict__ base, 
     sort_and_emit(buf, c, k, out_ids + int64_t(q) * k, out_dis + int64_t(q) * k);
 }
 
-// The same rerank for shortlists of at most NT (one row per thread): the
-// shortlist entry, the query and its norm are loaded together, a 128-byte
-// row (bytes, d = 128) is loaded as soon as its position is known (MODE 0
-// and 2: the row is the entry itself) while the id is still on its way, and
-// each key then goes straight to its rank (the count of smaller keys, equal
-// keys ordered by position).
-// Byte rows against a query whose values are all integers in 0..255 (checked
-// per query): every product and partial sum of |x|^2 and x.q is then an
-// integer below 128 * 255^2 < 2^24, so the sequential FP32 sums are exact
-// and equal the integer sums, which v_dot4_u32_u8 forms four bytes at a time.
+// The rerank for shortlists of at most NT (one row per thread): rerank_block
+// (rerank_dev.h) with the shortlist entry read from short_ids.
 template <int MODE, typename T = float>
 __global__ __launch_bounds__(NT) void rerank_small_kernel(const T *__restrict__ base, int64_t nbase,
                                                           const int32_t *__restrict__ list_ids,
@@ -270,99 +197,10 @@ __global__ __launch_bounds__(NT) void rerank_small_kernel(const T *__restrict__ 
                                                           const int32_t *__restrict__ short_ids,
                                                           int kl, int k, int32_t *__restrict__ out_ids,
                                                           float *__restrict__ out_dis) {
-    __shared__ uint64_t cand[NT];
-    __shared__ float qv[512];
-    __shared__ uint32_t qb[32];  // the query as bytes (d = 128, integer-valued queries)
-    __shared__ double qn;
-    const int tid = threadIdx.x;
+    __shared__ RerankShared rs;
     const int q = blockIdx.x;
-    const int32_t e = tid < kl ? short_ids[int64_t(q) * kl + tid] : -1;
-    bool qint = true;
-    for (int t = tid; t < d; t += NT) {
-        const float v = queries[int64_t(q) * d + t];
-        qv[t] = v;
-        qint = qint && v >= 0.0f && v <= 255.0f && v == __builtin_truncf(v);
-        if (t < 128) reinterpret_cast<uint8_t *>(qb)[t] = uint8_t(qint ? v : 0.0f);
-    }
-    if (tid == 0 && qnorm) qn = qnorm[q];
-    // whole byte rows of 128 values in registers (eight 16-byte loads)
-    constexpr bool kRow = sizeof(T) == 1 && MODE != 1;
-    const bool whole = kRow && d == 128 && (reinterpret_cast<uintptr_t>(base) & 15u) == 0;
-    uint4 pre[8];
-    if (whole && e >= 0 && int64_t(e) < nbase) {
-        const uint4 *x16 = reinterpret_cast<const uint4 *>(reinterpret_cast<const uint8_t *>(base) + int64_t(e) * 128);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) pre[j] = x16[j];
-    }
-    int32_t id = e;
-    if (MODE != 0 && e >= 0) id = list_ids[e];
-    const int64_t row = MODE == 1 ? int64_t(id) : int64_t(e);
-    const bool ibytes = __syncthreads_and(qint) && whole;  // qv, qb and qn are ready
-    if (!qnorm) {
-        if (tid == 0) {
-            double s = 0.0;
-            for (int t = 0; t < d; ++t) {
-                const float p = qv[t] * qv[t];
-                s = s + double(p);
-            }
-            qn = s;
-        }
-        __syncthreads();
-    }
-    uint64_t key = kNoKey;
-    if (e >= 0 && id >= 0 && row < nbase) {
-        float na = 0.0f, dot = 0.0f;
-        if (ibytes) {
-            uint32_t ina = 0u, idot = 0u;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint32_t wd[4] = {pre[j].x, pre[j].y, pre[j].z, pre[j].w};
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    ina = __builtin_amdgcn_udot4(wd[u], wd[u], ina, false);
-                    idot = __builtin_amdgcn_udot4(wd[u], qb[4 * j + u], idot, false);
-                }
-            }
-            na = float(ina);
-            dot = float(idot);
-        } else if (whole) {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint32_t wd[4] = {pre[j].x, pre[j].y, pre[j].z, pre[j].w};
-#pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const float el = float((wd[u >> 2] >> (8 * (u & 3))) & 255u);
-                    na = na + el * el;
-                    dot = dot + el * qv[16 * j + u];
-                }
-            }
-        } else {
-            const bool vec = (d % (64 / int(sizeof(T)))) == 0 && (reinterpret_cast<uintptr_t>(base) & 15u) == 0;
-            row_sums(base + row * d, qv, d, vec, na, dot);
-        }
-        const float c = float(qn + double(na));
-        const float m2 = -2.0f * dot;
-        key = ckey(m2 + c, id);
-    }
-    cand[tid] = key;
-    const int nreal = __syncthreads_count(key != kNoKey);
-    int32_t *oi = out_ids + int64_t(q) * k;
-    float *od = out_dis + int64_t(q) * k;
-    if (key != kNoKey) {
-        int rank = 0;
-        for (int j = 0; j < kl; ++j) {
-            const uint64_t o = cand[j];
-            rank += o < key || (o == key && j < tid);
-        }
-        if (rank < k) {
-            oi[rank] = key_id(key);
-            od[rank] = key_dist(key);
-        }
-    }
-    for (int i = nreal + tid; i < k; i += NT) {
-        oi[i] = -1;
-        od[i] = __builtin_huge_valf();
-    }
+    const int32_t e = threadIdx.x < kl ? short_ids[int64_t(q) * kl + threadIdx.x] : -1;
+    rerank_block<MODE, T, NT>(base, nbase, list_ids, queries, d, qnorm, q, e, kl, k, out_ids, out_dis, rs);
 }
 
 __global__ void gather_rows_kernel(const float *__restrict__ base, int d, const int32_t *__restrict__ ids,
