@@ -1023,8 +1023,9 @@ __global__ __launch_bounds__(256, (SC <= 2048 ? 6 : 4)) void seed_tau_kernel(con
 // the candidates (in any order; the rerank sorts).  A 256-bin histogram of
 // the keys' distance bits over [min, max] finds the bin b holding the kl-th
 // key; keys of lower bins are taken whole, and the few keys of bin b go to
-// LDS, where an exact radix select picks the missing ones.  When bin b holds
-// more than SELB keys (heavy ties), a radix select over all keys is used.
+// LDS, where their ranks (up to 256 keys) or an exact radix select pick the
+// missing ones.  When bin b holds more than SELB keys (heavy ties), a radix
+// select over all keys is used.
 constexpr int SELB = 1024;
 
 struct ShortShared {
@@ -1045,21 +1046,25 @@ __device__ void select_shortlist(int kl, const uint64_t *__restrict__ cand, cons
     uint32_t *wmn = ss.wmn, *wmx = ss.wmx;
     int &sbin = ss.sbin, &sbelow = ss.sbelow, &cnt = ss.cnt, &nb = ss.nb, &eqt = ss.eqt;
     const int tid = threadIdx.x;
-    const int n = min(cand_cnt[q], capq);
     const uint64_t *C = cand + int64_t(q) * capq;
-    if (n <= kl) {  // uniform: every candidate is in
-        for (int i = tid; i < kl; i += 256) out[i] = i < n ? key_id(C[i]) : -1;
-        return;
-    }
-    // up to 2048 candidates are read once, into registers; more are re-read
+    // up to 2048 candidates are read once, into registers (more are re-read);
+    // the loads go out together with the count's, entries past it are dropped
     constexpr int FR = 8;
-    const bool inreg = n <= 256 * FR;  // uniform
     uint64_t rk[FR];
 #pragma unroll
     for (int e = 0; e < FR; ++e) {
         const int i = tid + 256 * e;
-        rk[e] = inreg && i < n ? C[i] : kNoKey;
+        rk[e] = i < capq ? C[i] : kNoKey;
     }
+    const int n = min(cand_cnt[q], capq);
+    if (n <= kl) {  // uniform: every candidate is in
+        for (int i = tid; i < kl; i += 256) out[i] = i < n ? key_id(C[i]) : -1;
+        return;
+    }
+    const bool inreg = n <= 256 * FR;  // uniform
+#pragma unroll
+    for (int e = 0; e < FR; ++e)
+        if (!inreg || tid + 256 * e >= n) rk[e] = kNoKey;
     uint32_t mn = 0xFFFFFFFFu, mx = 0u;
     if (inreg) {
 #pragma unroll
@@ -1157,6 +1162,18 @@ __device__ void select_shortlist(int kl, const uint64_t *__restrict__ cand, cons
     }
     __syncthreads();
     const int m = nb, need = kl - below;
+    if (m <= 256) {  // uniform: the few keys of bin b by rank, one pass
+        if (tid < m) {
+            const uint64_t key = buf[tid];
+            int r = 0;
+            for (int i = 0; i < m; ++i) {
+                const uint64_t o = buf[i];
+                r += o < key || (o == key && i < tid);
+            }
+            if (r < need) out[atomicAdd(&cnt, 1)] = key_id(key);  // cnt starts at `below`
+        }
+        return;
+    }
     auto bkey = [&](int i) -> uint64_t { return buf[i]; };
     const SelectResult r = block_select<256>(bkey, m, need, sh);
     for (int i = tid; i < m; i += 256) {

@@ -88,7 +88,7 @@ struct RerankShared {
 // row (bytes, d = 128) is loaded as soon as its position is known (MODE 0
 // and 2: the row is the entry itself) while the id is still on its way, and
 // each key then goes straight to its rank (the count of smaller keys, equal
-// keys ordered by position).
+// keys ordered by position), found from per-wavefront sorted runs.
 // Byte rows against a query whose values are all integers in 0..255 (checked
 // per query): every product and partial sum of |x|^2 and x.q is then an
 // integer below 128 * 255^2 < 2^24, so the sequential FP32 sums are exact
@@ -98,7 +98,7 @@ __device__ void rerank_block(const T *__restrict__ base, int64_t nbase, const in
                              const float *__restrict__ queries, int d, const double *__restrict__ qnorm, int q,
                              int32_t e, int kl, int k, int32_t *__restrict__ out_ids, float *__restrict__ out_dis,
                              RerankShared &rs) {
-    static_assert(NT <= 256, "one shortlist entry per thread, at most 256");
+    static_assert(NT <= 256 && NT % 64 == 0, "one shortlist entry per thread, at most 256, whole wavefronts");
     uint64_t *cand = rs.cand;
     float *qv = rs.qv;
     uint32_t *qb = rs.qb;
@@ -171,19 +171,55 @@ __device__ void rerank_block(const T *__restrict__ base, int64_t nbase, const in
         const float m2 = -2.0f * dot;
         key = ckey(m2 + c, id);
     }
-    cand[tid] = key;
+    // Rank of each key among the workgroup's keys, equal keys ordered by
+    // thread: every wavefront sorts its 64 (key, thread) pairs (bitonic
+    // network over lanes), publishes the sorted run, and the pair at lane L
+    // of run w then has rank L + the count of smaller pairs in the other
+    // runs (binary searches: a key of run w2 < w comes first when equal).
+    const int lane = tid & 63, wave = tid >> 6;
+    uint64_t sk = key;
+    int sj = tid;
+#pragma unroll
+    for (int kk = 2; kk <= 64; kk <<= 1) {
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            const uint32_t plo = uint32_t(__shfl_xor(int(uint32_t(sk)), j, 64));
+            const uint32_t phi = uint32_t(__shfl_xor(int(uint32_t(sk >> 32)), j, 64));
+            const uint64_t pk = (uint64_t(phi) << 32) | plo;
+            const int pj = __shfl_xor(sj, j, 64);
+            const bool pless = pk < sk || (pk == sk && pj < sj);
+            // ascending blocks keep the smaller pair in the lower lane, descending ones the larger
+            const bool take = (((lane & kk) == 0) == ((lane & j) == 0)) ? pless : !pless;
+            if (take) {
+                sk = pk;
+                sj = pj;
+            }
+        }
+    }
+    cand[tid] = sk;  // run `wave`, ascending
     const int nreal = __syncthreads_count(key != kNoKey);
     int32_t *oi = out_ids + int64_t(q) * k;
     float *od = out_dis + int64_t(q) * k;
-    if (key != kNoKey) {
-        int rank = 0;
-        for (int j = 0; j < kl; ++j) {
-            const uint64_t o = cand[j];
-            rank += o < key || (o == key && j < tid);
+    if (sk != kNoKey) {
+        int rank = lane;
+#pragma unroll
+        for (int w2 = 0; w2 < NT / 64; ++w2) {
+            if (w2 == wave) continue;
+            const uint64_t *run = cand + 64 * w2;
+            const bool le = w2 < wave;  // equal keys of an earlier run come first
+            int pos = 0;
+#pragma unroll
+            for (int s = 32; s > 0; s >>= 1) {
+                const uint64_t o = run[pos + s - 1];
+                if (le ? o <= sk : o < sk) pos += s;
+            }
+            const uint64_t o = run[pos];
+            pos += (le ? o <= sk : o < sk) ? 1 : 0;
+            rank += pos;
         }
         if (rank < k) {
-            oi[rank] = key_id(key);
-            od[rank] = key_dist(key);
+            oi[rank] = key_id(sk);
+            od[rank] = key_dist(sk);
         }
     }
     for (int i = nreal + tid; i < k; i += NT) {
