@@ -87,7 +87,7 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     const int capq = capacity(ix, kmax, wmax);
     const int64_t np = int64_t(nq) * wmax;
     MIVF_TRY(x.ws.qn.ensure(sizeof(double) * nq));
-    MIVF_TRY(x.ws.dmat.ensure(sizeof(float) * size_t(nq) * nlist));
+    MIVF_TRY(x.ws.dmat.ensure(coarse_ws_bytes(nq, nlist)));
     MIVF_TRY(x.ws.probes.ensure(sizeof(int32_t) * std::max<int64_t>(np, 1)));
     MIVF_TRY(x.ws.coadis.ensure(sizeof(float) * std::max<int64_t>(np, 1)));
     MIVF_TRY(x.ns.ensure(sizeof(int32_t) * nq));

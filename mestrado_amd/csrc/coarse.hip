@@ -1,10 +1,11 @@
 // Coarse assignment (top-w lists per query) with the reference's exact FP32
 // arithmetic, computed in two steps:
 //
-// 1. coarse_approx_kernel: D'(q, c) = fl(|c|^2 + |q|^2) - 2 q.c with the dot
-//    products on the matrix cores in BF16 with FP32 accumulation
+// 1. coarse_approx_kernel: D'(q, c) = fl(fl(|q|^2) + |c|^2) - 2 q.c with the
+//    dot products on the matrix cores in BF16 with FP32 accumulation
 //    (v_mfma_f32_16x16x32_bf16).  Each operand is split x = hi + lo (two
-//    BF16 terms, hi = bf16(x), lo = bf16(x - hi)), and q.c is accumulated as
+//    BF16 terms, hi = bf16(x), lo = bf16(x - hi); centroids once per index,
+//    queries by coarse_prep_kernel), and q.c is accumulated as
 //    hi.hi + hi.lo + lo.hi: every BF16 x BF16 product is exact in FP32.
 // 2. coarse_select_*_kernel: v' = an upper bound of the w-th smallest D', then
 //    every centroid with D' <= v' + 2 M_q is re-evaluated with the exact
@@ -34,12 +35,12 @@ namespace {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int CD = 128;       // dimension of the MFMA path
-constexpr int CP = CD + 4;    // padded LDS row (floats): conflict-free column reads
+constexpr int CPB = 2 * CD + 8;  // padded LDS row of a query's hi and lo terms (bf16)
 constexpr int NTC = 256;
 constexpr int RCAP = 1024;    // recheck capacity per query
 
 // One 64-query x (64 TW)-centroid tile of D'.  The query tile sits in LDS
-// (FP32, split to BF16 in registers); wave wv owns centroids 16 TW wv ..
+// (its BF16 hi and lo terms, one padded row per query); wave wv owns centroids 16 TW wv ..
 // 16 TW wv + 16 TW - 1 of the tile (TW 16-wide column tiles whose BF16 hi/lo
 // rows are read from global memory one k block ahead) and all four 16-query
 // row tiles.  In a 32-wide k block, lane group h = l/16 supplies elements
@@ -51,11 +52,40 @@ constexpr int CTW = 2;   // column tiles per wave of the launched shape
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-__device__ inline void split_bf16(const float (&x)[8], bf16x8 &hi, bf16x8 &lo) {
+// Query norms (f32 squares accumulated in f64 in element order, as
+// launch_norm_f64) and the BF16 hi/lo split of the queries: 64 rows per
+// block, loaded once through LDS.
+__global__ __launch_bounds__(NTC) void coarse_prep_kernel(const float *__restrict__ queries, int nq,
+                                                          double *__restrict__ qn, __bf16 *__restrict__ qhi,
+                                                          __bf16 *__restrict__ qlo) {
+    __shared__ float tile[64][CD + 1];
+    const int tid = threadIdx.x;
+    const int64_t r0 = int64_t(blockIdx.x) * 64;
+    for (int e = tid; e < 64 * CD / 4; e += NTC) {
+        const int rr = e / (CD / 4), cc = (e % (CD / 4)) * 4;
+        const int64_t row = r0 + rr;
+        if (row >= nq) continue;
+        const float4 v = *reinterpret_cast<const float4 *>(queries + row * CD + cc);
+        const float xs[4] = {v.x, v.y, v.z, v.w};
+        __bf16 h[4], l[4];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        hi[j] = __bf16(x[j]);
-        lo[j] = __bf16(x[j] - float(hi[j]));
+        for (int u = 0; u < 4; ++u) {
+            tile[rr][cc + u] = xs[u];
+            h[u] = __bf16(xs[u]);
+            l[u] = __bf16(xs[u] - float(h[u]));
+        }
+        *reinterpret_cast<uint2 *>(qhi + row * CD + cc) = *reinterpret_cast<const uint2 *>(h);
+        *reinterpret_cast<uint2 *>(qlo + row * CD + cc) = *reinterpret_cast<const uint2 *>(l);
+    }
+    __syncthreads();
+    if (tid < 64 && r0 + tid < nq) {
+        double sum = 0.0;
+        for (int t = 0; t < CD; ++t) {
+            const float v = tile[tid][t];
+            const float p = v * v;
+            sum = sum + double(p);
+        }
+        qn[r0 + tid] = sum;
     }
 }
 
@@ -63,18 +93,21 @@ template <int TW>
 __global__ __launch_bounds__(NTC, (TW <= 2 ? 4 : 2)) void coarse_approx_kernel(const __bf16 *__restrict__ chi,
                                                             const __bf16 *__restrict__ clo,
                                                             const float *__restrict__ cn, int nlist,
-                                                            const float *__restrict__ queries,
+                                                            const __bf16 *__restrict__ qhi,
+                                                            const __bf16 *__restrict__ qlo,
                                                             const double *__restrict__ qn, int nq,
-                                                            float *__restrict__ out, int64_t ldo) {
+                                                            float *__restrict__ out, int64_t ldo,
+                                                            float *__restrict__ gmin, int64_t ldg) {
     extern __shared__ __align__(16) float lds[];
-    float *Qs = lds;  // [CQ][CP]
+    __bf16 *Qs = reinterpret_cast<__bf16 *>(lds);  // [CQ][CPB]: hi terms, then lo terms
     const int tid = threadIdx.x;
     const int64_t q0 = int64_t(blockIdx.y) * CQ;
-    for (int e = tid; e < CQ * CD / 4; e += NTC) {
-        const int row = e / (CD / 4), col = (e % (CD / 4)) * 4;
-        float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (q0 + row < nq) qv = *reinterpret_cast<const float4 *>(queries + (q0 + row) * CD + col);
-        *reinterpret_cast<float4 *>(Qs + row * CP + col) = qv;
+    for (int e = tid; e < CQ * 2 * CD / 8; e += NTC) {
+        const int row = e / (2 * CD / 8), part = (e % (2 * CD / 8)) * 8;  // part < CD: hi, else lo
+        uint4 x = make_uint4(0u, 0u, 0u, 0u);
+        if (q0 + row < nq)
+            x = *reinterpret_cast<const uint4 *>((part < CD ? qhi : qlo) + (q0 + row) * CD + (part % CD));
+        *reinterpret_cast<uint4 *>(Qs + row * CPB + part) = x;
     }
     const int lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 4, i16 = lane & 15;
@@ -88,6 +121,13 @@ __global__ __launch_bounds__(NTC, (TW <= 2 ? 4 : 2)) void coarse_approx_kernel(c
         bhn[t] = *reinterpret_cast<const bf16x8 *>(chi + crow[t]);
         bln[t] = *reinterpret_cast<const bf16x8 *>(clo + crow[t]);
     }
+    // the norms of this lane's 16 query rows, loaded now: a load in the
+    // epilogue would wait for the stores before it (one counter for both)
+    float qf[4][4];
+#pragma unroll
+    for (int qt = 0; qt < 4; ++qt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) qf[qt][r] = float(qn[min(q0 + 16 * qt + 4 * h + r, int64_t(nq) - 1)]);
     f32x4 acc[4][TW];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -111,12 +151,9 @@ __global__ __launch_bounds__(NTC, (TW <= 2 ? 4 : 2)) void coarse_approx_kernel(c
         }
 #pragma unroll
         for (int qt = 0; qt < 4; ++qt) {
-            const float *qr = Qs + (16 * qt + i16) * CP + k0 + 8 * h;
-            const float4 x0 = *reinterpret_cast<const float4 *>(qr);
-            const float4 x1 = *reinterpret_cast<const float4 *>(qr + 4);
-            const float xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-            bf16x8 ah, al;
-            split_bf16(xv, ah, al);
+            const __bf16 *qr = Qs + (16 * qt + i16) * CPB + k0 + 8 * h;
+            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(qr);
+            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(qr + CD);
 #pragma unroll
             for (int t = 0; t < TW; ++t) {
                 acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[qt][t], 0, 0, 0);
@@ -135,11 +172,11 @@ __global__ __launch_bounds__(NTC, (TW <= 2 ? 4 : 2)) void coarse_approx_kernel(c
     constexpr int RPS = 64 / (16 * TW);  // rows per store instruction
     __syncthreads();  // every wave is done with the query tile
     float *T = lds + wave * (16 * TS);
-    double cnc[TW];
+    float cnc[TW];
 #pragma unroll
     for (int t = 0; t < TW; ++t) {
         const int c = c0 + 16 * t + i16;
-        cnc[t] = c < nlist ? double(cn[c]) : 0.0;
+        cnc[t] = c < nlist ? cn[c] : 0.0f;
     }
     const int sc = lane % (16 * TW), sr = lane / (16 * TW);  // this lane's column and row within a store
     const int cl = c0 + sc;
@@ -147,10 +184,8 @@ __global__ __launch_bounds__(NTC, (TW <= 2 ? 4 : 2)) void coarse_approx_kernel(c
     for (int qt = 0; qt < 4; ++qt) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int64_t q = q0 + 16 * qt + 4 * h + r;
-            const double qq = q < nq ? qn[q] : 0.0;
 #pragma unroll
-            for (int t = 0; t < TW; ++t) T[(4 * h + r) * TS + 16 * t + i16] = float(qq + cnc[t]) - 2.0f * acc[qt][t][r];
+            for (int t = 0; t < TW; ++t) T[(4 * h + r) * TS + 16 * t + i16] = (qf[qt][r] + cnc[t]) - 2.0f * acc[qt][t][r];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -160,6 +195,26 @@ __global__ __launch_bounds__(NTC, (TW <= 2 ? 4 : 2)) void coarse_approx_kernel(c
             for (int row = 0; row < 16; row += RPS) {
                 const int64_t q = q0 + 16 * qt + row + sr;
                 if (q < nq) out[q * ldo + cl] = T[(row + sr) * TS + sc];
+            }
+        }
+        if (gmin && lane < 16 * TW) {
+            // the minimum of D' over the 16 centroids of one column tile
+            // (group (c0 + 16 g) / 16) for one query row
+            const int row = lane / TW, g = lane % TW;
+            const int cg = c0 + 16 * g;
+            const int64_t q = q0 + 16 * qt + row;
+            if (q < nq && cg < nlist) {
+                const float4 *tr = reinterpret_cast<const float4 *>(T + row * TS + 16 * g);
+                float mv = __builtin_huge_valf();
+#pragma unroll
+                for (int j4 = 0; j4 < 4; ++j4) {
+                    const float4 x = tr[j4];
+                    const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (cg + 4 * j4 + u < nlist) mv = fminf(mv, xs[u]);
+                }
+                gmin[q * ldg + cg / 16] = mv;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -206,7 +261,6 @@ __global__ __launch_bounds__(NTC) void coarse_select_kernel(const float *__restr
                                                             float *__restrict__ pdis,
                                                             const int32_t *__restrict__ only,
                                                             const int *__restrict__ only_cnt) {
-    if (only && int(blockIdx.x) >= *only_cnt) return;
     extern __shared__ __align__(16) unsigned char smem[];
     float *row = reinterpret_cast<float *>(smem);                        // [nlist] D', then exact D (fallback)
     uint64_t *ck = reinterpret_cast<uint64_t *>(smem + ((size_t(nlist) * 4 + 15) & ~size_t(15)));  // [RCAP]
@@ -215,128 +269,134 @@ __global__ __launch_bounds__(NTC) void coarse_select_kernel(const float *__restr
     __shared__ float qv[CD];
     __shared__ int ncand, cnt, eqt;
     const int tid = threadIdx.x;
-    const int64_t q = only ? only[blockIdx.x] : int64_t(blockIdx.x);
-    const double bn = qn[q];
-    if (tid < CD) qv[tid] = queries[q * CD + tid];
-    if (tid == 0) ncand = 0;
-    for (int c = tid; c < nlist; c += NTC) row[c] = approx[q * lda + c];
-    // An upper bound V of the w-th smallest D': a 256-bin histogram over
-    // [min, max], then V = the largest D' in the bins up to the one where the
-    // count reaches w (at least w values are <= V).
-    float mn = __builtin_huge_valf(), mx = -__builtin_huge_valf();
-    for (int c = tid; c < nlist; c += NTC) {
-        mn = fminf(mn, row[c]);
-        mx = fmaxf(mx, row[c]);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        mn = fminf(mn, __shfl_xor(mn, off));
-        mx = fmaxf(mx, __shfl_xor(mx, off));
-    }
-    __shared__ float wmn[NTC / 64], wmx[NTC / 64];
-    __shared__ uint32_t hist[256];
-    __shared__ int sdig, sbelow;
-    __shared__ uint32_t vmax;
-    if ((tid & 63) == 0) {
-        wmn[tid >> 6] = mn;
-        wmx[tid >> 6] = mx;
-    }
-    hist[tid] = 0u;
-    if (tid == 0) vmax = 0u;
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NTC / 64; ++i) {
-        mn = fminf(mn, wmn[i]);
-        mx = fmaxf(mx, wmx[i]);
-    }
-    const float scale = mx > mn ? 256.0f / (mx - mn) : 0.0f;
-    auto bin = [&](float v) { return min(255, max(0, int((v - mn) * scale))); };
-    for (int c = tid; c < nlist; c += NTC) atomicAdd(&hist[bin(row[c])], 1u);
-    __syncthreads();
-    if (tid < 64) {
-        const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
-        const uint32_t tot = h0 + h1 + h2 + h3;
-        uint32_t incl = tot;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t o = __shfl_up(incl, off, 64);
-            if (tid >= off) incl += o;
+    // without `only`: query blockIdx.x; with it: list entries blockIdx.x,
+    // blockIdx.x + gridDim.x, ...
+    for (int it = int(blockIdx.x);; it += int(gridDim.x)) {
+        if (only ? it >= *only_cnt : it != int(blockIdx.x)) break;  // uniform
+        const int64_t q = only ? only[it] : int64_t(it);
+        const double bn = qn[q];
+        if (tid < CD) qv[tid] = queries[q * CD + tid];
+        if (tid == 0) ncand = 0;
+        for (int c = tid; c < nlist; c += NTC) row[c] = approx[q * lda + c];
+        // An upper bound V of the w-th smallest D': a 256-bin histogram over
+        // [min, max], then V = the largest D' in the bins up to the one where the
+        // count reaches w (at least w values are <= V).
+        float mn = __builtin_huge_valf(), mx = -__builtin_huge_valf();
+        for (int c = tid; c < nlist; c += NTC) {
+            mn = fminf(mn, row[c]);
+            mx = fmaxf(mx, row[c]);
         }
-        const uint32_t excl = incl - tot;
-        const uint32_t need = uint32_t(min(w, nlist));
-        if (excl < need && incl >= need) {
-            uint32_t c = excl;
-            int dg;
-            if (c + h0 >= need) dg = 0;
-            else if (c + h0 + h1 >= need) { c += h0; dg = 1; }
-            else if (c + h0 + h1 + h2 >= need) { c += h0 + h1; dg = 2; }
-            else { c += h0 + h1 + h2; dg = 3; }
-            sdig = 4 * tid + dg;
-            sbelow = int(c);
+    #pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = fminf(mn, __shfl_xor(mn, off));
+            mx = fmaxf(mx, __shfl_xor(mx, off));
         }
-    }
-    __syncthreads();
-    const int bmax = sdig;
-    uint32_t mine = 0u;
-    for (int c = tid; c < nlist; c += NTC)
-        if (bin(row[c]) <= bmax) mine = max(mine, fkey(row[c]));
-    atomicMax(&vmax, mine);
-    __syncthreads();
-    const float vw = key_dist(uint64_t(vmax) << 32);
-    auto akey = [&](int i) -> uint64_t { return ckey(row[i], i); };
-    const float lim = vw + 3.0e-4f * float(bn + double(maxcn));
-    // recheck set
-    for (int i = tid; i < nlist; i += NTC) {
-        if (row[i] <= lim) {
-            const int p = atomicAdd(&ncand, 1);
-            if (p < RCAP) ck[p] = uint64_t(i);
+        __shared__ float wmn[NTC / 64], wmx[NTC / 64];
+        __shared__ uint32_t hist[256];
+        __shared__ int sdig, sbelow;
+        __shared__ uint32_t vmax;
+        if ((tid & 63) == 0) {
+            wmn[tid >> 6] = mn;
+            wmx[tid >> 6] = mx;
         }
-    }
-    __syncthreads();
-    const int n = ncand;
-    SelectResult r;
-    int nk;
-    if (n <= RCAP) {
-        for (int i = tid; i < n; i += NTC) {
-            const int c = int(ck[i]);
-            ck[i] = ckey(exact_dist(coarse + int64_t(c) * CD, qv, cn[c], bn), c);
+        hist[tid] = 0u;
+        if (tid == 0) vmax = 0u;
+        __syncthreads();
+    #pragma unroll
+        for (int i = 0; i < NTC / 64; ++i) {
+            mn = fminf(mn, wmn[i]);
+            mx = fmaxf(mx, wmx[i]);
+        }
+        const float scale = mx > mn ? 256.0f / (mx - mn) : 0.0f;
+        auto bin = [&](float v) { return min(255, max(0, int((v - mn) * scale))); };
+        for (int c = tid; c < nlist; c += NTC) atomicAdd(&hist[bin(row[c])], 1u);
+        __syncthreads();
+        if (tid < 64) {
+            const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
+            const uint32_t tot = h0 + h1 + h2 + h3;
+            uint32_t incl = tot;
+    #pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t o = __shfl_up(incl, off, 64);
+                if (tid >= off) incl += o;
+            }
+            const uint32_t excl = incl - tot;
+            const uint32_t need = uint32_t(min(w, nlist));
+            if (excl < need && incl >= need) {
+                uint32_t c = excl;
+                int dg;
+                if (c + h0 >= need) dg = 0;
+                else if (c + h0 + h1 >= need) { c += h0; dg = 1; }
+                else if (c + h0 + h1 + h2 >= need) { c += h0 + h1; dg = 2; }
+                else { c += h0 + h1 + h2; dg = 3; }
+                sdig = 4 * tid + dg;
+                sbelow = int(c);
+            }
         }
         __syncthreads();
-        auto kf = [&](int i) { return ck[i]; };
-        nk = n;
-        r = block_select<NTC>(kf, nk, w, sh);
-    } else {
-        // recheck set too large: exact distances of every centroid
+        const int bmax = sdig;
+        uint32_t mine = 0u;
+        for (int c = tid; c < nlist; c += NTC)
+            if (bin(row[c]) <= bmax) mine = max(mine, fkey(row[c]));
+        atomicMax(&vmax, mine);
         __syncthreads();
-        for (int c = tid; c < nlist; c += NTC) row[c] = exact_dist(coarse + int64_t(c) * CD, qv, cn[c], bn);
-        __syncthreads();
-        nk = nlist;
-        r = block_select<NTC>(akey, nk, w, sh);
-    }
-    // collect the w best into buf and sort
-    if (tid == 0) {
-        cnt = 0;
-        eqt = 0;
-    }
-    __syncthreads();
-    for (int i = tid; i < nk; i += NTC) {
-        const uint64_t k = (n <= RCAP) ? ck[i] : akey(i);
-        bool take = k < r.thr;
-        if (!take && r.eq > 0 && k == r.thr) take = atomicAdd(&eqt, 1) < r.eq;
-        if (take) {
-            const int p = atomicAdd(&cnt, 1);
-            if (p < RCAP) buf[p] = k;
+        const float vw = key_dist(uint64_t(vmax) << 32);
+        auto akey = [&](int i) -> uint64_t { return ckey(row[i], i); };
+        const float lim = vw + 3.0e-4f * float(bn + double(maxcn));
+        // recheck set
+        for (int i = tid; i < nlist; i += NTC) {
+            if (row[i] <= lim) {
+                const int p = atomicAdd(&ncand, 1);
+                if (p < RCAP) ck[p] = uint64_t(i);
+            }
         }
-    }
-    __syncthreads();
-    const int c = min(cnt, RCAP);
-    const int np = next_pow2(c > 0 ? c : 1);
-    for (int i = c + tid; i < np; i += NTC) buf[i] = kNoKey;
-    __syncthreads();
-    block_bitonic_sort<NTC>(buf, np);
-    for (int i = tid; i < w; i += NTC) {
-        probes[q * w + i] = i < c ? key_id(buf[i]) : -1;
-        pdis[q * w + i] = i < c ? key_dist(buf[i]) : __builtin_huge_valf();
+        __syncthreads();
+        const int n = ncand;
+        SelectResult r;
+        int nk;
+        if (n <= RCAP) {
+            for (int i = tid; i < n; i += NTC) {
+                const int c = int(ck[i]);
+                ck[i] = ckey(exact_dist(coarse + int64_t(c) * CD, qv, cn[c], bn), c);
+            }
+            __syncthreads();
+            auto kf = [&](int i) { return ck[i]; };
+            nk = n;
+            r = block_select<NTC>(kf, nk, w, sh);
+        } else {
+            // recheck set too large: exact distances of every centroid
+            __syncthreads();
+            for (int c = tid; c < nlist; c += NTC) row[c] = exact_dist(coarse + int64_t(c) * CD, qv, cn[c], bn);
+            __syncthreads();
+            nk = nlist;
+            r = block_select<NTC>(akey, nk, w, sh);
+        }
+        // collect the w best into buf and sort
+        if (tid == 0) {
+            cnt = 0;
+            eqt = 0;
+        }
+        __syncthreads();
+        for (int i = tid; i < nk; i += NTC) {
+            const uint64_t k = (n <= RCAP) ? ck[i] : akey(i);
+            bool take = k < r.thr;
+            if (!take && r.eq > 0 && k == r.thr) take = atomicAdd(&eqt, 1) < r.eq;
+            if (take) {
+                const int p = atomicAdd(&cnt, 1);
+                if (p < RCAP) buf[p] = k;
+            }
+        }
+        __syncthreads();
+        const int c = min(cnt, RCAP);
+        const int np = next_pow2(c > 0 ? c : 1);
+        for (int i = c + tid; i < np; i += NTC) buf[i] = kNoKey;
+        __syncthreads();
+        block_bitonic_sort<NTC>(buf, np);
+        for (int i = tid; i < w; i += NTC) {
+            probes[q * w + i] = i < c ? key_id(buf[i]) : -1;
+            pdis[q * w + i] = i < c ? key_dist(buf[i]) : __builtin_huge_valf();
+        }
+        __syncthreads();  // the shared buffers are reused by the next query
     }
 }
 
@@ -415,6 +475,141 @@ __global__ __launch_bounds__(NTC) void coarse_select_wave_kernel(const float *__
     for (int j = lane; j < n; j += 64) {
         const int c = int(cidx[j]);
         keys[j] = ckey(exact_dist(coarse + int64_t(c) * CD, qv, cn[c], bn), c);
+    }
+    wave_sync();
+    for (int j = lane; j < n; j += 64) {
+        const uint64_t k = keys[j];
+        int rank = 0;
+        for (int i = 0; i < n; ++i) rank += keys[i] < k;
+        if (rank < w) {
+            probes[q * w + rank] = key_id(k);
+            pdis[q * w + rank] = key_dist(k);
+        }
+    }
+}
+
+// One wavefront per query from the 16-centroid group minima of D' (written
+// by coarse_approx_kernel): the w-th smallest group minimum V bounds the w-th
+// smallest D' from above (the minima are w distinct centroids), found by a
+// bitwise radix search over ballot counts of their order keys; a centroid of
+// the recheck set {D' <= V + 2 M_q} lies in a group whose minimum is within
+// the limit, so only those groups' D' are read (16 lanes per group).  Lane l
+// holds the minima of groups l + 64 j, j < GM.  A query whose passing groups
+// exceed GCAP, or whose recheck set exceeds WCAP or is short of w, goes to
+// `redo`.
+constexpr int GCAP = 256;
+
+template <int GM>
+__global__ __launch_bounds__(NTC) void coarse_select_group_kernel(const float *__restrict__ coarse,
+                                                                  const float *__restrict__ cn, float maxcn,
+                                                                  int nlist, const float *__restrict__ queries,
+                                                                  const double *__restrict__ qn, int nq,
+                                                                  const float *__restrict__ approx, int64_t lda,
+                                                                  const float *__restrict__ gmin, int64_t ldg,
+                                                                  int w, int32_t *__restrict__ probes,
+                                                                  float *__restrict__ pdis,
+                                                                  int32_t *__restrict__ redo,
+                                                                  int *__restrict__ redo_cnt) {
+    __shared__ uint32_t cidx_all[NTC / 64][WCAP];
+    __shared__ uint64_t keys_all[NTC / 64][WCAP];
+    __shared__ float qv_all[NTC / 64][CD];
+    __shared__ uint16_t gl_all[NTC / 64][GCAP];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t q = int64_t(blockIdx.x) * (NTC / 64) + wv;
+    if (q >= nq) return;  // wave-uniform; no workgroup barriers below
+    uint32_t *cidx = cidx_all[wv];
+    uint64_t *keys = keys_all[wv];
+    float *qv = qv_all[wv];
+    uint16_t *gl = gl_all[wv];
+    const int ng = (nlist + 15) / 16;
+    const float *gq = gmin + q * ldg;
+    uint32_t gk[GM];
+#pragma unroll
+    for (int j = 0; j < GM; ++j) {
+        const int g = lane + 64 * j;
+        gk[j] = fkey(g < ng ? gq[g] : __builtin_huge_valf());
+    }
+    const double bn = qn[q];
+    qv[lane] = queries[q * CD + lane];
+    qv[lane + 64] = queries[q * CD + lane + 64];
+    // An upper bound of the w-th smallest group minimum: MSB-first search for
+    // the smallest key with at least w minima at or below it (integer order
+    // = float order), from the first bit below those all minima share, over
+    // 16 bits; the bound is the largest key of the bucket found.
+    uint32_t kmn = 0xFFFFFFFFu, kmx = 0u;
+#pragma unroll
+    for (int j = 0; j < GM; ++j)
+        if (lane + 64 * j < ng) {
+            kmn = min(kmn, gk[j]);
+            kmx = max(kmx, gk[j]);
+        }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        kmn = min(kmn, uint32_t(__shfl_xor(int(kmn), off)));
+        kmx = max(kmx, uint32_t(__shfl_xor(int(kmx), off)));
+    }
+    const int cp = kmn == kmx ? 32 : __clz(int(kmn ^ kmx));  // shared leading bits
+    uint32_t prefix = cp == 32 ? kmn : (cp == 0 ? 0u : (kmn >> (32 - cp)) << (32 - cp));
+    const int top = 31 - cp, bot = max(0, top - 15);
+    int need = w;
+    for (int bit = top; bit >= bot; --bit) {
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < GM; ++j) cnt += __popcll(__ballot(((gk[j] ^ prefix) >> bit) == 0u));
+        if (cnt < need) {
+            need -= cnt;
+            prefix |= 1u << bit;
+        }
+    }
+    if (bot > 0) prefix |= (1u << bot) - 1u;
+    const float vw = key_dist(uint64_t(prefix) << 32);
+    const float lim = vw + 3.0e-4f * float(bn + double(maxcn));
+    const uint32_t klim = fkey(lim);
+    int ngp = 0;
+#pragma unroll
+    for (int j = 0; j < GM; ++j) {
+        const int g = lane + 64 * j;
+        const bool pass = g < ng && gk[j] <= klim;
+        const uint64_t bal = __ballot(pass);
+        const int pos = ngp + int(__builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u)));
+        if (pass && pos < GCAP) gl[pos] = uint16_t(g);
+        ngp += __popcll(bal);
+    }
+    if (ngp > GCAP || !(vw < __builtin_huge_valf())) {
+        if (lane == 0) redo[atomicAdd(redo_cnt, 1)] = int32_t(q);
+        return;
+    }
+    wave_sync();
+    // D' of the passing groups, four groups per load round, four rounds in flight
+    const float *row = approx + q * lda;
+    int n = 0;
+    for (int g0 = 0; g0 < ngp; g0 += 16) {
+        float v[4];
+        int c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int gi = g0 + 4 * u + (lane >> 4);
+            c[u] = gi < ngp ? 16 * int(gl[gi]) + (lane & 15) : nlist;
+            v[u] = c[u] < nlist ? row[c[u]] : __builtin_huge_valf();
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool pass = c[u] < nlist && v[u] <= lim;
+            const uint64_t bal = __ballot(pass);
+            const int pos = n + int(__builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo(uint32_t(bal), 0u)));
+            if (pass && pos < WCAP) cidx[pos] = uint32_t(c[u]);
+            n += __popcll(bal);
+        }
+    }
+    if (n > WCAP || n < w) {
+        if (lane == 0) redo[atomicAdd(redo_cnt, 1)] = int32_t(q);
+        return;
+    }
+    wave_sync();
+    for (int j = lane; j < n; j += 64) {
+        const int ci = int(cidx[j]);
+        keys[j] = ckey(exact_dist(coarse + int64_t(ci) * CD, qv, cn[ci], bn), ci);
     }
     wave_sync();
     for (int j = lane; j < n; j += 64) {
@@ -559,20 +754,29 @@ hipError_t launch_coarse_split(const float *coarse, int64_t n, void *hi, void *l
     return hipGetLastError();
 }
 
-hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const float *queries,
-                                const double *qn, int nq, float *approx, int64_t lda, hipStream_t s) {
+hipError_t launch_coarse_prep(const float *queries, int nq, double *qn, void *qhi, void *qlo, hipStream_t s) {
+    if (nq <= 0) return hipSuccess;
+    coarse_prep_kernel<<<unsigned((nq + 63) / 64), NTC, 0, s>>>(queries, nq, qn, static_cast<__bf16 *>(qhi),
+                                                                static_cast<__bf16 *>(qlo));
+    return hipGetLastError();
+}
+
+hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const void *qhi,
+                                const void *qlo, const double *qn, int nq, float *approx, int64_t lda, float *gmin,
+                                int64_t ldg, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     constexpr int CC = 64 * CTW;  // centroids per tile
     dim3 grid(unsigned((nlist + CC - 1) / CC), unsigned((nq + CQ - 1) / CQ));
-    coarse_approx_kernel<CTW><<<grid, NTC, CQ * CP * 4, s>>>(static_cast<const __bf16 *>(chi),
-                                                             static_cast<const __bf16 *>(clo), cn, nlist, queries, qn,
-                                                             nq, approx, lda);
+    coarse_approx_kernel<CTW><<<grid, NTC, CQ * CPB * 2, s>>>(
+        static_cast<const __bf16 *>(chi), static_cast<const __bf16 *>(clo), cn, nlist,
+        static_cast<const __bf16 *>(qhi), static_cast<const __bf16 *>(qlo), qn, nq, approx, lda, gmin, ldg);
     return hipGetLastError();
 }
 
 hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxcn, int nlist, const float *queries,
-                                const double *qn, int nq, int w, const float *approx, int64_t lda, int32_t *probes,
-                                float *pdis, int32_t *redo, int *redo_cnt, hipStream_t s) {
+                                const double *qn, int nq, int w, const float *approx, int64_t lda,
+                                const float *gmin, int64_t ldg, int32_t *probes, float *pdis, int32_t *redo,
+                                int *redo_cnt, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     if (nlist > kMaxMfmaLists) return hipErrorInvalidValue;
     static size_t lds_set = 64 * 1024;  // dynamic LDS allowed so far (64 KiB needs no attribute)
@@ -590,7 +794,21 @@ hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxc
 #define MIVF_STREAM_SELECT(G)                                                                                   \
     coarse_select_stream_kernel<G><<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda, w, \
                                                          probes, pdis, redo, redo_cnt)
-        if (nlist <= 64 * NPL && w <= 32)
+        const int ngm = (nlist + 15) / 16 <= 256 ? 4 : (nlist + 15) / 16 <= 512 ? 8 : (nlist + 15) / 16 <= 1024 ? 16 : 32;
+#define MIVF_GROUP_SELECT(G)                                                                                    \
+    coarse_select_group_kernel<G><<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda,  \
+                                                        gmin, ldg, w, probes, pdis, redo, redo_cnt)
+        const bool grp = gmin && w <= 64 && (nlist + 15) / 16 >= w;
+        if (grp && ngm == 4)
+            MIVF_GROUP_SELECT(4);
+        else if (grp && ngm == 8)
+            MIVF_GROUP_SELECT(8);
+        else if (grp && ngm == 16)
+            MIVF_GROUP_SELECT(16);
+        else if (grp)
+            MIVF_GROUP_SELECT(32);
+#undef MIVF_GROUP_SELECT
+        else if (nlist <= 64 * NPL && w <= 32)
             coarse_select_wave_kernel<<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda,
                                                             w, probes, pdis, redo, redo_cnt);
         else if (w <= 16)
@@ -602,7 +820,8 @@ hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxc
         else
             MIVF_STREAM_SELECT(8);
 #undef MIVF_STREAM_SELECT
-        coarse_select_kernel<<<unsigned(nq), NTC, coarse_select_lds(nlist), s>>>(
+        // the few queries sent back: a small grid walks the list
+        coarse_select_kernel<<<unsigned(std::min(nq, 1024)), NTC, coarse_select_lds(nlist), s>>>(
             coarse, cn, maxcn, nlist, queries, qn, approx, lda, w, probes, pdis, redo, redo_cnt);
     } else {
         coarse_select_kernel<<<unsigned(nq), NTC, coarse_select_lds(nlist), s>>>(

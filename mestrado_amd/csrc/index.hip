@@ -298,7 +298,7 @@ int search_core(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, cons
     const int32_t *probes = d_probes_in;
     if (!probes) {
         MIVF_TRY(ws.qn.ensure(sizeof(double) * nq));
-        MIVF_TRY(ws.dmat.ensure(sizeof(float) * size_t(nq) * nlist));
+        MIVF_TRY(ws.dmat.ensure(coarse_ws_bytes(nq, nlist)));
         MIVF_TRY(ws.probes.ensure(sizeof(int32_t) * size_t(nq) * wmax));
         MIVF_TRY(ws.coadis.ensure(sizeof(float) * size_t(nq) * wmax));
         MIVF_TRY(coarse_stage(ix, ws, d_q, nq, wmax, ws.probes.as<int32_t>(), ws.coadis.as<float>(), st, &tm));
@@ -384,23 +384,30 @@ int finish_timer(mivf_index *ix, Timer &tm) {
 
 namespace mivf {
 
-// Coarse top-w of nq queries into probes/dis [nq][w] (needs ix->ws.dmat sized
-// nq * nlist): query norms, distances (timer mark), selection.
+// Coarse top-w of nq queries into probes/dis [nq][w] (needs ws.dmat sized
+// coarse_ws_bytes(nq, nlist)): query norms, distances (timer mark), selection.
 int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_t *probes, float *dis, hipStream_t st,
                  Timer *tm) {
     const int d = ix->d, nlist = ix->nlist;
     MIVF_TRY(ws.qn.ensure(sizeof(double) * nq));
-    MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ws.qn.as<double>(), st));
     if (ix->coarse_mfma && coarse_mfma_ok(d, nlist, w) && (reinterpret_cast<uintptr_t>(d_q) & 15u) == 0) {
-        MIVF_CHECK_HIP(launch_coarse_approx(ix->coarse_hi.p, ix->coarse_lo.p, ix->coarse_n32.as<float>(), nlist, d_q,
-                                            ws.qn.as<double>(), nq, ws.dmat.as<float>(), nlist, st));
+        // group minima and query terms follow the matrix in ws.dmat
+        const CoarseWs lay = coarse_ws(nq, nlist);
+        unsigned char *wb = ws.dmat.as<unsigned char>();
+        float *gmin = reinterpret_cast<float *>(wb + lay.gmin);
+        const int64_t ldg = (nlist + 15) / 16;
+        MIVF_CHECK_HIP(launch_coarse_prep(d_q, nq, ws.qn.as<double>(), wb + lay.qhi, wb + lay.qlo, st));
+        MIVF_CHECK_HIP(launch_coarse_approx(ix->coarse_hi.p, ix->coarse_lo.p, ix->coarse_n32.as<float>(), nlist,
+                                            wb + lay.qhi, wb + lay.qlo, ws.qn.as<double>(), nq, ws.dmat.as<float>(),
+                                            nlist, gmin, ldg, st));
         if (tm) MIVF_TRY(tm->mark());
         MIVF_TRY(ws.redo.ensure(sizeof(int32_t) * (size_t(nq) + 1)));
         int32_t *redo = ws.redo.as<int32_t>();
         MIVF_CHECK_HIP(launch_coarse_select(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), ix->maxcn, nlist,
-                                            d_q, ws.qn.as<double>(), nq, w, ws.dmat.as<float>(), nlist, probes,
-                                            dis, redo + 1, redo, st));
+                                            d_q, ws.qn.as<double>(), nq, w, ws.dmat.as<float>(), nlist, gmin, ldg,
+                                            probes, dis, redo + 1, redo, st));
     } else {
+        MIVF_CHECK_HIP(launch_norm_f64(d_q, nq, d, d, ws.qn.as<double>(), st));
         MIVF_CHECK_HIP(launch_dist_store(ix->coarse.as<float>(), d, ix->coarse_n32.as<float>(), nlist, d_q, d,
                                          ws.qn.as<double>(), nq, d, ws.dmat.as<float>(), nlist, st));
         if (tm) MIVF_TRY(tm->mark());
@@ -421,7 +428,7 @@ int search_entry(mivf_index *ix, Ws &ws, const float *d_q, int nq, int wmax, con
     std::fill(ix->timings, ix->timings + 8, 0.0f);
     // bound the coarse distance matrix and the residual rows of the probes
     // to ~512 MiB each per chunk
-    const int64_t per_q = std::max<int64_t>(4 * int64_t(ix->nlist), int64_t(wmax) * (4 * ix->d + 8 * ix->m));
+    const int64_t per_q = std::max<int64_t>(int64_t(coarse_ws_bytes(1, ix->nlist)), int64_t(wmax) * (4 * ix->d + 8 * ix->m));
     const int64_t maxq = std::max<int64_t>(1, (int64_t(512) << 20) / per_q);
     for (int64_t q0 = 0; q0 < nq; q0 += maxq) {
         const int nc = int(std::min<int64_t>(maxq, nq - q0));
@@ -482,7 +489,7 @@ size_t noalloc_layout(const mivf_index *ix, int nq, int w, int k, unsigned char 
     int ppg = 0;
     const int ngroups = choose_groups(ix, nq, w, &ppg);
     take(ws.qn, sizeof(double) * q);
-    take(ws.dmat, sizeof(float) * q * size_t(ix->nlist));
+    take(ws.dmat, coarse_ws_bytes(int64_t(q), ix->nlist));
     take(ws.probes, sizeof(int32_t) * qw);
     take(ws.coadis, sizeof(float) * qw);
     take(ws.redo, sizeof(int32_t) * (q + 1));
@@ -824,11 +831,11 @@ int mivf_coarse(mivf_index *ix, const float *d_q, int nq, int w, int32_t *d_idx,
     if (nq <= 0) return kOk;
     MIVF_REQUIRE(d_q && d_idx && d_dis, kInvalid, "coarse: null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const int64_t maxq = std::max<int64_t>(1, (int64_t(512) << 20) / (4 * int64_t(ix->nlist)));
+    const int64_t maxq = std::max<int64_t>(1, (int64_t(512) << 20) / int64_t(coarse_ws_bytes(1, ix->nlist)));
     for (int64_t q0 = 0; q0 < nq; q0 += maxq) {
         const int nc = int(std::min<int64_t>(maxq, nq - q0));
         MIVF_TRY(ix->ws.qn.ensure(sizeof(double) * nc));
-        MIVF_TRY(ix->ws.dmat.ensure(sizeof(float) * size_t(nc) * ix->nlist));
+        MIVF_TRY(ix->ws.dmat.ensure(coarse_ws_bytes(nc, ix->nlist)));
         MIVF_TRY(coarse_stage(ix, ix->ws, d_q + q0 * ix->d, nc, w, d_idx + q0 * w, d_dis + q0 * w, st, nullptr));
     }
     return kOk;
@@ -921,7 +928,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(tm.start(ix->profile, st, ix->ev));
     // 1. coarse top-w
     MIVF_TRY(ix->ws.qn.ensure(sizeof(double) * nq));
-    MIVF_TRY(ix->ws.dmat.ensure(sizeof(float) * size_t(nq) * nlist));
+    MIVF_TRY(ix->ws.dmat.ensure(coarse_ws_bytes(nq, nlist)));
     MIVF_TRY(ix->ws.probes.ensure(sizeof(int32_t) * size_t(nq) * w));
     MIVF_TRY(ix->ws.coadis.ensure(sizeof(float) * size_t(nq) * w));
     const int64_t np = int64_t(nq) * w;
@@ -1166,7 +1173,7 @@ int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
                  "fast_seed_tau: null or unaligned pointer");
     if (nq <= 0) return kOk;
     MIVF_TRY(upload_lists(ix));
-    MIVF_TRY(ix->ws.dmat.ensure(sizeof(float) * size_t(nq) * ix->nlist));
+    MIVF_TRY(ix->ws.dmat.ensure(coarse_ws_bytes(nq, ix->nlist)));
     MIVF_TRY(ix->ws.probes.ensure(sizeof(int32_t) * size_t(nq) * w));
     MIVF_TRY(ix->ws.coadis.ensure(sizeof(float) * size_t(nq) * w));
     MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));

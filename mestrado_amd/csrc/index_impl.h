@@ -217,7 +217,23 @@ int list_major_enqueue(mivf_index *ix, void *ws, const float *d_q, int nq, int w
                        const int32_t *d_k, int kmax, int32_t *d_ids, float *d_dis, int *h_flags, hipStream_t st);
 int list_major_finish(mivf_index *ix, void *ws, const float *d_q, int nq, const int32_t *h_w, const int32_t *h_k,
                       int kmax, int32_t *d_ids, float *d_dis, const int *h_flags, hipStream_t st, int *n_over);
-// Coarse top-w of nq queries into probes/dis [nq][w] (ws.dmat sized nq * nlist).
+// ws.dmat of a coarse stage of nq queries: the distance matrix [nq][nlist]
+// (offset 0), the 16-centroid group minima [nq][ceil(nlist / 16)] and the
+// BF16 hi / lo terms of the queries [nq][128] each (byte offsets, 256-aligned).
+struct CoarseWs {
+    size_t gmin, qhi, qlo, bytes;
+};
+inline CoarseWs coarse_ws(int64_t nq, int nlist) {
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    CoarseWs l;
+    l.gmin = up(sizeof(float) * size_t(nq) * size_t(nlist));
+    l.qhi = up(l.gmin + sizeof(float) * size_t(nq) * size_t((nlist + 15) / 16));
+    l.qlo = up(l.qhi + 2 * size_t(nq) * 128);
+    l.bytes = up(l.qlo + 2 * size_t(nq) * 128);
+    return l;
+}
+inline size_t coarse_ws_bytes(int64_t nq, int nlist) { return coarse_ws(nq, nlist).bytes; }
+// Coarse top-w of nq queries into probes/dis [nq][w] (ws.dmat sized coarse_ws_bytes).
 int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_t *probes, float *dis, hipStream_t st,
                  Timer *tm);
 

@@ -123,12 +123,20 @@ bool coarse_mfma_ok(int d, int nlist, int w);
 size_t coarse_select_lds(int nlist);
 // chi / clo: BF16 hi and lo terms of the centroids [nlist][d] (launch_coarse_split).
 hipError_t launch_coarse_split(const float *coarse, int64_t n, void *hi, void *lo, hipStream_t s);
-hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const float *queries,
-                                const double *qn, int nq, float *approx, int64_t lda, hipStream_t s);
-// redo [nq] and redo_cnt [1]: scratch for queries whose recheck set is large
+// Query norms qn [nq] (as launch_norm_f64) and BF16 hi / lo terms of the
+// queries qhi, qlo [nq][128].
+hipError_t launch_coarse_prep(const float *queries, int nq, double *qn, void *qhi, void *qlo, hipStream_t s);
+// approx [nq][lda]: D'; gmin [nq][ldg] (optional, ldg >= ceil(nlist / 16)):
+// the minimum of D' over each group of 16 consecutive centroids.
+hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const void *qhi,
+                                const void *qlo, const double *qn, int nq, float *approx, int64_t lda, float *gmin,
+                                int64_t ldg, hipStream_t s);
+// redo [nq] and redo_cnt [1]: scratch for queries whose recheck set is large;
+// gmin: the group minima of launch_coarse_approx, or null
 hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxcn, int nlist, const float *queries,
-                                const double *qn, int nq, int w, const float *approx, int64_t lda, int32_t *probes,
-                                float *pdis, int32_t *redo, int *redo_cnt, hipStream_t s);
+                                const double *qn, int nq, int w, const float *approx, int64_t lda,
+                                const float *gmin, int64_t ldg, int32_t *probes, float *pdis, int32_t *redo,
+                                int *redo_cnt, hipStream_t s);
 
 // Residuals: r[i] = x[i] - c[assign[i]]
 hipError_t launch_residual(const float *x, int64_t n, int d, const float *c,

@@ -90,10 +90,10 @@ struct mivf_server {
 namespace {
 
 // Device state a request holds while in flight: its share of the coarse
-// distance matrix, probes, residual rows, partial top-k lists and the slot's
-// query/output rows.
+// distance matrix (with the group minima), probes, residual rows, partial
+// top-k lists and the slot's query/output rows.
 int64_t request_mem(const mivf_index *ix, int nq, int w, int k) {
-    const int64_t per_q = int64_t(ix->nlist) * 4 + int64_t(w) * 8 + 8 + int64_t(ix->d) * 4 +
+    const int64_t per_q = int64_t(coarse_ws_bytes(1, ix->nlist)) + int64_t(w) * 8 + 8 + int64_t(ix->d) * 4 +
                           int64_t(k) * 8 * (1 + 8) + int64_t(w) * (int64_t(ix->d) * 4 + int64_t(ix->m) * 8);
     return per_q * nq;
 }
