@@ -228,7 +228,10 @@ int mivf_fast_select(const uint64_t *d_cand, const int32_t *d_cnt, int capq, int
  * >= 2048 queries grouped by nearest list; schedule only, default 1),
  * "fast_seed_lists" (lists per query scanned exactly to seed
  * mivf_search_fast, default 1), "fast_seed_cap" (vectors of those lists
- * that enter the seed bound, default 2048, 0 = all), "fast_capacity",
+ * that enter the seed bound, default 2048, 0 = all), "fast_seed_slack"
+ * (8-bit table quanta above the seed bound that still pass the filter of
+ * mivf_search_fast, 0..64, default 2; 3m/4 + 2 keeps every vector whose
+ * ADC distance is within the seed bound), "fast_capacity",
  * "fast_running_bound" (1 =
  * the 8-bit scan tightens each query's filter as its candidates accumulate;
  * results unchanged, default 1), "batched_group", "batched_threads",

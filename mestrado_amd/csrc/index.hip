@@ -1027,6 +1027,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         a.capq = capq;
         a.ks = ix->ks;
         a.err = ix->errflag.as<int>();
+        a.slack = float(ix->fast_slack);
         a.tau = ix->f_tau.as<float>();
         a.runthr = ix->fast_running ? ix->f_rthr.as<float>() : nullptr;
         a.rhist = ix->fast_running ? ix->f_rhist.as<int>() : nullptr;
@@ -1137,6 +1138,7 @@ int mivf_fast_tables(mivf_index *ix, const float *d_q, int n, int list, uint8_t 
     a.pb = ix->fpb.p;
     a.pmax = ix->pmax.as<float>();
     a.ks = ix->ks;
+    a.slack = float(ix->fast_slack);
     MIVF_CHECK_HIP(launch_fast_tables(a, ix->m, list, n, d_t8, d_delta, nullptr));
     MIVF_CHECK_HIP(hipDeviceSynchronize());
     return kOk;
@@ -1225,6 +1227,9 @@ int mivf_set_option(mivf_index *ix, const char *name, int64_t value) try {
     if (n == "probes_per_group") {
         MIVF_REQUIRE(value >= 0 && value <= 1024, kInvalid, "probes_per_group must be in 0..1024");
         ix->ppg_opt = int(value);
+    } else if (n == "fast_seed_slack") {
+        MIVF_REQUIRE(value >= 0 && value <= 64, kInvalid, "fast_seed_slack must be in 0..64");
+        ix->fast_slack = int(value);
     } else if (n == "fast_seed_cap") {
         MIVF_REQUIRE(value == 0 || (value >= 1 && value <= 8192), kInvalid, "fast_seed_cap must be 0 or in 1..8192");
         ix->fast_seed_cap = int(value);

@@ -145,6 +145,7 @@ struct mivf_index {
     bool fast_running = true;  // running per-query bound in the reduced-precision scan
     int fast_seed = 1;        // lists scanned exactly to seed the reduced-precision filter
     int fast_seed_cap = 2048; // vectors entering the seed bound (0 = all of the seed lists)
+    int fast_slack = 2;       // option fast_seed_slack: quanta above the seed bound that pass the 8-bit filter
     int exact_seed_cap = 4096; // the same for the list-major exact kernels
     int fast_cap = 65536;     // candidate capacity per query of the reduced-precision form
     int batched_cap = 16384;  // candidate capacity per query of the batched exact form (>= 2 kmax)

@@ -217,6 +217,7 @@ struct FastArgs {
     float *runthr;              // [nq] running bound on the kl-th candidate distance (+inf), or nullptr
     int *rhist;                 // [nq][fast_hist_bins()] candidate-distance histograms (zeroed), with runthr
     int kl;
+    float slack;                // seed filter slack in table quanta (8-bit tables of fast.hip)
     const float *pq;            // [m][ks][ds] (the FP32-built table types of fastvar.hip)
     const double *pq_norm;      // [m][ks] f64 norms
 };

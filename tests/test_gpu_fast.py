@@ -373,6 +373,43 @@ def test_fast_running_bound_changes_nothing(mivf, zipf16, front):
     assert max(cands) <= ref_c
 
 
+def test_fast_seed_slack(mivf, orc, zipf16, zipf16_gt):
+    """The seed filter's slack (8-bit quanta above the seed bound that still
+    pass): every setting is deterministic and returns exact rerank rows.
+    With the running bound off, the candidates are exactly the vectors that
+    pass the filter, so their count grows with the slack; 3m/4 + 2 = 14 keeps
+    every vector within the seed bound, and the default (2) keeps its recall
+    to within a few queries."""
+    ix, index, query, base = zipf16
+    gt = zipf16_gt
+    coarse_ids, _ = ix.coarse(query, 32)
+    lo = list_of_ids(index[2], index[3], base.shape[0])
+    res = {}
+    ix.set_option("profile", 1)
+    try:
+        ix.set_option("fast_running_bound", 0)
+        for slack in (0, 2, 14):
+            ix.set_option("fast_seed_slack", slack)
+            ids, dis = ix.search_fast(query, 32, 256, 100, base)
+            cands, over = ix.fast_stats()
+            ids2, dis2 = ix.search_fast(query, 32, 256, 100, base)
+            assert over == 0
+            assert np.array_equal(ids, ids2) and np.array_equal(dis.view(np.uint32), dis2.view(np.uint32)), \
+                f"slack {slack}: not deterministic"
+            check_exact_rows(orc, ids, dis, base, query, coarse_ids, lo)
+            res[slack] = (ids, cands)
+    finally:
+        ix.set_option("fast_seed_slack", 2)  # the default
+        ix.set_option("fast_running_bound", 1)
+        ix.set_option("profile", 0)
+    assert res[0][1] <= res[2][1] <= res[14][1], [res[s][1] for s in (0, 2, 14)]
+    r2 = orc.recall_at(res[2][0], gt[:, None], 100)
+    r14 = orc.recall_at(res[14][0], gt[:, None], 100)
+    assert r2 >= r14 - 3 * 100.0 / query.shape[0], f"recall@100 {r2} with slack 2 vs {r14} with 14"
+    with pytest.raises(mivf.MivfError):
+        ix.set_option("fast_seed_slack", 65)
+
+
 def var_adc_bounds(orc, pq, res, codes, tables):
     """Exact ADC of codes for residual res (oracle tables + sumidxtab2) and
     the error bound of table type `tables` for each vector."""
