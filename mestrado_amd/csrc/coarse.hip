@@ -26,6 +26,9 @@
 // <= v' + M_q; a centroid of the exact top-w therefore has D' <= D + M_q <=
 // v' + 2 M_q.  A centroid outside the set has D > v' + M_q >= the w-th exact
 // distance, so it loses to all of the top-w, ties included.
+#include <algorithm>
+#include <atomic>
+
 #include "select_dev.h"
 
 namespace mivf {
@@ -39,14 +42,16 @@ constexpr int CPB = 2 * CD + 8;  // padded LDS row of a query's hi and lo terms 
 constexpr int NTC = 256;
 constexpr int RCAP = 1024;    // recheck capacity per query
 
-// One 64-query x (64 TW)-centroid tile of D'.  The query tile sits in LDS
-// (its BF16 hi and lo terms, one padded row per query); wave wv owns centroids 16 TW wv ..
-// 16 TW wv + 16 TW - 1 of the tile (TW 16-wide column tiles whose BF16 hi/lo
-// rows are read from global memory one k block ahead) and all four 16-query
+// Units of D': one 64-query tile x one column block of 64 TW centroids.  A
+// persistent grid gives each workgroup a contiguous range of units (query
+// tile major, balanced to within one unit); the query tile (BF16 hi and lo
+// terms, one padded row per query) stays in LDS while the range stays in it.
+// Wave wv owns centroids 16 TW wv .. 16 TW wv + 16 TW - 1 of each column
+// block (TW 16-wide column tiles whose BF16 hi/lo rows are read from global
+// memory one k block ahead, into the next unit too) and all four 16-query
 // row tiles.  In a 32-wide k block, lane group h = l/16 supplies elements
 // k0 + 8h .. k0 + 8h + 7 of both operands (the same k permutation on both
-// sides of the sum).  TW = 2 halves the accumulators of TW = 4, so four waves
-// per SIMD fit instead of two.
+// sides of the sum).  TW = 2 keeps the accumulators at 32 registers.
 constexpr int CQ = 64;   // queries per tile
 constexpr int CTW = 2;   // column tiles per wave of the launched shape
 
@@ -90,136 +95,151 @@ __global__ __launch_bounds__(NTC) void coarse_prep_kernel(const float *__restric
 }
 
 template <int TW>
-__global__ __launch_bounds__(NTC, (TW <= 2 ? 4 : 2)) void coarse_approx_kernel(const __bf16 *__restrict__ chi,
-                                                            const __bf16 *__restrict__ clo,
-                                                            const float *__restrict__ cn, int nlist,
-                                                            const __bf16 *__restrict__ qhi,
-                                                            const __bf16 *__restrict__ qlo,
-                                                            const double *__restrict__ qn, int nq,
-                                                            float *__restrict__ out, int64_t ldo,
-                                                            float *__restrict__ gmin, int64_t ldg) {
+__global__ __launch_bounds__(NTC, 3) void coarse_approx_kernel(const __bf16 *__restrict__ chi,
+                                                              const __bf16 *__restrict__ clo,
+                                                              const float *__restrict__ cn, int nlist,
+                                                              const __bf16 *__restrict__ qhi,
+                                                              const __bf16 *__restrict__ qlo,
+                                                              const double *__restrict__ qn, int nq,
+                                                              float *__restrict__ out, int64_t ldo,
+                                                              float *__restrict__ gmin, int64_t ldg) {
+    constexpr int CC = 64 * TW;          // centroids per column block
+    constexpr int TS = 16 * TW + 4;      // staging row stride (floats)
+    constexpr int RPS = 64 / (16 * TW);  // rows per store instruction
     extern __shared__ __align__(16) float lds[];
     __bf16 *Qs = reinterpret_cast<__bf16 *>(lds);  // [CQ][CPB]: hi terms, then lo terms
     const int tid = threadIdx.x;
-    const int64_t q0 = int64_t(blockIdx.y) * CQ;
-    for (int e = tid; e < CQ * 2 * CD / 8; e += NTC) {
-        const int row = e / (2 * CD / 8), part = (e % (2 * CD / 8)) * 8;  // part < CD: hi, else lo
-        uint4 x = make_uint4(0u, 0u, 0u, 0u);
-        if (q0 + row < nq)
-            x = *reinterpret_cast<const uint4 *>((part < CD ? qhi : qlo) + (q0 + row) * CD + (part % CD));
-        *reinterpret_cast<uint4 *>(Qs + row * CPB + part) = x;
-    }
     const int lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 4, i16 = lane & 15;
-    const int c0 = blockIdx.x * (64 * TW) + wave * (16 * TW);
-    int64_t crow[TW];
-#pragma unroll
-    for (int t = 0; t < TW; ++t) crow[t] = int64_t(min(c0 + 16 * t + i16, nlist - 1)) * CD + 8 * h;
+    // this wave's epilogue staging tile, after the query tile
+    float *T = lds + CQ * CPB / 2 + wave * (16 * TS);
+    const int ncb = (nlist + CC - 1) / CC;
+    const int units = ((nq + CQ - 1) / CQ) * ncb;
+    const int u0 = int(int64_t(units) * blockIdx.x / gridDim.x);
+    const int u1 = int(int64_t(units) * (blockIdx.x + 1) / gridDim.x);
+    // B operands (and centroid norms) of the next k block, one k block ahead
     bf16x8 bhn[TW], bln[TW];
-#pragma unroll
-    for (int t = 0; t < TW; ++t) {
-        bhn[t] = *reinterpret_cast<const bf16x8 *>(chi + crow[t]);
-        bln[t] = *reinterpret_cast<const bf16x8 *>(clo + crow[t]);
-    }
-    // the norms of this lane's 16 query rows, loaded now: a load in the
-    // epilogue would wait for the stores before it (one counter for both)
-    float qf[4][4];
-#pragma unroll
-    for (int qt = 0; qt < 4; ++qt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) qf[qt][r] = float(qn[min(q0 + 16 * qt + 4 * h + r, int64_t(nq) - 1)]);
-    f32x4 acc[4][TW];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int t = 0; t < TW; ++t) acc[a][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    __syncthreads();
-#pragma unroll 1
-    for (int k0 = 0; k0 < CD; k0 += 32) {
-        bf16x8 bh[TW], bl[TW];
+    float cnn[TW];
+    auto fetch = [&](int u, int k0) {
+        const int cbn = u % ncb;
 #pragma unroll
         for (int t = 0; t < TW; ++t) {
-            bh[t] = bhn[t];
-            bl[t] = bln[t];
+            const int c = cbn * CC + wave * (16 * TW) + 16 * t + i16;
+            const int64_t off = int64_t(min(c, nlist - 1)) * CD + 8 * h + k0;
+            bhn[t] = *reinterpret_cast<const bf16x8 *>(chi + off);
+            bln[t] = *reinterpret_cast<const bf16x8 *>(clo + off);
+            if (k0 == 0) cnn[t] = c < nlist ? cn[c] : 0.0f;
         }
-        if (k0 + 32 < CD) {
+    };
+    if (u0 < u1) fetch(u0, 0);
+    int qtile = -1;
+    float qf[4][4];
+    for (int u = u0; u < u1; ++u) {
+        const int cb = u % ncb;
+        if (u / ncb != qtile) {  // uniform: this unit's query tile
+            if (qtile >= 0) __syncthreads();  // every wave is done with the previous tile
+            qtile = u / ncb;
+            const int64_t q0 = int64_t(qtile) * CQ;
+            for (int e = tid; e < CQ * 2 * CD / 8; e += NTC) {
+                const int row = e / (2 * CD / 8), part = (e % (2 * CD / 8)) * 8;  // part < CD: hi, else lo
+                uint4 x = make_uint4(0u, 0u, 0u, 0u);
+                if (q0 + row < nq)
+                    x = *reinterpret_cast<const uint4 *>((part < CD ? qhi : qlo) + (q0 + row) * CD + (part % CD));
+                *reinterpret_cast<uint4 *>(Qs + row * CPB + part) = x;
+            }
+            // the norms of this lane's 16 query rows
+#pragma unroll
+            for (int qt = 0; qt < 4; ++qt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) qf[qt][r] = float(qn[min(q0 + 16 * qt + 4 * h + r, int64_t(nq) - 1)]);
+            __syncthreads();
+        }
+        const int64_t q0 = int64_t(qtile) * CQ;
+        const int c0 = cb * CC + wave * (16 * TW);
+        float cnc[TW];
+#pragma unroll
+        for (int t = 0; t < TW; ++t) cnc[t] = cnn[t];
+        f32x4 acc[4][TW];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int t = 0; t < TW; ++t) acc[a][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int k0 = 0; k0 < CD; k0 += 32) {
+            bf16x8 bh[TW], bl[TW];
 #pragma unroll
             for (int t = 0; t < TW; ++t) {
-                bhn[t] = *reinterpret_cast<const bf16x8 *>(chi + crow[t] + k0 + 32);
-                bln[t] = *reinterpret_cast<const bf16x8 *>(clo + crow[t] + k0 + 32);
+                bh[t] = bhn[t];
+                bl[t] = bln[t];
+            }
+            // the next k block: this unit's next one, or the next unit's first
+            if (k0 + 32 < CD)
+                fetch(u, k0 + 32);
+            else if (u + 1 < u1)
+                fetch(u + 1, 0);
+#pragma unroll
+            for (int qt = 0; qt < 4; ++qt) {
+                const __bf16 *qr = Qs + (16 * qt + i16) * CPB + k0 + 8 * h;
+                const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(qr);
+                const bf16x8 al = *reinterpret_cast<const bf16x8 *>(qr + CD);
+#pragma unroll
+                for (int t = 0; t < TW; ++t) {
+                    acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[qt][t], 0, 0, 0);
+                    acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[qt][t], 0, 0, 0);
+                    acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[qt][t], 0, 0, 0);
+                }
             }
         }
+        // accumulator r of lane l: query row 16 qt + 4h + r, centroid column
+        // 16 t + l % 16.  Each 16 x 16 TW block of the wave goes through LDS so
+        // that a store writes whole row pieces (16 TW centroids, 64 TW
+        // contiguous bytes; with TW = 2 the two halves of the wave store two
+        // rows).  Row stride TS = 16 TW + 4 floats puts the four rows a write
+        // touches (h = 0..3) in distinct banks.
+        const int sc = lane % (16 * TW), sr = lane / (16 * TW);  // this lane's column and row within a store
+        const int cl = c0 + sc;
 #pragma unroll
         for (int qt = 0; qt < 4; ++qt) {
-            const __bf16 *qr = Qs + (16 * qt + i16) * CPB + k0 + 8 * h;
-            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(qr);
-            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(qr + CD);
 #pragma unroll
-            for (int t = 0; t < TW; ++t) {
-                acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[qt][t], 0, 0, 0);
-                acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[qt][t], 0, 0, 0);
-                acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[qt][t], 0, 0, 0);
+            for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                for (int t = 0; t < TW; ++t)
+                    T[(4 * h + r) * TS + 16 * t + i16] = (qf[qt][r] + cnc[t]) - 2.0f * acc[qt][t][r];
             }
-        }
-    }
-    // accumulator r of lane l: query row 16 qt + 4h + r, centroid column 16 t + l % 16.
-    // Each 16 x 16 TW block of the wave goes through LDS so that a store
-    // writes whole row pieces (16 TW centroids, 64 TW contiguous bytes; with
-    // TW = 2 the two halves of the wave store two rows).  Row stride
-    // TS = 16 TW + 4 floats puts the four rows a write touches (h = 0..3) in
-    // distinct banks.
-    constexpr int TS = 16 * TW + 4;
-    constexpr int RPS = 64 / (16 * TW);  // rows per store instruction
-    __syncthreads();  // every wave is done with the query tile
-    float *T = lds + wave * (16 * TS);
-    float cnc[TW];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (cl < nlist) {
 #pragma unroll
-    for (int t = 0; t < TW; ++t) {
-        const int c = c0 + 16 * t + i16;
-        cnc[t] = c < nlist ? cn[c] : 0.0f;
-    }
-    const int sc = lane % (16 * TW), sr = lane / (16 * TW);  // this lane's column and row within a store
-    const int cl = c0 + sc;
-#pragma unroll
-    for (int qt = 0; qt < 4; ++qt) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-#pragma unroll
-            for (int t = 0; t < TW; ++t) T[(4 * h + r) * TS + 16 * t + i16] = (qf[qt][r] + cnc[t]) - 2.0f * acc[qt][t][r];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (cl < nlist) {
-#pragma unroll
-            for (int row = 0; row < 16; row += RPS) {
-                const int64_t q = q0 + 16 * qt + row + sr;
-                if (q < nq) out[q * ldo + cl] = T[(row + sr) * TS + sc];
-            }
-        }
-        if (gmin && lane < 16 * TW) {
-            // the minimum of D' over the 16 centroids of one column tile
-            // (group (c0 + 16 g) / 16) for one query row
-            const int row = lane / TW, g = lane % TW;
-            const int cg = c0 + 16 * g;
-            const int64_t q = q0 + 16 * qt + row;
-            if (q < nq && cg < nlist) {
-                const float4 *tr = reinterpret_cast<const float4 *>(T + row * TS + 16 * g);
-                float mv = __builtin_huge_valf();
-#pragma unroll
-                for (int j4 = 0; j4 < 4; ++j4) {
-                    const float4 x = tr[j4];
-                    const float xs[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (cg + 4 * j4 + u < nlist) mv = fminf(mv, xs[u]);
+                for (int row = 0; row < 16; row += RPS) {
+                    const int64_t q = q0 + 16 * qt + row + sr;
+                    if (q < nq) out[q * ldo + cl] = T[(row + sr) * TS + sc];
                 }
-                gmin[q * ldg + cg / 16] = mv;
             }
+            if (gmin && lane < 16 * TW) {
+                // the minimum of D' over the 16 centroids of one column tile
+                // (group (c0 + 16 g) / 16) for one query row
+                const int row = lane / TW, g = lane % TW;
+                const int cg = c0 + 16 * g;
+                const int64_t q = q0 + 16 * qt + row;
+                if (q < nq && cg < nlist) {
+                    const float4 *tr = reinterpret_cast<const float4 *>(T + row * TS + 16 * g);
+                    float mv = __builtin_huge_valf();
+#pragma unroll
+                    for (int j4 = 0; j4 < 4; ++j4) {
+                        const float4 x = tr[j4];
+                        const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            if (cg + 4 * j4 + v < nlist) mv = fminf(mv, xs[v]);
+                    }
+                    gmin[q * ldg + cg / 16] = mv;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -765,9 +785,24 @@ hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *c
                                 const void *qlo, const double *qn, int nq, float *approx, int64_t lda, float *gmin,
                                 int64_t ldg, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
-    constexpr int CC = 64 * CTW;  // centroids per tile
-    dim3 grid(unsigned((nlist + CC - 1) / CC), unsigned((nq + CQ - 1) / CQ));
-    coarse_approx_kernel<CTW><<<grid, NTC, CQ * CPB * 2, s>>>(
+    constexpr int CC = 64 * CTW;  // centroids per column block
+    const int64_t units = int64_t((nq + CQ - 1) / CQ) * ((nlist + CC - 1) / CC);
+    if (units >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+    // three workgroups per CU (their LDS: query tile + staging tiles)
+    static std::atomic<int> cus[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    int ncu = cus[dev].load(std::memory_order_relaxed);
+    if (!ncu) {
+        e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        cus[dev].store(ncu, std::memory_order_relaxed);
+    }
+    const unsigned grid = unsigned(std::min<int64_t>(units, int64_t(3) * std::max(ncu, 1)));
+    const size_t lds = size_t(CQ) * CPB * 2 + 4 * 16 * (16 * CTW + 4) * 4;
+    coarse_approx_kernel<CTW><<<grid, NTC, lds, s>>>(
         static_cast<const __bf16 *>(chi), static_cast<const __bf16 *>(clo), cn, nlist,
         static_cast<const __bf16 *>(qhi), static_cast<const __bf16 *>(qlo), qn, nq, approx, lda, gmin, ldg);
     return hipGetLastError();
