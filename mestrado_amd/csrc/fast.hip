@@ -892,10 +892,12 @@ constexpr int SCAP = 8192;  // seed keys held in LDS (at most)
 
 // SC: keys held in LDS.  A capped seed runs with the smallest SC that holds
 // the cap, so that more blocks share a CU; with SC <= 2048 the launch bound
-// (6 waves per SIMD, at most 80 VGPRs) lets six of them run at once (larger
-// key buffers hold the CU to four blocks or fewer through their LDS).
+// (5 waves per SIMD, at most 102 VGPRs) lets five of them run at once and
+// leaves LDS for the (query, list) grouping kernels that run beside the seed
+// on a second stream (larger key buffers hold the CU to four blocks or fewer
+// through their LDS).
 template <int M, int SC>
-__global__ __launch_bounds__(256, (SC <= 2048 ? 6 : 4)) void seed_tau_kernel(const float *__restrict__ queries,
+__global__ __launch_bounds__(256, (SC <= 2048 ? 5 : 4)) void seed_tau_kernel(const float *__restrict__ queries,
                                                        const float *__restrict__ coarse,
                                                        const float *__restrict__ pq, const float *__restrict__ pnorm,
                                                        const float *__restrict__ pmax,
@@ -937,15 +939,22 @@ __global__ __launch_bounds__(256, (SC <= 2048 ? 6 : 4)) void seed_tau_kernel(con
             const int l = probes[q * w + rr];
             if (l < 0 || l >= nlist) continue;  // uniform
             if (off >= tot) break;  // uniform: the cap is reached
-            marg = fmaxf(marg, seed_table<M>(queries, coarse, pq, pnorm, pmax, q, l, ks, tab, r, As));
             const int64_t beg = list_off[l];
             const int len = int(min(list_off[l + 1] - beg, tot - off));
             const uint8_t *lc = codes + beg * M;
-            // four vectors per thread per round: their code loads are issued together
-            for (int v0 = tid; v0 < len; v0 += 1024) {
-                uint32_t cw[4][M / 4];
+            // four vectors per thread per round, their code loads issued
+            // together; the first round's go out before the table is built
+            uint32_t cw[4][M / 4];
+            if (len > 0) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) load_codes<M>(lc, min(v0 + 256 * u, len - 1), cw[u]);
+                for (int u = 0; u < 4; ++u) load_codes<M>(lc, min(tid + 256 * u, len - 1), cw[u]);
+            }
+            marg = fmaxf(marg, seed_table<M>(queries, coarse, pq, pnorm, pmax, q, l, ks, tab, r, As));
+            for (int v0 = tid; v0 < len; v0 += 1024) {
+                if (v0 != tid) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) load_codes<M>(lc, min(v0 + 256 * u, len - 1), cw[u]);
+                }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     float acc = 0.0f;
@@ -1054,11 +1063,12 @@ __device__ void select_shortlist(int kl, const uint64_t *__restrict__ cand, cons
     const int tid = threadIdx.x;
     const uint64_t *C = cand + int64_t(q) * capq;
     // up to 2048 candidates are read once, into registers (more are re-read);
-    // the loads go out together with the count's, entries past it are dropped
-    constexpr int FR = 8;
+    // the first 1024 loads go out together with the count's (entries past
+    // it are dropped), the rest once the count says they exist
+    constexpr int FR = 8, FS = 4;
     uint64_t rk[FR];
 #pragma unroll
-    for (int e = 0; e < FR; ++e) {
+    for (int e = 0; e < FS; ++e) {
         const int i = tid + 256 * e;
         rk[e] = i < capq ? C[i] : kNoKey;
     }
@@ -1069,8 +1079,11 @@ __device__ void select_shortlist(int kl, const uint64_t *__restrict__ cand, cons
     }
     const bool inreg = n <= 256 * FR;  // uniform
 #pragma unroll
-    for (int e = 0; e < FR; ++e)
-        if (!inreg || tid + 256 * e >= n) rk[e] = kNoKey;
+    for (int e = 0; e < FR; ++e) {
+        const int i = tid + 256 * e;
+        if (!inreg || i >= n) rk[e] = kNoKey;
+        else if (e >= FS) rk[e] = C[i];
+    }
     uint32_t mn = 0xFFFFFFFFu, mx = 0u;
     if (inreg) {
 #pragma unroll
