@@ -129,31 +129,63 @@ __device__ void flush_pool(const FastArgs &a, unsigned char *smem, int64_t beg, 
     if (a.runthr) {
         // Running bound per query: add this flush's counts to the query's
         // global histogram of candidate distances (bin b holds distances in
-        // [b, b + 1) tau / NBH), then the first bin whose cumulative count
-        // reaches kl bounds the kl-th smallest distance emitted so far.
-        // Counts read back may miss concurrent additions; a smaller count only
-        // gives a larger (still valid) bound.
+        // [b, b + 1) tau / NBH).  Nothing is read back here: the work items
+        // that start later derive their bounds from the histograms
+        // (item_bounds), so the flush never waits on these atomics.
         for (int i = tid; i < FG * NBH; i += FNT) {
             const int c = lhist[i];
             if (c > 0) atomicAdd(a.rhist + int64_t(qid[i / NBH]) * NBH + (i % NBH), c);
         }
-        const int lane = tid & 63;
-        for (int g = tid >> 6; g < FG; g += FNT / 64) {  // one wavefront per slot
-            if (!(hsc[g] > 0.0f) || gcnt[g] == 0) continue;  // wave-uniform
-            const int q = qid[g];
-            int c = lane < NBH - 1 ? __hip_atomic_load(a.rhist + int64_t(q) * NBH + lane, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT) : 0;
+    }
+}
+
+// Slots per wavefront of the item bounds: each wavefront reads the
+// histograms of FG / (FNT / 64) query slots.
+constexpr int HSW = FG / (FNT / 64);
+
+// Running bounds of a work item's queries: the loads go out at the item's
+// start (load_item_hist) and are used once its tables are built
+// (item_bounds), so their latency hides behind the table phase.  In a
+// query's histogram, the first bin whose cumulative count reaches kl bounds
+// the kl-th smallest candidate distance emitted so far: R = its upper edge
+// (+1 %), and a vector with dlt * sum above R (with rounding room) cannot
+// enter the shortlist.  Counts may miss concurrent additions; a smaller
+// count only gives a larger (still valid) bound.
+__device__ inline void load_item_hist(const FastArgs &a, const int *qid, int (&hc)[HSW]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int o = __shfl_up(c, off, 64);
-                if (lane >= off) c += o;
-            }
-            const uint64_t reach = __ballot(lane < NBH - 1 && c >= a.kl);
-            if (reach != 0 && lane == 0) {
-                const int b = __ffsll((unsigned long long)reach) - 1;
-                const float R = (float(b + 1) + 0.01f) / hsc[g];
-                atomicMin(reinterpret_cast<int *>(a.runthr) + q, __float_as_int(R));
-            }
+    for (int j = 0; j < HSW; ++j) {
+        const int q = qid[wave + j * (FNT / 64)];
+        hc[j] = (a.runthr && q >= 0 && lane < NBH - 1)
+                    ? __hip_atomic_load(a.rhist + int64_t(q) * NBH + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                    : 0;
+    }
+}
+
+template <int M>
+__device__ inline void item_bounds(const FastArgs &a, unsigned char *smem, int (&hc)[HSW]) {
+    using L = FLay<M>;
+    const int *qid = reinterpret_cast<const int *>(smem + L::qid);
+    const float *hsc = reinterpret_cast<const float *>(smem + L::hsc);
+    const float *delt = reinterpret_cast<const float *>(smem + L::delt);
+    uint32_t *stmp = reinterpret_cast<uint32_t *>(smem + L::stmp);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < HSW; ++j) {
+        const int g = wave + j * (FNT / 64);
+        if (qid[g] < 0 || !(hsc[g] > 0.0f)) continue;  // wave-uniform
+        int c = hc[j];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(c, off, 64);
+            if (lane >= off) c += o;
+        }
+        const uint64_t reach = __ballot(lane < NBH - 1 && c >= a.kl);
+        if (reach != 0 && lane == 0) {
+            const int b = __ffsll((unsigned long long)reach) - 1;
+            const float R = (float(b + 1) + 0.01f) / hsc[g];
+            const uint32_t lr = uint32_t(fminf(R / delt[g] * 1.0002f + 1.0f, 65534.0f)) + 1u;
+            stmp[g] = min(stmp[g], lr);
         }
     }
 }
@@ -247,14 +279,7 @@ __device__ void build_item_tables(const FastArgs &a, unsigned char *smem, const 
             if (qid[g] < 0) lim = 0u;                     // empty slot: nothing passes
             else if (!seeded) lim = 0xFFFFu;              // no seed bound: keep everything
             else lim = uint32_t(fminf(tau * invd + a.slack, 65534.0f)) + 1u;
-            if (qid[g] >= 0 && a.runthr) {
-                // running bound R: >= kl candidates of the query already have an
-                // approximate distance <= R, so a vector above R cannot enter its
-                // shortlist.  A sum passes if dlt * sum <= R (with rounding room).
-                const float R = a.runthr[qid[g]];
-                if (R < 3.0e38f) lim = min(lim, uint32_t(fminf(R * invd * 1.0002f + 1.0f, 65534.0f)) + 1u);
-            }
-            stmp[g] = lim;
+            stmp[g] = lim;  // lowered by the running bound after the tables (item_bounds)
             hsc[g] = (qid[g] >= 0 && seeded && tau > 0.0f && a.runthr) ? float(NBH) / tau : 0.0f;
         }
     }
@@ -394,7 +419,11 @@ __global__ __launch_bounds__(FNT, 4) void fast_scan_kernel(FastArgs a) {  // 4 w
         if (tid < FG) qid[tid] = rec->q[tid];
         if (tid < 3) pcnt[tid] = 0;
         __syncthreads();
+        int hc[HSW];
+        load_item_hist(a, qid, hc);
         build_item_tables<M>(a, smem, in, a.tau ? nullptr : rec->tau);
+        item_bounds<M>(a, smem, hc);
+        lds_barrier();  // the limits are final
 
         // prefetch of the next item: 0 index wanted, 1 record in registers,
         // 2 record in LDS, 3 inputs in registers (stage is uniform)
