@@ -26,7 +26,7 @@ struct XCfg {
 
 struct XWs {
     Ws ws;
-    DevBuf wq, kq, ns, tau, cnt, poff, ioff, fill, pairq, items, ctrl, cand, ccnt, ovf, runthr, rhist;
+    DevBuf wq, kq, tau, cnt, poff, ioff, fill, pairq, items, ctrl, cand, ccnt, ovf, runthr, rhist;
     // side stream of the pair grouping (runs beside the seed scan)
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
@@ -90,7 +90,6 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     MIVF_TRY(x.ws.dmat.ensure(coarse_ws_bytes(nq, nlist)));
     MIVF_TRY(x.ws.probes.ensure(sizeof(int32_t) * std::max<int64_t>(np, 1)));
     MIVF_TRY(x.ws.coadis.ensure(sizeof(float) * std::max<int64_t>(np, 1)));
-    MIVF_TRY(x.ns.ensure(sizeof(int32_t) * nq));
     MIVF_TRY(x.tau.ensure(sizeof(float) * nq));
     MIVF_TRY(x.cnt.ensure(sizeof(int) * (nlist + 1)));
     MIVF_TRY(x.poff.ensure(sizeof(int) * (nlist + 1)));
@@ -144,8 +143,6 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
                                 ix->item_order ? ix->lorder.as<int32_t>() : nullptr, x.fill.as<int>(),
                                 x.pairq.as<int32_t>(), x.items.as<FastItem>(), sp));
     if (overlap) MIVF_CHECK_HIP(hipEventRecord(x.join, sp));
-    MIVF_CHECK_HIP(launch_seed_count(probes, nq, wmax, ix->off.as<int64_t>(), nlist, kmax, x.kq.as<int32_t>(), 1,
-                                     x.ns.as<int32_t>(), st));
     SeedArgs sa{};
     sa.queries = d_q;
     sa.coarse = ix->coarse.as<float>();
@@ -155,7 +152,7 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     sa.list_off = ix->off.as<int64_t>();
     sa.codes = ix->codes.as<uint8_t>();
     sa.probes = probes;
-    sa.nseed = x.ns.as<int32_t>();
+    sa.min_seed = 1;
     sa.klq = x.kq.as<int32_t>();
     sa.nq = nq;
     sa.w = wmax;
@@ -352,7 +349,7 @@ int search_list_major(mivf_index *ix, const float *d_q, int nq, int w, int k, in
 std::shared_ptr<void> new_list_major_ws(hipStream_t st) {
     auto x = std::make_shared<XWs>();
     x->ws.order_on(st);
-    for (DevBuf *b : {&x->runthr, &x->rhist, &x->wq, &x->kq, &x->ns, &x->tau, &x->cnt, &x->poff, &x->ioff, &x->fill, &x->pairq, &x->items,
+    for (DevBuf *b : {&x->runthr, &x->rhist, &x->wq, &x->kq, &x->tau, &x->cnt, &x->poff, &x->ioff, &x->fill, &x->pairq, &x->items,
                       &x->ctrl, &x->cand, &x->ccnt, &x->ovf})
         b->ord = st;
     return x;

@@ -856,7 +856,7 @@ hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxc
             MIVF_STREAM_SELECT(8);
 #undef MIVF_STREAM_SELECT
         // the few queries sent back: a small grid walks the list
-        coarse_select_kernel<<<unsigned(std::min(nq, 1024)), NTC, coarse_select_lds(nlist), s>>>(
+        coarse_select_kernel<<<unsigned(std::min(nq, 256)), NTC, coarse_select_lds(nlist), s>>>(
             coarse, cn, maxcn, nlist, queries, qn, approx, lda, w, probes, pdis, redo, redo_cnt);
     } else {
         coarse_select_kernel<<<unsigned(nq), NTC, coarse_select_lds(nlist), s>>>(

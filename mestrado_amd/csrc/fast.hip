@@ -619,22 +619,6 @@ __global__ __launch_bounds__(FNT) void fast_tables_kernel(FastArgs a, int l, int
 }
 
 // ---- seed lists per query
-__global__ void seed_count_kernel(const int32_t *__restrict__ probes, int nq, int w,
-                                  const int64_t *__restrict__ list_off, int nlist, int kl0,
-                                  const int32_t *__restrict__ klq, int min_seed, int32_t *__restrict__ ns) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
-    const int kl = klq ? klq[q] : kl0;
-    int64_t have = 0;
-    int r = 0;
-    for (; r < w; ++r) {
-        if (r >= min_seed && have >= kl) break;
-        const int l = probes[int64_t(q) * w + r];
-        if (l >= 0 && l < nlist) have += list_off[l + 1] - list_off[l];
-    }
-    ns[q] = r;
-}
-
 __global__ void fill_i32_kernel(int *__restrict__ p, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -933,7 +917,7 @@ __global__ __launch_bounds__(256, (SC <= 2048 ? 5 : 4)) void seed_tau_kernel(con
                                                        const int64_t *__restrict__ list_off,
                                                        const uint8_t *__restrict__ codes,
                                                        const int32_t *__restrict__ probes, int w,
-                                                       const int32_t *__restrict__ nseed, int nlist, int ks, int kl0,
+                                                       int min_seed, int nlist, int ks, int kl0,
                                                        const int32_t *__restrict__ klq, int cap,
                                                        float *__restrict__ tau) {
     __shared__ float tab[M * 256];
@@ -944,8 +928,18 @@ __global__ __launch_bounds__(256, (SC <= 2048 ? 5 : 4)) void seed_tau_kernel(con
     __shared__ int sdig, sbelow;
     const int64_t q = blockIdx.x;
     const int tid = threadIdx.x;
-    const int ns = nseed[q];
     const int kl = klq ? klq[q] : kl0;
+    // seed lists: the smallest ns >= min_seed whose first ns probed lists
+    // hold >= kl vectors (or ns = w)
+    int ns = 0;
+    {
+        int64_t have = 0;
+        for (; ns < w; ++ns) {
+            if (ns >= min_seed && have >= kl) break;  // uniform
+            const int l = probes[q * w + ns];
+            if (l >= 0 && l < nlist) have += list_off[l + 1] - list_off[l];
+        }
+    }
     // with a cap, only the first cap vectors (list prefixes in probe order, a
     // sample of the lists) enter: the kl-th smallest of a subset of the
     // probed vectors still bounds the kl-th smallest of all of them
@@ -1318,12 +1312,6 @@ hipError_t launch_fast_prep(const float *pq, const float *pmax, int m, int ks, v
     return hipGetLastError();
 }
 
-hipError_t launch_seed_count(const int32_t *probes, int nq, int w, const int64_t *list_off, int nlist, int kl,
-                             const int32_t *klq, int min_seed, int32_t *ns, hipStream_t s) {
-    if (nq <= 0) return hipSuccess;
-    seed_count_kernel<<<nblk(nq, 256), 256, 0, s>>>(probes, nq, w, list_off, nlist, kl, klq, min_seed, ns);
-    return hipGetLastError();
-}
 
 hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t s) {
     if (n <= 0) return hipSuccess;
@@ -1372,7 +1360,7 @@ hipError_t launch_pairs(const int32_t *probes, int nq, int w, int rlo, int rhi, 
 template <int M, int SC>
 hipError_t launch_seed_m(const SeedArgs &a, hipStream_t s) {
     seed_tau_kernel<M, SC><<<a.nq, 256, 0, s>>>(a.queries, a.coarse, a.pq, a.pnorm, a.pmax, a.list_off, a.codes,
-                                                 a.probes, a.w, a.nseed, a.nlist, a.ks, a.kl, a.klq, a.cap, a.tau);
+                                                 a.probes, a.w, a.min_seed, a.nlist, a.ks, a.kl, a.klq, a.cap, a.tau);
     return hipGetLastError();
 }
 

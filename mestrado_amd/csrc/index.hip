@@ -943,7 +943,6 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(ix->f_ccnt.ensure(sizeof(int) * nq));
     MIVF_TRY(ix->f_rthr.ensure(sizeof(float) * nq));
     MIVF_TRY(ix->f_rhist.ensure(sizeof(int) * size_t(nq) * fast_hist_bins()));
-    MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
     MIVF_TRY(ix->f_tau.ensure(sizeof(float) * nq));
     int *ctrl = ix->f_ctrl.as<int>();  // [0] total items, [1] next item
     {
@@ -984,10 +983,8 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
                                     ix->f_ioff.as<int>(), ctrl, ctrl, lorder, ix->f_fill.as<int>(),
                                     ix->f_pairq.as<int32_t>(), ix->f_items.as<FastItem>(), sa));
     MIVF_CHECK_HIP(hipEventRecord(ix->aux_ev[1], sa));
-    // seed threshold from each query's ns[q] nearest lists (at least
-    // fast_seed, then as many as it takes to hold kl vectors)
-    MIVF_CHECK_HIP(launch_seed_count(ix->ws.probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), nlist, kl, nullptr,
-                                     std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), st));
+    // seed threshold from each query's nearest lists (at least fast_seed,
+    // then as many as it takes to hold kl vectors)
     {
         SeedArgs a{};
         a.queries = d_q;
@@ -998,7 +995,7 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
         a.list_off = ix->off.as<int64_t>();
         a.codes = ix->codes.as<uint8_t>();
         a.probes = ix->ws.probes.as<int32_t>();
-        a.nseed = ix->f_ns.as<int32_t>();
+        a.min_seed = std::min(ix->fast_seed, w);
         a.nq = nq;
         a.w = w;
         a.nlist = nlist;
@@ -1178,10 +1175,7 @@ int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
     MIVF_TRY(ix->ws.dmat.ensure(coarse_ws_bytes(nq, ix->nlist)));
     MIVF_TRY(ix->ws.probes.ensure(sizeof(int32_t) * size_t(nq) * w));
     MIVF_TRY(ix->ws.coadis.ensure(sizeof(float) * size_t(nq) * w));
-    MIVF_TRY(ix->f_ns.ensure(sizeof(int32_t) * nq));
     MIVF_TRY(coarse_stage(ix, ix->ws, d_q, nq, w, ix->ws.probes.as<int32_t>(), ix->ws.coadis.as<float>(), nullptr, nullptr));
-    MIVF_CHECK_HIP(launch_seed_count(ix->ws.probes.as<int32_t>(), nq, w, ix->off.as<int64_t>(), ix->nlist, kl,
-                                     nullptr, std::min(ix->fast_seed, w), ix->f_ns.as<int32_t>(), nullptr));
     SeedArgs a{};
     a.queries = d_q;
     a.coarse = ix->coarse.as<float>();
@@ -1191,7 +1185,7 @@ int mivf_fast_seed_tau(mivf_index *ix, const float *d_q, int nq, int w, int kl, 
     a.list_off = ix->off.as<int64_t>();
     a.codes = ix->codes.as<uint8_t>();
     a.probes = ix->ws.probes.as<int32_t>();
-    a.nseed = ix->f_ns.as<int32_t>();
+    a.min_seed = std::min(ix->fast_seed, w);
     a.nq = nq;
     a.w = w;
     a.nlist = ix->nlist;

@@ -135,7 +135,7 @@ struct mivf_index {
     DevBuf lorder;           // [nlist] lists by decreasing length (set by upload_lists)
     // batched reduced-precision form scratch
     DevBuf f_tau, f_cnt, f_poff, f_ioff, f_fill, f_pairq, f_items, f_ctrl, f_cand, f_ccnt,
-        f_short, f_ns, f_rthr, f_rhist;
+        f_short, f_rthr, f_rhist;
     int64_t f_last_cands = 0, f_last_overflow = 0;
     DevBuf errflag;          // kernel-reported precondition failures (int)
     bool scan_checked = false;

@@ -225,10 +225,6 @@ int fast_group_size();
 int fast_hist_bins();
 size_t fast_pb_bytes(int m);
 hipError_t launch_fast_prep(const float *pq, const float *pmax, int m, int ks, void *pb, hipStream_t s);
-// Seed lists per query: the smallest ns >= min_seed whose first ns probed
-// lists hold >= kl vectors (or ns = w).
-hipError_t launch_seed_count(const int32_t *probes, int nq, int w, const int64_t *list_off, int nlist, int kl,
-                             const int32_t *klq, int min_seed, int32_t *ns, hipStream_t s);
 hipError_t launch_fill_i32(int *p, int n, int v, hipStream_t s);
 // Up to six int fills (p[j][0..n[j]) = v[j]) in one kernel launch.
 struct FillSet {
@@ -291,7 +287,7 @@ struct SeedArgs {
     const int64_t *list_off;
     const uint8_t *codes;
     const int32_t *probes;  // [nq][w]
-    const int32_t *nseed;   // [nq] seed lists per query
+    int min_seed;           // seed lists per query: at least this many, then as many as hold kl vectors
     const int32_t *klq;     // [nq] per-query kl or null (uniform kl)
     int nq, w, nlist, ks, kl;
     int cap;                // > 0: at most cap vectors (list prefixes, in probe order) enter the seed
