@@ -112,10 +112,20 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
     }
     int32_t *probes = x.ws.probes.as<int32_t>();
     int *ctrl = x.ctrl.as<int>();  // [0] total items, [1] next item
+    MIVF_TRY(coarse_stage(ix, x.ws, d_q, nq, wmax, probes, x.ws.coadis.as<float>(), st, nullptr));
+    if (tm) MIVF_TRY(tm->mark());
+    MIVF_CHECK_HIP(launch_mask_probes(probes, nq, wmax, x.wq.as<int32_t>(), st));
+    if (overlap) MIVF_TRY(x.ensure_side());
+    hipStream_t sp = overlap ? x.side : st;  // stream of the pair grouping
+    if (overlap) {
+        MIVF_CHECK_HIP(hipEventRecord(x.fork, st));
+        MIVF_CHECK_HIP(hipStreamWaitEvent(sp, x.fork, 0));
+    }
     {
         // candidate counts, the work counter and the overflow flags start
-        // cleared; the stream's order puts this after every earlier search
-        // with this workspace and before the coarse stage
+        // cleared, on the grouping's stream: it follows every earlier search
+        // with this workspace (the fork comes after them), and the scan
+        // waits for it
         FillSet f{};
         f.p[0] = x.ccnt.as<int>(), f.n[0] = nq, f.v[0] = 0;
         f.p[1] = ctrl + 1, f.n[1] = 1, f.v[1] = 0;
@@ -126,17 +136,7 @@ int batched_enqueue(mivf_index *ix, XWs &x, const XCfg &cfg, const float *d_q, i
             f.p[4] = x.rhist.as<int>(), f.n[4] = int64_t(nq) * kExactBins, f.v[4] = 0;
             f.count = 5;
         }
-        MIVF_CHECK_HIP(launch_fill_multi(f, st));
-    }
-
-    MIVF_TRY(coarse_stage(ix, x.ws, d_q, nq, wmax, probes, x.ws.coadis.as<float>(), st, nullptr));
-    if (tm) MIVF_TRY(tm->mark());
-    MIVF_CHECK_HIP(launch_mask_probes(probes, nq, wmax, x.wq.as<int32_t>(), st));
-    if (overlap) MIVF_TRY(x.ensure_side());
-    hipStream_t sp = overlap ? x.side : st;  // stream of the pair grouping
-    if (overlap) {
-        MIVF_CHECK_HIP(hipEventRecord(x.fork, st));
-        MIVF_CHECK_HIP(hipStreamWaitEvent(sp, x.fork, 0));
+        MIVF_CHECK_HIP(launch_fill_multi(f, sp));
     }
     MIVF_CHECK_HIP(launch_pairs(probes, nq, wmax, 0, wmax, ix->off.as<int64_t>(), nullptr, nlist, cfg.g,
                                 x.cnt.as<int>(), x.poff.as<int>(), x.ioff.as<int>(), ctrl, nullptr,

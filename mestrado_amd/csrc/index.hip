@@ -945,18 +945,6 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     MIVF_TRY(ix->f_rhist.ensure(sizeof(int) * size_t(nq) * fast_hist_bins()));
     MIVF_TRY(ix->f_tau.ensure(sizeof(float) * nq));
     int *ctrl = ix->f_ctrl.as<int>();  // [0] total items, [1] next item
-    {
-        // candidate counts, the work counter, the running bounds (+inf) and
-        // their histograms start cleared; the stream's order puts this after
-        // every earlier search on the index and before the coarse stage
-        FillSet f{};
-        f.p[0] = ix->f_ccnt.as<int>(), f.n[0] = nq, f.v[0] = 0;
-        f.p[1] = ctrl + 1, f.n[1] = 1, f.v[1] = 0;
-        f.p[2] = ix->f_rthr.as<int>(), f.n[2] = nq, f.v[2] = 0x7f800000;
-        f.p[3] = ix->f_rhist.as<int>(), f.n[3] = int64_t(nq) * fast_hist_bins(), f.v[3] = 0;
-        f.count = 4;
-        MIVF_CHECK_HIP(launch_fill_multi(f, st));
-    }
     MIVF_TRY(coarse_stage(ix, ix->ws, d_q, nq, w, ix->ws.probes.as<int32_t>(), ix->ws.coadis.as<float>(), st, nullptr));
     MIVF_TRY(tm.mark());
     // 2. (list, query) grouping and the buffer clears on the second stream,
@@ -968,6 +956,19 @@ int mivf_search_fast(mivf_index *ix, const float *d_q, int nq, int w, int kl, in
     hipStream_t sa = ix->aux;
     MIVF_CHECK_HIP(hipEventRecord(ix->aux_ev[0], st));
     MIVF_CHECK_HIP(hipStreamWaitEvent(sa, ix->aux_ev[0], 0));
+    {
+        // candidate counts, the work counter, the running bounds (+inf) and
+        // their histograms start cleared, off the caller's stream: the fork
+        // follows every earlier search on the index, and the scan waits for
+        // the second stream
+        FillSet f{};
+        f.p[0] = ix->f_ccnt.as<int>(), f.n[0] = nq, f.v[0] = 0;
+        f.p[1] = ctrl + 1, f.n[1] = 1, f.v[1] = 0;
+        f.p[2] = ix->f_rthr.as<int>(), f.n[2] = nq, f.v[2] = 0x7f800000;
+        f.p[3] = ix->f_rhist.as<int>(), f.n[3] = int64_t(nq) * fast_hist_bins(), f.v[3] = 0;
+        f.count = 4;
+        MIVF_CHECK_HIP(launch_fill_multi(f, sa));
+    }
     // work items of each query's fast_front nearest lists come first, so that
     // their candidates set the queries' running bounds before the other
     // lists are scanned (the order changes no result)
