@@ -329,10 +329,13 @@ __global__ __launch_bounds__(NT, (xwaves<M, G, NT>())) void exact_scan_kernel(Ex
             for (int g = 0; g < G; ++g) pm |= uint32_t(acc[g] <= lm[g]) << g;
             if (v >= len) pm = 0u;
             if (__ballot(pm != 0u) != 0) {  // wave-uniform
+                // wave prefix sum of the per-lane pass counts (<= G: the bit
+                // planes of G)
+                constexpr int NPL = G <= 1 ? 1 : 32 - __builtin_clz(unsigned(G));
                 const uint32_t n = uint32_t(__popc(pm));
                 uint32_t pre = 0u, tot = 0u;
 #pragma unroll
-                for (int b = 0; b < 5; ++b) {
+                for (int b = 0; b < NPL; ++b) {
                     const uint64_t bm = __ballot((n >> b) & 1u);
                     pre += uint32_t(__builtin_amdgcn_mbcnt_hi(uint32_t(bm >> 32),
                                                               __builtin_amdgcn_mbcnt_lo(uint32_t(bm), 0u))) << b;

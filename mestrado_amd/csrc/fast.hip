@@ -518,15 +518,22 @@ __global__ __launch_bounds__(FNT, 4) void fast_scan_kernel(FastArgs a) {  // 4 w
                         pm |= uint32_t(16 * u + i < rem && acc[u][i] < limi) << (4 * u + i);
             }
             if (__ballot(pm != 0u) != 0) {  // wave-uniform
-                // wave prefix sum of the per-lane pass counts (<= 16: five bit planes)
+                // wave prefix sum of the per-lane pass counts (<= 16: five bit
+                // planes; two when no lane has more than three, the usual case)
                 const uint32_t n = uint32_t(__popc(pm));
                 uint32_t pre = 0u, tot = 0u;
-#pragma unroll
-                for (int b = 0; b < 5; ++b) {
+                auto plane = [&](int b) {
                     const uint64_t bm = __ballot((n >> b) & 1u);
                     pre += uint32_t(__builtin_amdgcn_mbcnt_hi(uint32_t(bm >> 32),
                                                               __builtin_amdgcn_mbcnt_lo(uint32_t(bm), 0u))) << b;
                     tot += uint32_t(__popcll(bm)) << b;
+                };
+                plane(0);
+                plane(1);
+                if (__ballot(n > 3u) != 0) {  // wave-uniform
+                    plane(2);
+                    plane(3);
+                    plane(4);
                 }
                 int slot = 0;
                 if (lane == 0) slot = atomicAdd(pcnt + rot, int(tot));
