@@ -257,14 +257,20 @@ __global__ void coarse_split_kernel(const float *__restrict__ coarse, int64_t n,
 // Exact reference distance of centroid c to the query in LDS.
 __device__ inline float exact_dist(const float *__restrict__ crow, const float *qv, float an, double bn) {
     float acc = 0.0f;
-#pragma unroll 4
-    for (int t = 0; t < CD; t += 4) {
-        const float4 c4 = *reinterpret_cast<const float4 *>(crow + t);
-        const float cc[4] = {c4.x, c4.y, c4.z, c4.w};
+    // the row in four batches of 32 values, each batch's loads issued together
+#pragma unroll 1
+    for (int t0 = 0; t0 < CD; t0 += 32) {
+        float4 c4[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const float p = cc[u] * qv[t + u];
-            acc = acc + p;
+        for (int j = 0; j < 8; ++j) c4[j] = *reinterpret_cast<const float4 *>(crow + t0 + 4 * j);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float cc[4] = {c4[j].x, c4[j].y, c4[j].z, c4[j].w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float p = cc[u] * qv[t0 + 4 * j + u];
+                acc = acc + p;
+            }
         }
     }
     const float c = float(bn + double(an));
@@ -600,20 +606,21 @@ __global__ __launch_bounds__(NTC) void coarse_select_group_kernel(const float *_
         return;
     }
     wave_sync();
-    // D' of the passing groups, four groups per load round, four rounds in flight
+    // D' of the passing groups, four groups per load round, twelve rounds in flight
+    constexpr int RB = 12;
     const float *row = approx + q * lda;
     int n = 0;
-    for (int g0 = 0; g0 < ngp; g0 += 16) {
-        float v[4];
-        int c[4];
+    for (int g0 = 0; g0 < ngp; g0 += 4 * RB) {
+        float v[RB];
+        int c[RB];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < RB; ++u) {
             const int gi = g0 + 4 * u + (lane >> 4);
             c[u] = gi < ngp ? 16 * int(gl[gi]) + (lane & 15) : nlist;
             v[u] = c[u] < nlist ? row[c[u]] : __builtin_huge_valf();
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < RB; ++u) {
             const bool pass = c[u] < nlist && v[u] <= lim;
             const uint64_t bal = __ballot(pass);
             const int pos = n + int(__builtin_amdgcn_mbcnt_hi(uint32_t(bal >> 32),
