@@ -291,45 +291,39 @@ __device__ void build_item_tables(const FastArgs &a, unsigned char *smem, const 
     // the [s][code][16 queries] table row.
     const int lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 4, c16 = lane & 15;
-    constexpr int TPW = 256 / 16 / (FNT / 64);  // code tiles per wave
+    // wave wv builds the tables of subquantizers wv, wv + 8, ...: all 256
+    // codes of one subquantizer in two halves of 8 code tiles, the B loads of
+    // a half issued together
+    constexpr int HT = 8;  // code tiles per half
     const _Float16 *pb = reinterpret_cast<const _Float16 *>(a.pb);
-    f16x4 bn[TPW][NMF];
-#pragma unroll
-    for (int t = 0; t < TPW; ++t)
-#pragma unroll
-        for (int f = 0; f < NMF; ++f)
-            bn[t][f] = *reinterpret_cast<const f16x4 *>(pb + (size_t((wave * TPW + t) * 16 + c16)) * KP + f * 16 + 4 * h);
-#pragma unroll 2
-    for (int s = 0; s < M; ++s) {
-        f16x4 b[TPW][NMF];
-#pragma unroll
-        for (int t = 0; t < TPW; ++t)
-#pragma unroll
-            for (int f = 0; f < NMF; ++f) b[t][f] = bn[t][f];
-        if (s + 1 < M) {
-#pragma unroll
-            for (int t = 0; t < TPW; ++t)
-#pragma unroll
-                for (int f = 0; f < NMF; ++f)
-                    bn[t][f] = *reinterpret_cast<const f16x4 *>(
-                        pb + (size_t(s + 1) * 256 + (wave * TPW + t) * 16 + c16) * KP + f * 16 + 4 * h);
-        }
+#pragma unroll 1
+    for (int s = wave; s < M; s += FNT / 64) {
         f16x4 av[NMF];
 #pragma unroll
         for (int f = 0; f < NMF; ++f)
             av[f] = *reinterpret_cast<const f16x4 *>(ao + (size_t(s) * FG + c16) * KP + f * 16 + 4 * h);
+#pragma unroll 1
+        for (int hf = 0; hf < 256 / 16 / HT; ++hf) {
+            f16x4 bt[HT][NMF];
 #pragma unroll
-        for (int t = 0; t < TPW; ++t) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int t = 0; t < HT; ++t)
 #pragma unroll
-            for (int f = 0; f < NMF; ++f) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(av[f], b[t][f], acc, 0, 0, 0);
-            uint32_t w = 0u;
+                for (int f = 0; f < NMF; ++f)
+                    bt[t][f] = *reinterpret_cast<const f16x4 *>(
+                        pb + (size_t(s) * 256 + (hf * HT + t) * 16 + c16) * KP + f * 16 + 4 * h);
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                w = __builtin_amdgcn_cvt_pk_u8_f32(__builtin_amdgcn_fmed3f(acc[r], 0.0f, 255.0f), uint32_t(r), w);
-            const int c = (wave * TPW + t) * 16 + c16;
-            // stored as signed bytes t - 128, the operand form of the i8 MFMA sums
-            *reinterpret_cast<uint32_t *>(smem + L::t8 + (size_t(s) * 256 + c) * FG + 4 * h) = w ^ 0x80808080u;
+            for (int t = 0; t < HT; ++t) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int f = 0; f < NMF; ++f) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(av[f], bt[t][f], acc, 0, 0, 0);
+                uint32_t w = 0u;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    w = __builtin_amdgcn_cvt_pk_u8_f32(__builtin_amdgcn_fmed3f(acc[r], 0.0f, 255.0f), uint32_t(r), w);
+                const int c = (hf * HT + t) * 16 + c16;
+                // stored as signed bytes t - 128, the operand form of the i8 MFMA sums
+                *reinterpret_cast<uint32_t *>(smem + L::t8 + (size_t(s) * 256 + c) * FG + 4 * h) = w ^ 0x80808080u;
+            }
         }
     }
     __syncthreads();  // tables ready; the pool may now overwrite the A operands
