@@ -179,10 +179,10 @@ __device__ void xtables(const ExactArgs &a, unsigned char *smem, int l) {
             for (int i = 0; i < DS; ++i) r[g][i] = R[(g0 + g) * D + s * DS + i];
             rn64[g] = double(rn[(g0 + g) * M + s]);
         }
-        // (loading the next code's centroid row during this code's arithmetic
-        // gains nothing here: the phase is bound by the VALU, and the copies
-        // of the prefetched row cost instructions)
-#pragma unroll 1
+        // all E codes unrolled: their centroid rows load together (a
+        // prefetch of the next code's row into copies gains nothing: the
+        // copies cost VALU instructions, which bound this phase)
+#pragma unroll
         for (int j = 0; j < E; ++j) {
             const int c = c0 + CPS * j;
             float vals[GC];
