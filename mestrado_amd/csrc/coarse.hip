@@ -94,10 +94,16 @@ __global__ __launch_bounds__(NTC) void coarse_prep_kernel(const float *__restric
     }
 }
 
-template <int TW>
+// SPLIT: the query tile is split to BF16 here, from the FP32 queries, and the
+// row norms of D' are FP32 sums over the tile (D' is approximate: an FP32
+// norm in any order is within 128 * 2^-24 |q|^2 of the exact one, far inside
+// the recheck margin); otherwise the terms and norms of coarse_prep_kernel
+// are read.
+template <int TW, bool SPLIT>
 __global__ __launch_bounds__(NTC, 3) void coarse_approx_kernel(const __bf16 *__restrict__ chi,
                                                               const __bf16 *__restrict__ clo,
                                                               const float *__restrict__ cn, int nlist,
+                                                              const float *__restrict__ queries,
                                                               const __bf16 *__restrict__ qhi,
                                                               const __bf16 *__restrict__ qlo,
                                                               const double *__restrict__ qn, int nq,
@@ -111,8 +117,10 @@ __global__ __launch_bounds__(NTC, 3) void coarse_approx_kernel(const __bf16 *__r
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 4, i16 = lane & 15;
-    // this wave's epilogue staging tile, after the query tile
+    // this wave's epilogue staging tile, after the query tile; then the
+    // tile's row norms (SPLIT)
     float *T = lds + CQ * CPB / 2 + wave * (16 * TS);
+    float *qfs = lds + CQ * CPB / 2 + 4 * 16 * TS;
     const int ncb = (nlist + CC - 1) / CC;
     const int units = ((nq + CQ - 1) / CQ) * ncb;
     const int u0 = int(int64_t(units) * blockIdx.x / gridDim.x);
@@ -140,18 +148,47 @@ __global__ __launch_bounds__(NTC, 3) void coarse_approx_kernel(const __bf16 *__r
             if (qtile >= 0) __syncthreads();  // every wave is done with the previous tile
             qtile = u / ncb;
             const int64_t q0 = int64_t(qtile) * CQ;
-            for (int e = tid; e < CQ * 2 * CD / 8; e += NTC) {
-                const int row = e / (2 * CD / 8), part = (e % (2 * CD / 8)) * 8;  // part < CD: hi, else lo
-                uint4 x = make_uint4(0u, 0u, 0u, 0u);
-                if (q0 + row < nq)
-                    x = *reinterpret_cast<const uint4 *>((part < CD ? qhi : qlo) + (q0 + row) * CD + (part % CD));
-                *reinterpret_cast<uint4 *>(Qs + row * CPB + part) = x;
+            if constexpr (SPLIT) {
+                // a row's 32 float4 are 32 consecutive lanes: one half-wave
+                // sums its squares
+                for (int e = tid; e < CQ * CD / 4; e += NTC) {
+                    const int row = e / (CD / 4), col = (e % (CD / 4)) * 4;
+                    float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (q0 + row < nq) x = *reinterpret_cast<const float4 *>(queries + (q0 + row) * CD + col);
+                    const float xs[4] = {x.x, x.y, x.z, x.w};
+                    __bf16 hh[4], ll[4];
+                    float sq = 0.0f;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        hh[u] = __bf16(xs[u]);
+                        ll[u] = __bf16(xs[u] - float(hh[u]));
+                        sq = __builtin_fmaf(xs[u], xs[u], sq);
+                    }
+                    *reinterpret_cast<uint2 *>(Qs + row * CPB + col) = *reinterpret_cast<const uint2 *>(hh);
+                    *reinterpret_cast<uint2 *>(Qs + row * CPB + CD + col) = *reinterpret_cast<const uint2 *>(ll);
+#pragma unroll
+                    for (int off = 16; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 32);
+                    if ((e & 31) == 0) qfs[row] = sq;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int qt = 0; qt < 4; ++qt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) qf[qt][r] = qfs[16 * qt + 4 * h + r];
+            } else {
+                for (int e = tid; e < CQ * 2 * CD / 8; e += NTC) {
+                    const int row = e / (2 * CD / 8), part = (e % (2 * CD / 8)) * 8;  // part < CD: hi, else lo
+                    uint4 x = make_uint4(0u, 0u, 0u, 0u);
+                    if (q0 + row < nq)
+                        x = *reinterpret_cast<const uint4 *>((part < CD ? qhi : qlo) + (q0 + row) * CD + (part % CD));
+                    *reinterpret_cast<uint4 *>(Qs + row * CPB + part) = x;
+                }
+                // the norms of this lane's 16 query rows
+#pragma unroll
+                for (int qt = 0; qt < 4; ++qt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) qf[qt][r] = float(qn[min(q0 + 16 * qt + 4 * h + r, int64_t(nq) - 1)]);
             }
-            // the norms of this lane's 16 query rows
-#pragma unroll
-            for (int qt = 0; qt < 4; ++qt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) qf[qt][r] = float(qn[min(q0 + 16 * qt + 4 * h + r, int64_t(nq) - 1)]);
             __syncthreads();
         }
         const int64_t q0 = int64_t(qtile) * CQ;
@@ -535,7 +572,8 @@ __global__ __launch_bounds__(NTC) void coarse_select_group_kernel(const float *_
                                                                   int w, int32_t *__restrict__ probes,
                                                                   float *__restrict__ pdis,
                                                                   int32_t *__restrict__ redo,
-                                                                  int *__restrict__ redo_cnt) {
+                                                                  int *__restrict__ redo_cnt,
+                                                                  double *__restrict__ qn_out) {
     __shared__ uint32_t cidx_all[NTC / 64][WCAP];
     __shared__ uint64_t keys_all[NTC / 64][WCAP];
     __shared__ float qv_all[NTC / 64][CD];
@@ -555,9 +593,28 @@ __global__ __launch_bounds__(NTC) void coarse_select_group_kernel(const float *_
         const int g = lane + 64 * j;
         gk[j] = fkey(g < ng ? gq[g] : __builtin_huge_valf());
     }
-    const double bn = qn[q];
     qv[lane] = queries[q * CD + lane];
     qv[lane + 64] = queries[q * CD + lane + 64];
+    double bn;
+    if (qn_out) {
+        // the query's norm (f32 squares summed in f64 in element order, as
+        // launch_norm_f64), for this kernel and the later ones
+        wave_sync();
+        double sum = 0.0;
+        if (lane == 0) {
+            for (int t = 0; t < CD; ++t) {
+                const float p = qv[t] * qv[t];
+                sum = sum + double(p);
+            }
+            qn_out[q] = sum;
+        }
+        const uint64_t bits = __builtin_bit_cast(uint64_t, sum);
+        const uint32_t blo = uint32_t(__shfl(int(uint32_t(bits)), 0));
+        const uint32_t bhi = uint32_t(__shfl(int(uint32_t(bits >> 32)), 0));
+        bn = __builtin_bit_cast(double, (uint64_t(bhi) << 32) | blo);
+    } else {
+        bn = qn[q];
+    }
     // An upper bound of the w-th smallest group minimum: MSB-first search for
     // the smallest key with at least w minima at or below it (integer order
     // = float order), from the first bit below those all minima share, over
@@ -788,9 +845,9 @@ hipError_t launch_coarse_prep(const float *queries, int nq, double *qn, void *qh
     return hipGetLastError();
 }
 
-hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const void *qhi,
-                                const void *qlo, const double *qn, int nq, float *approx, int64_t lda, float *gmin,
-                                int64_t ldg, hipStream_t s) {
+hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const float *queries,
+                                const void *qhi, const void *qlo, const double *qn, int nq, float *approx,
+                                int64_t lda, float *gmin, int64_t ldg, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     constexpr int CC = 64 * CTW;  // centroids per column block
     const int64_t units = int64_t((nq + CQ - 1) / CQ) * ((nlist + CC - 1) / CC);
@@ -808,19 +865,28 @@ hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *c
         cus[dev].store(ncu, std::memory_order_relaxed);
     }
     const unsigned grid = unsigned(std::min<int64_t>(units, int64_t(3) * std::max(ncu, 1)));
-    const size_t lds = size_t(CQ) * CPB * 2 + 4 * 16 * (16 * CTW + 4) * 4;
-    coarse_approx_kernel<CTW><<<grid, NTC, lds, s>>>(
-        static_cast<const __bf16 *>(chi), static_cast<const __bf16 *>(clo), cn, nlist,
-        static_cast<const __bf16 *>(qhi), static_cast<const __bf16 *>(qlo), qn, nq, approx, lda, gmin, ldg);
+    const size_t lds = size_t(CQ) * CPB * 2 + 4 * 16 * (16 * CTW + 4) * 4 + CQ * 4;
+    if (queries)
+        coarse_approx_kernel<CTW, true><<<grid, NTC, lds, s>>>(
+            static_cast<const __bf16 *>(chi), static_cast<const __bf16 *>(clo), cn, nlist, queries, nullptr,
+            nullptr, nullptr, nq, approx, lda, gmin, ldg);
+    else
+        coarse_approx_kernel<CTW, false><<<grid, NTC, lds, s>>>(
+            static_cast<const __bf16 *>(chi), static_cast<const __bf16 *>(clo), cn, nlist, nullptr,
+            static_cast<const __bf16 *>(qhi), static_cast<const __bf16 *>(qlo), qn, nq, approx, lda, gmin, ldg);
     return hipGetLastError();
 }
+
+bool coarse_group_select_ok(int nlist, int w) { return nlist >= 64 && w <= 64 && (nlist + 15) / 16 >= w; }
 
 hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxcn, int nlist, const float *queries,
                                 const double *qn, int nq, int w, const float *approx, int64_t lda,
                                 const float *gmin, int64_t ldg, int32_t *probes, float *pdis, int32_t *redo,
-                                int *redo_cnt, hipStream_t s) {
+                                int *redo_cnt, double *qn_out, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     if (nlist > kMaxMfmaLists) return hipErrorInvalidValue;
+    // the norms come from the group select only
+    if (qn_out && !(gmin && redo && redo_cnt && coarse_group_select_ok(nlist, w))) return hipErrorInvalidValue;
     static size_t lds_set = 64 * 1024;  // dynamic LDS allowed so far (64 KiB needs no attribute)
     const size_t lds = coarse_select_lds(nlist);
     if (lds > lds_set) {
@@ -839,8 +905,8 @@ hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxc
         const int ngm = (nlist + 15) / 16 <= 256 ? 4 : (nlist + 15) / 16 <= 512 ? 8 : (nlist + 15) / 16 <= 1024 ? 16 : 32;
 #define MIVF_GROUP_SELECT(G)                                                                                    \
     coarse_select_group_kernel<G><<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda,  \
-                                                        gmin, ldg, w, probes, pdis, redo, redo_cnt)
-        const bool grp = gmin && w <= 64 && (nlist + 15) / 16 >= w;
+                                                        gmin, ldg, w, probes, pdis, redo, redo_cnt, qn_out)
+        const bool grp = gmin && coarse_group_select_ok(nlist, w);
         if (grp && ngm == 4)
             MIVF_GROUP_SELECT(4);
         else if (grp && ngm == 8)

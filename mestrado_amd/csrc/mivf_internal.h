@@ -127,16 +127,23 @@ hipError_t launch_coarse_split(const float *coarse, int64_t n, void *hi, void *l
 // queries qhi, qlo [nq][128].
 hipError_t launch_coarse_prep(const float *queries, int nq, double *qn, void *qhi, void *qlo, hipStream_t s);
 // approx [nq][lda]: D'; gmin [nq][ldg] (optional, ldg >= ceil(nlist / 16)):
-// the minimum of D' over each group of 16 consecutive centroids.
-hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const void *qhi,
-                                const void *qlo, const double *qn, int nq, float *approx, int64_t lda, float *gmin,
-                                int64_t ldg, hipStream_t s);
+// the minimum of D' over each group of 16 consecutive centroids.  With
+// queries non-null the kernel splits the FP32 queries itself (no qhi, qlo,
+// qn needed); otherwise it reads launch_coarse_prep's terms and norms.
+hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const float *queries,
+                                const void *qhi, const void *qlo, const double *qn, int nq, float *approx,
+                                int64_t lda, float *gmin, int64_t ldg, hipStream_t s);
+// True when launch_coarse_select takes every query through the group-minima
+// select (nlist >= 64, w <= 64, w <= nlist / 16 groups).
+bool coarse_group_select_ok(int nlist, int w);
 // redo [nq] and redo_cnt [1]: scratch for queries whose recheck set is large;
-// gmin: the group minima of launch_coarse_approx, or null
+// gmin: the group minima of launch_coarse_approx, or null.  qn_out non-null
+// (the group select only): the select computes the query norms (as
+// launch_norm_f64) into qn_out and uses them (qn is then not read).
 hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxcn, int nlist, const float *queries,
                                 const double *qn, int nq, int w, const float *approx, int64_t lda,
                                 const float *gmin, int64_t ldg, int32_t *probes, float *pdis, int32_t *redo,
-                                int *redo_cnt, hipStream_t s);
+                                int *redo_cnt, double *qn_out, hipStream_t s);
 
 // Residuals: r[i] = x[i] - c[assign[i]]
 hipError_t launch_residual(const float *x, int64_t n, int d, const float *c,
