@@ -349,14 +349,14 @@ __global__ __launch_bounds__(NTC) void coarse_select_kernel(const float *__restr
             mn = fminf(mn, row[c]);
             mx = fmaxf(mx, row[c]);
         }
-    #pragma unroll
+#pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             mn = fminf(mn, __shfl_xor(mn, off));
             mx = fmaxf(mx, __shfl_xor(mx, off));
         }
         __shared__ float wmn[NTC / 64], wmx[NTC / 64];
         __shared__ uint32_t hist[256];
-        __shared__ int sdig, sbelow;
+        __shared__ int sdig;
         __shared__ uint32_t vmax;
         if ((tid & 63) == 0) {
             wmn[tid >> 6] = mn;
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(NTC) void coarse_select_kernel(const float *__restr
         hist[tid] = 0u;
         if (tid == 0) vmax = 0u;
         __syncthreads();
-    #pragma unroll
+#pragma unroll
         for (int i = 0; i < NTC / 64; ++i) {
             mn = fminf(mn, wmn[i]);
             mx = fmaxf(mx, wmx[i]);
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(NTC) void coarse_select_kernel(const float *__restr
             const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
             const uint32_t tot = h0 + h1 + h2 + h3;
             uint32_t incl = tot;
-    #pragma unroll
+#pragma unroll
             for (int off = 1; off < 64; off <<= 1) {
                 const uint32_t o = __shfl_up(incl, off, 64);
                 if (tid >= off) incl += o;
@@ -393,7 +393,6 @@ __global__ __launch_bounds__(NTC) void coarse_select_kernel(const float *__restr
                 else if (c + h0 + h1 + h2 >= need) { c += h0 + h1; dg = 2; }
                 else { c += h0 + h1 + h2; dg = 3; }
                 sdig = 4 * tid + dg;
-                sbelow = int(c);
             }
         }
         __syncthreads();

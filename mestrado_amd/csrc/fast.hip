@@ -911,7 +911,7 @@ constexpr int SCAP = 8192;  // seed keys held in LDS (at most)
 // on a second stream (larger key buffers hold the CU to four blocks or fewer
 // through their LDS).
 template <int M, int SC>
-__global__ __launch_bounds__(256, (SC <= 2048 ? 5 : 4)) void seed_tau_kernel(const float *__restrict__ queries,
+__global__ __launch_bounds__(256, (SC <= 2048 ? 5 : SC <= 4096 ? 4 : 3)) void seed_tau_kernel(const float *__restrict__ queries,
                                                        const float *__restrict__ coarse,
                                                        const float *__restrict__ pq, const float *__restrict__ pnorm,
                                                        const float *__restrict__ pmax,
@@ -1253,7 +1253,7 @@ __global__ __launch_bounds__(256) void fast_select_rerank_kernel(int kl, const u
     select_shortlist(kl, cand, cand_cnt, capq, q, sl, ss);
     __syncthreads();
     const int32_t e = int(threadIdx.x) < kl ? sl[threadIdx.x] : -1;
-    rerank_block<MODE, T, 256>(rows, nrows, list_ids, queries, d, qnorm, q, e, kl, k, out_ids, out_dis, rs);
+    rerank_block<MODE, T, 256>(rows, nrows, list_ids, queries, d, qnorm, q, e, k, out_ids, out_dis, rs);
 }
 
 inline unsigned nblk(int64_t n, int t) { return unsigned((n + t - 1) / t); }

@@ -96,7 +96,7 @@ struct RerankShared {
 template <int MODE, typename T, int NT>
 __device__ void rerank_block(const T *__restrict__ base, int64_t nbase, const int32_t *__restrict__ list_ids,
                              const float *__restrict__ queries, int d, const double *__restrict__ qnorm, int q,
-                             int32_t e, int kl, int k, int32_t *__restrict__ out_ids, float *__restrict__ out_dis,
+                             int32_t e, int k, int32_t *__restrict__ out_ids, float *__restrict__ out_dis,
                              RerankShared &rs) {
     static_assert(NT <= 256 && NT % 64 == 0, "one shortlist entry per thread, at most 256, whole wavefronts");
     uint64_t *cand = rs.cand;

@@ -199,8 +199,8 @@ __global__ __launch_bounds__(NT) void rerank_small_kernel(const T *__restrict__ 
                                                           float *__restrict__ out_dis) {
     __shared__ RerankShared rs;
     const int q = blockIdx.x;
-    const int32_t e = threadIdx.x < kl ? short_ids[int64_t(q) * kl + threadIdx.x] : -1;
-    rerank_block<MODE, T, NT>(base, nbase, list_ids, queries, d, qnorm, q, e, kl, k, out_ids, out_dis, rs);
+    const int32_t e = int(threadIdx.x) < kl ? short_ids[int64_t(q) * kl + threadIdx.x] : -1;
+    rerank_block<MODE, T, NT>(base, nbase, list_ids, queries, d, qnorm, q, e, k, out_ids, out_dis, rs);
 }
 
 __global__ void gather_rows_kernel(const float *__restrict__ base, int d, const int32_t *__restrict__ ids,
