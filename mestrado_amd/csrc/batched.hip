@@ -320,8 +320,13 @@ int search_list_major(mivf_index *ix, const float *d_q, int nq, int w, int k, in
     const XCfg cfg{ix->batched_g, ix->batched_nt};
     MIVF_TRY(x.wq.ensure(sizeof(int32_t) * nq));
     MIVF_TRY(x.kq.ensure(sizeof(int32_t) * nq));
-    MIVF_CHECK_HIP(launch_fill_i32(x.wq.as<int>(), nq, w, st));
-    MIVF_CHECK_HIP(launch_fill_i32(x.kq.as<int>(), nq, k, st));
+    {
+        FillSet f{};  // every query's w and k, in one launch
+        f.p[0] = x.wq.as<int>(), f.n[0] = nq, f.v[0] = w;
+        f.p[1] = x.kq.as<int>(), f.n[1] = nq, f.v[1] = k;
+        f.count = 2;
+        MIVF_CHECK_HIP(launch_fill_multi(f, st));
+    }
     Timer tm;
     std::fill(ix->timings, ix->timings + 8, 0.0f);
     MIVF_TRY(tm.start(ix->profile, st, ix->ev));

@@ -108,7 +108,8 @@ __global__ __launch_bounds__(NTC, 3) void coarse_approx_kernel(const __bf16 *__r
                                                               const __bf16 *__restrict__ qlo,
                                                               const double *__restrict__ qn, int nq,
                                                               float *__restrict__ out, int64_t ldo,
-                                                              float *__restrict__ gmin, int64_t ldg) {
+                                                              float *__restrict__ gmin, int64_t ldg,
+                                                              int *__restrict__ zero) {
     constexpr int CC = 64 * TW;          // centroids per column block
     constexpr int TS = 16 * TW + 4;      // staging row stride (floats)
     constexpr int RPS = 64 / (16 * TW);  // rows per store instruction
@@ -121,6 +122,7 @@ __global__ __launch_bounds__(NTC, 3) void coarse_approx_kernel(const __bf16 *__r
     // tile's row norms (SPLIT)
     float *T = lds + CQ * CPB / 2 + wave * (16 * TS);
     float *qfs = lds + CQ * CPB / 2 + 4 * 16 * TS;
+    if (zero && blockIdx.x == 0 && tid == 0) *zero = 0;  // the select's redo counter, for the next kernel
     const int ncb = (nlist + CC - 1) / CC;
     const int units = ((nq + CQ - 1) / CQ) * ncb;
     const int u0 = int(int64_t(units) * blockIdx.x / gridDim.x);
@@ -846,7 +848,7 @@ hipError_t launch_coarse_prep(const float *queries, int nq, double *qn, void *qh
 
 hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const float *queries,
                                 const void *qhi, const void *qlo, const double *qn, int nq, float *approx,
-                                int64_t lda, float *gmin, int64_t ldg, hipStream_t s) {
+                                int64_t lda, float *gmin, int64_t ldg, int *zero, hipStream_t s) {
     if (nq <= 0) return hipSuccess;
     constexpr int CC = 64 * CTW;  // centroids per column block
     const int64_t units = int64_t((nq + CQ - 1) / CQ) * ((nlist + CC - 1) / CC);
@@ -868,11 +870,12 @@ hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *c
     if (queries)
         coarse_approx_kernel<CTW, true><<<grid, NTC, lds, s>>>(
             static_cast<const __bf16 *>(chi), static_cast<const __bf16 *>(clo), cn, nlist, queries, nullptr,
-            nullptr, nullptr, nq, approx, lda, gmin, ldg);
+            nullptr, nullptr, nq, approx, lda, gmin, ldg, zero);
     else
         coarse_approx_kernel<CTW, false><<<grid, NTC, lds, s>>>(
             static_cast<const __bf16 *>(chi), static_cast<const __bf16 *>(clo), cn, nlist, nullptr,
-            static_cast<const __bf16 *>(qhi), static_cast<const __bf16 *>(qlo), qn, nq, approx, lda, gmin, ldg);
+            static_cast<const __bf16 *>(qhi), static_cast<const __bf16 *>(qlo), qn, nq, approx, lda, gmin, ldg,
+            zero);
     return hipGetLastError();
 }
 
@@ -896,8 +899,6 @@ hipError_t launch_coarse_select(const float *coarse, const float *cn, float maxc
     }
     const unsigned wgrid = unsigned((nq + NTC / 64 - 1) / (NTC / 64));
     if (nlist >= 64 && w <= 128 && redo && redo_cnt) {
-        hipError_t e = launch_fill_i32(redo_cnt, 1, 0, s);  // a kernel, not a memset: graph-capturable
-        if (e != hipSuccess) return e;
 #define MIVF_STREAM_SELECT(G)                                                                                   \
     coarse_select_stream_kernel<G><<<wgrid, NTC, 0, s>>>(coarse, cn, maxcn, nlist, queries, qn, nq, approx, lda, w, \
                                                          probes, pdis, redo, redo_cnt)

@@ -399,13 +399,13 @@ int coarse_stage(mivf_index *ix, Ws &ws, const float *d_q, int nq, int w, int32_
         // with the group select, the distance kernel splits the queries and
         // the select writes their norms: no separate prep kernel
         const bool fused = coarse_group_select_ok(nlist, w);
+        MIVF_TRY(ws.redo.ensure(sizeof(int32_t) * (size_t(nq) + 1)));
+        int32_t *redo = ws.redo.as<int32_t>();  // [0] count (cleared by the distance kernel), then the queries
         if (!fused) MIVF_CHECK_HIP(launch_coarse_prep(d_q, nq, ws.qn.as<double>(), wb + lay.qhi, wb + lay.qlo, st));
         MIVF_CHECK_HIP(launch_coarse_approx(ix->coarse_hi.p, ix->coarse_lo.p, ix->coarse_n32.as<float>(), nlist,
                                             fused ? d_q : nullptr, wb + lay.qhi, wb + lay.qlo, ws.qn.as<double>(), nq,
-                                            ws.dmat.as<float>(), nlist, gmin, ldg, st));
+                                            ws.dmat.as<float>(), nlist, gmin, ldg, redo, st));
         if (tm) MIVF_TRY(tm->mark());
-        MIVF_TRY(ws.redo.ensure(sizeof(int32_t) * (size_t(nq) + 1)));
-        int32_t *redo = ws.redo.as<int32_t>();
         MIVF_CHECK_HIP(launch_coarse_select(ix->coarse.as<float>(), ix->coarse_n32.as<float>(), ix->maxcn, nlist,
                                             d_q, ws.qn.as<double>(), nq, w, ws.dmat.as<float>(), nlist, gmin, ldg,
                                             probes, dis, redo + 1, redo, fused ? ws.qn.as<double>() : nullptr, st));

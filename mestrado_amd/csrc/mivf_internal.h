@@ -130,13 +130,15 @@ hipError_t launch_coarse_prep(const float *queries, int nq, double *qn, void *qh
 // the minimum of D' over each group of 16 consecutive centroids.  With
 // queries non-null the kernel splits the FP32 queries itself (no qhi, qlo,
 // qn needed); otherwise it reads launch_coarse_prep's terms and norms.
+// zero (or null): set to 0 by the kernel: launch_coarse_select's redo_cnt.
 hipError_t launch_coarse_approx(const void *chi, const void *clo, const float *cn, int nlist, const float *queries,
                                 const void *qhi, const void *qlo, const double *qn, int nq, float *approx,
-                                int64_t lda, float *gmin, int64_t ldg, hipStream_t s);
+                                int64_t lda, float *gmin, int64_t ldg, int *zero, hipStream_t s);
 // True when launch_coarse_select takes every query through the group-minima
 // select (nlist >= 64, w <= 64, w <= nlist / 16 groups).
 bool coarse_group_select_ok(int nlist, int w);
-// redo [nq] and redo_cnt [1]: scratch for queries whose recheck set is large;
+// redo [nq] and redo_cnt [1]: scratch for queries whose recheck set is large
+// (redo_cnt cleared beforehand, by launch_coarse_approx);
 // gmin: the group minima of launch_coarse_approx, or null.  qn_out non-null
 // (the group select only): the select computes the query norms (as
 // launch_norm_f64) into qn_out and uses them (qn is then not read).
