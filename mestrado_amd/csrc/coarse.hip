@@ -248,11 +248,20 @@ __global__ __launch_bounds__(NTC, 3) void coarse_approx_kernel(const __bf16 *__r
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (cl < nlist) {
+            // the tile's values are read from LDS together, then stored
+            // (a whole tile without per-row checks)
+            float ov[16 / RPS];
 #pragma unroll
-                for (int row = 0; row < 16; row += RPS) {
-                    const int64_t q = q0 + 16 * qt + row + sr;
-                    if (q < nq) out[q * ldo + cl] = T[(row + sr) * TS + sc];
+            for (int i = 0; i < 16 / RPS; ++i) ov[i] = T[(RPS * i + sr) * TS + sc];
+            const int64_t qr = q0 + 16 * qt + sr;
+            if (cl < nlist) {
+                if (q0 + 16 * qt + 16 <= nq) {  // uniform
+#pragma unroll
+                    for (int i = 0; i < 16 / RPS; ++i) out[(qr + RPS * i) * ldo + cl] = ov[i];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 16 / RPS; ++i)
+                        if (qr + RPS * i < nq) out[(qr + RPS * i) * ldo + cl] = ov[i];
                 }
             }
             if (gmin && lane < 16 * TW) {
